@@ -1,0 +1,194 @@
+"""Builders of the benchmark problems of the transport path.
+
+Each builder returns (model, nuclides) and can write itself into a directory
+(model.xml + cross_sections.xml + nuclide files) with ``write_problem``.
+Sizes default to the full benchmark definitions; tests shrink them through
+the keyword arguments.  Nuclide data are drawn from fixed seeds
+(``openmc_amd.data.synthetic_nuclide``), so a problem is reproducible from
+its arguments.
+
+Choices the benchmark text leaves open (recorded here and in the design notes):
+  * atomic weight ratios ~ U[2, 240] from each nuclide's seed; T = 294 K;
+  * config 0 uses the atom densities themselves ("sum" density units); configs
+    1-2 draw atom fractions ~ U[0.005, 0.05] and scale each material to
+    0.1 atom/b-cm so that 20 or 300 nuclides give a realistic total density;
+  * config 1's source is a Watt spectrum sampled uniformly in the fuel
+    cylinders: a box over the lattice with a cell-domain constraint;
+  * config 2's source is a Watt spectrum sampled uniformly in the cube.
+"""
+import hashlib
+import json
+import os
+import tempfile
+
+import numpy as np
+
+from . import data, stats
+from . import model as om
+
+DATA_SEED_BASE = 1000
+
+
+def _nuclide_set(prefix, count, seed, grid_lo, grid_hi, n_res=30, pts_per_res=40):
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(grid_lo, grid_hi + 1, size=count)
+    return [data.synthetic_nuclide("%s%03d" % (prefix, k), seed * 1000 + k, n_grid=int(sizes[k]),
+                                   n_res=n_res, pts_per_res=pts_per_res) for k in range(count)]
+
+
+def config0(n_particles=100000, n_batches=10, n_grid=10000, mesh=(10, 10, 10), event_based=True):
+    """20 cm cube, one material of 4 nuclides, 1 MeV point source, 10^3 flux mesh."""
+    om.reset_auto_ids()
+    nucs = [data.synthetic_nuclide("N%d" % k, DATA_SEED_BASE + k, n_grid=n_grid) for k in range(4)]
+    rng = np.random.default_rng(DATA_SEED_BASE)
+    mat = om.Material(name="mix")
+    for n in nucs:
+        mat.add_nuclide(n.name, float(rng.uniform(0.005, 0.05)))
+    mat.set_density("sum")
+    planes = [om.XPlane(-10.0, boundary_type="vacuum"), om.XPlane(10.0, boundary_type="vacuum"),
+              om.YPlane(-10.0, boundary_type="vacuum"), om.YPlane(10.0, boundary_type="vacuum"),
+              om.ZPlane(-10.0, boundary_type="vacuum"), om.ZPlane(10.0, boundary_type="vacuum")]
+    region = +planes[0] & -planes[1] & +planes[2] & -planes[3] & +planes[4] & -planes[5]
+    cell = om.Cell(fill=mat, region=region)
+    geom = om.Geometry(om.Universe(cells=[cell]))
+    s = om.Settings()
+    s.particles = n_particles
+    s.batches = n_batches
+    s.event_based = event_based
+    s.source = om.IndependentSource(space=stats.Point((0.0, 0.0, 0.0)), angle=stats.Isotropic(),
+                                    energy=stats.Discrete([1.0e6], [1.0]))
+    m = om.RegularMesh()
+    m.dimension = mesh
+    m.lower_left = (-10.0, -10.0, -10.0)
+    m.upper_right = (10.0, 10.0, 10.0)
+    t = om.Tally(name="flux")
+    t.filters = [om.MeshFilter(m)]
+    t.scores = ["flux"]
+    model = om.Model(geom, om.Materials([mat]), s, om.Tallies([t]))
+    return model, nucs
+
+
+def config1(n_particles=10_000_000, n_batches=20, grid_lo=20000, grid_hi=100000, n_nuclides=60,
+            lattice_n=17, mesh=(100, 100, 100), event_based=True, half_height=50.0,
+            energy_bins=None):
+    """17x17 pin lattice, three materials of 20 nuclides, Watt source in the fuel."""
+    om.reset_auto_ids()
+    nucs = _nuclide_set("L", n_nuclides, DATA_SEED_BASE + 1, grid_lo, grid_hi)
+    per = n_nuclides // 3
+    rng = np.random.default_rng(DATA_SEED_BASE + 11)
+    mats = []
+    for k, name in enumerate(["fuel", "clad", "moderator"]):
+        mt = om.Material(name=name)
+        for n in nucs[k * per:(k + 1) * per]:
+            mt.add_nuclide(n.name, float(rng.uniform(0.005, 0.05)))
+        mt.set_density("atom/b-cm", 0.1)
+        mats.append(mt)
+    pitch = 1.26
+    half = 0.5 * pitch * lattice_n
+    cyl1 = om.ZCylinder(r=0.4)
+    cyl2 = om.ZCylinder(r=0.47)
+    fuel = om.Cell(fill=mats[0], region=-cyl1, name="fuel")
+    clad = om.Cell(fill=mats[1], region=+cyl1 & -cyl2, name="clad")
+    mod = om.Cell(fill=mats[2], region=+cyl2, name="moderator")
+    pin = om.Universe(cells=[fuel, clad, mod])
+    lat = om.RectLattice()
+    lat.lower_left = (-half, -half)
+    lat.pitch = (pitch, pitch)
+    lat.universes = [[pin] * lattice_n for _ in range(lattice_n)]
+    xm = om.XPlane(-half, boundary_type="reflective")
+    xp = om.XPlane(half, boundary_type="reflective")
+    ym = om.YPlane(-half, boundary_type="reflective")
+    yp = om.YPlane(half, boundary_type="reflective")
+    zm = om.ZPlane(-half_height, boundary_type="vacuum")
+    zp = om.ZPlane(half_height, boundary_type="vacuum")
+    root = om.Cell(fill=lat, region=+xm & -xp & +ym & -yp & +zm & -zp)
+    geom = om.Geometry(om.Universe(cells=[root]))
+    s = om.Settings()
+    s.particles = n_particles
+    s.batches = n_batches
+    s.event_based = event_based
+    s.source = om.IndependentSource(
+        space=stats.Box((-half, -half, -half_height), (half, half, half_height)),
+        angle=stats.Isotropic(), energy=stats.Watt(0.988e6, 2.249e-6),
+        constraints={"domains": [fuel]})
+    m = om.RegularMesh()
+    m.dimension = mesh
+    m.lower_left = (-half, -half, -half_height)
+    m.upper_right = (half, half, half_height)
+    t = om.Tally(name="flux")
+    t.filters = [om.MeshFilter(m)]
+    if energy_bins is not None:
+        t.filters.append(om.EnergyFilter(energy_bins))
+    model = om.Model(geom, om.Materials(mats), s, om.Tallies([t]))
+    return model, nucs
+
+
+def config2(n_particles=10_000_000, n_batches=20, n_nuclides=300, n_grid=200000, mesh=(100, 100, 100),
+            event_based=True, half=50.0):
+    """One homogeneous material of 300 nuclides with 200k-point grids in a 1 m cube."""
+    om.reset_auto_ids()
+    nucs = [data.synthetic_nuclide("H%03d" % k, (DATA_SEED_BASE + 2) * 1000 + k, n_grid=n_grid)
+            for k in range(n_nuclides)]
+    rng = np.random.default_rng(DATA_SEED_BASE + 22)
+    mat = om.Material(name="mix300")
+    for n in nucs:
+        mat.add_nuclide(n.name, float(rng.uniform(0.005, 0.05)))
+    mat.set_density("atom/b-cm", 0.1)
+    planes = [om.XPlane(-half, boundary_type="vacuum"), om.XPlane(half, boundary_type="vacuum"),
+              om.YPlane(-half, boundary_type="vacuum"), om.YPlane(half, boundary_type="vacuum"),
+              om.ZPlane(-half, boundary_type="vacuum"), om.ZPlane(half, boundary_type="vacuum")]
+    region = +planes[0] & -planes[1] & +planes[2] & -planes[3] & +planes[4] & -planes[5]
+    cell = om.Cell(fill=mat, region=region)
+    geom = om.Geometry(om.Universe(cells=[cell]))
+    s = om.Settings()
+    s.particles = n_particles
+    s.batches = n_batches
+    s.event_based = event_based
+    s.source = om.IndependentSource(space=stats.Box((-half,) * 3, (half,) * 3), angle=stats.Isotropic(),
+                                    energy=stats.Watt(0.988e6, 2.249e-6))
+    m = om.RegularMesh()
+    m.dimension = mesh
+    m.lower_left = (-half,) * 3
+    m.upper_right = (half,) * 3
+    t = om.Tally(name="flux")
+    t.filters = [om.MeshFilter(m)]
+    model = om.Model(geom, om.Materials([mat]), s, om.Tallies([t]))
+    return model, nucs
+
+
+BUILDERS = {0: config0, 1: config1, 2: config2}
+
+
+def cache_root():
+    return os.environ.get("OPENMC_AMD_DATA", os.path.join(tempfile.gettempdir(), "openmc_amd_data"))
+
+
+def write_problem(model, nuclides, directory):
+    """Write nuclide files, cross_sections.xml and model.xml into directory."""
+    os.makedirs(directory, exist_ok=True)
+    xs = data.write_library(nuclides, os.path.join(directory, "data"))
+    model.materials.cross_sections = os.path.relpath(xs, directory)
+    model.export_to_model_xml(os.path.join(directory, "model.xml"))
+    return directory
+
+
+def problem_key(config, **kwargs):
+    return "cfg%d_%s" % (config, hashlib.sha1(
+        json.dumps([config, sorted(kwargs.items())], default=str).encode()).hexdigest()[:16])
+
+
+def problem_dir(config, **kwargs):
+    """Build (or reuse from the cache) the directory of a configured problem.
+
+    Returns (directory, model, nuclides); the model and nuclides are rebuilt
+    from their seeds (cheap next to transport) so callers can inspect them.
+    """
+    d = os.path.join(cache_root(), problem_key(config, **kwargs))
+    model, nucs = BUILDERS[config](**kwargs)
+    stamp = os.path.join(d, "complete")
+    if not os.path.exists(stamp):
+        write_problem(model, nucs, d)
+        with open(stamp, "w") as f:
+            f.write(json.dumps({"config": config, "kwargs": kwargs, "nuclides": [n.name for n in nucs]},
+                               default=str))
+    return d, model, nucs

@@ -1,0 +1,1069 @@
+// libopenmc.so: the C-ABI driver of the MI355X transport path.
+//
+// openmc_init reads the model (host_model.cpp), uploads every array the
+// kernels read into HBM once, and openmc_next_batch runs one batch entirely
+// on the GPU: source sampling, event cycles until the particle population is
+// empty, then tally accumulation.  Only queue counts (and, at the end of a
+// batch, the leakage scalar) cross PCIe during a batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unistd.h>
+#include <vector>
+
+#include "host_model.h"
+#include "learned.h"
+#include "openmc_amd/capi.h"
+#include "transport.h"
+
+// ---------------------------------------------------------------------------
+// Exported globals (names as bound by openmc/lib)
+// ---------------------------------------------------------------------------
+extern "C" {
+int OPENMC_E_UNASSIGNED = -1;
+int OPENMC_E_ALLOCATE = -2;
+int OPENMC_E_OUT_OF_BOUNDS = -3;
+int OPENMC_E_INVALID_SIZE = -4;
+int OPENMC_E_INVALID_ARGUMENT = -5;
+int OPENMC_E_INVALID_TYPE = -6;
+int OPENMC_E_INVALID_ID = -7;
+int OPENMC_E_GEOMETRY = -8;
+int OPENMC_E_DATA = -9;
+int OPENMC_E_PHYSICS = -10;
+int OPENMC_E_WARNING = 1;
+char openmc_err_msg[256] = {0};
+
+bool event_based = false;
+int64_t n_particles = 0;
+int32_t n_inactive = 0;
+int32_t gen_per_batch = 1;
+int current_batch = 0;
+int32_t n_realizations = 0;
+int run_mode = 1;
+int verbosity = 7;
+bool run_CE = true;
+int n_coord_levels = 0;
+bool DAGMC_ENABLED = false;
+bool LIBMESH_ENABLED = false;
+bool UWUW_ENABLED = false;
+}
+
+namespace omc {
+
+namespace {
+
+void set_errmsg(const std::string& s)
+{
+  std::snprintf(openmc_err_msg, sizeof(openmc_err_msg), "%s", s.c_str());
+}
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess)
+    throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr int N_GLOBAL = 4; // K_COLLISION, K_ABSORPTION, K_TRACKLENGTH, LEAKAGE
+constexpr int GT_LEAKAGE = 3;
+
+struct KernelTimer {
+  std::vector<hipEvent_t> start, stop;
+  std::vector<int> kind;
+  double ms[8] = {0};
+  long long launches[8] = {0};
+};
+
+struct Sim {
+  HostModel m;
+  bool initialized = false;
+  bool sim_initialized = false;
+  DevProblem P {};
+  DevParticles S {};
+  hipStream_t stream = nullptr;
+  std::vector<void*> dev_allocs;
+  std::vector<void*> part_allocs;
+  int64_t cap = 0;
+  int blocks = 0;
+  double* d_results = nullptr;
+  std::vector<double> h_results;
+  bool results_dirty = false;
+  int n_real = 0;
+  std::vector<double> gt = std::vector<double>(3 * N_GLOBAL, 0.0);
+  double total_weight = 0;
+  int* h_qcount = nullptr; // pinned
+  double* h_leak = nullptr; // pinned
+  std::vector<uint64_t> h_hashes;
+  // options
+  bool trace_hash = false;
+  bool use_learned = false;
+  bool event_timing = false;
+  bool use_graph = true;
+  int cycles_per_check = 16;
+  int64_t max_in_flight_opt = 0;
+  // graph
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool graph_valid = false;
+  // stats
+  double t_transport_ms = 0, t_batch_wall_s = 0;
+  int active_batches = 0;
+  unsigned long long counters[CNT_N] = {0};
+  long long n_cycles = 0, n_checks = 0;
+  KernelTimer kt;
+  Learned learned;
+};
+
+Sim* g = nullptr;
+
+template<class T>
+T* dev_upload(Sim& s, const std::vector<T>& v)
+{
+  if (v.empty())
+    return nullptr;
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, sizeof(T) * v.size()), "hipMalloc");
+  hip_check(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  s.dev_allocs.push_back(p);
+  return static_cast<T*>(p);
+}
+
+template<class T>
+T* dev_alloc(std::vector<void*>& list, size_t n)
+{
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)), "hipMalloc");
+  hip_check(hipMemset(p, 0, sizeof(T) * std::max<size_t>(n, 1)), "hipMemset");
+  list.push_back(p);
+  return static_cast<T*>(p);
+}
+
+void free_list(std::vector<void*>& list)
+{
+  for (void* p : list)
+    (void)hipFree(p);
+  list.clear();
+}
+
+void destroy_graph(Sim& s)
+{
+  if (s.graph_exec)
+    (void)hipGraphExecDestroy(s.graph_exec);
+  if (s.graph)
+    (void)hipGraphDestroy(s.graph);
+  s.graph_exec = nullptr;
+  s.graph = nullptr;
+  s.graph_valid = false;
+}
+
+void upload_problem(Sim& s)
+{
+  HostModel& m = s.m;
+  DevProblem& P = s.P;
+  std::memset(&P, 0, sizeof(P));
+
+  std::vector<DevNuclide> nucs;
+  std::vector<double> E, EL;
+  std::vector<double2> XS;
+  int64_t off = 0;
+  for (const auto& n : m.nuclides) {
+    DevNuclide d {};
+    d.off = off;
+    d.n = static_cast<int>(n.energy.size());
+    d.awr = n.awr;
+    d.kT = n.kT;
+    nucs.push_back(d);
+    off += d.n;
+  }
+  E.reserve(off);
+  EL.reserve(off);
+  XS.reserve(off);
+  for (const auto& n : m.nuclides) {
+    E.insert(E.end(), n.energy.begin(), n.energy.end());
+    EL.insert(EL.end(), n.elastic.begin(), n.elastic.end());
+    for (size_t k = 0; k < n.energy.size(); ++k)
+      XS.push_back(double2 {n.total[k], n.absorption[k]});
+  }
+  P.nuc = dev_upload(s, nucs);
+  P.E = dev_upload(s, E);
+  P.XS = dev_upload(s, XS);
+  P.EL = dev_upload(s, EL);
+  P.grid_index = dev_upload(s, m.grid_index);
+  P.n_nuclides = static_cast<int>(m.nuclides.size());
+  P.n_log_bins = m.n_log_bins;
+  P.energy_min = m.energy_min;
+  P.energy_max = m.energy_max;
+  P.log_spacing = m.log_spacing;
+
+  P.mat_off = dev_upload(s, m.mat_off);
+  P.mat_nuc = dev_upload(s, m.mat_nuc);
+  P.mat_dens = dev_upload(s, m.mat_dens);
+  P.n_materials = static_cast<int>(m.mat_ids.size());
+  int mx = 0;
+  for (size_t k = 0; k + 1 < m.mat_off.size(); ++k)
+    mx = std::max(mx, m.mat_off[k + 1] - m.mat_off[k]);
+  P.max_mat_nuc = mx;
+
+  P.surf = dev_upload(s, m.surfaces);
+  P.cell = dev_upload(s, m.cells);
+  P.tokens = dev_upload(s, m.tokens);
+  P.univ = dev_upload(s, m.universes);
+  P.univ_cells = dev_upload(s, m.univ_cells);
+  P.lat = dev_upload(s, m.lattices);
+  P.lat_univ = dev_upload(s, m.lat_univ);
+  P.root_universe = m.root_universe;
+  P.n_levels = m.n_levels;
+  P.n_surfaces = static_cast<int>(m.surfaces.size());
+
+  P.src = m.src;
+  P.src.disc_x = dev_upload(s, m.disc_x);
+  P.src.disc_prob = dev_upload(s, m.disc_prob);
+  P.src.disc_alias = dev_upload(s, m.disc_alias);
+  P.src.domain_ids = dev_upload(s, m.domain_ids);
+  P.src.n_domain = static_cast<int>(m.domain_ids.size());
+
+  if (!m.tallies.empty()) {
+    const HostTally& t = m.tallies[0];
+    DevTally& T = P.tally;
+    T.enabled = 0; // switched on for active batches
+    T.n_dim = t.n_dim;
+    for (int k = 0; k < 3; ++k) {
+      T.shape[k] = t.shape[k];
+      T.ll[k] = t.ll[k];
+      T.width[k] = t.width[k];
+    }
+    T.n_energy = t.ebins.empty() ? 0 : static_cast<int>(t.ebins.size()) - 1;
+    T.ebins = dev_upload(s, t.ebins);
+    T.n_bins = t.n_filter_bins;
+    T.value = dev_alloc<double>(s.dev_allocs, T.n_bins);
+    s.d_results = dev_alloc<double>(s.dev_allocs, 3 * T.n_bins);
+    s.h_results.assign(3 * T.n_bins, 0.0);
+  }
+
+  P.n_particles = m.n_particles;
+  P.max_events = m.max_events;
+  P.master_seed = m.seed;
+  P.stride = m.stride;
+  P.free_gas_threshold = m.free_gas_threshold;
+  P.energy_cutoff = m.energy_cutoff;
+  P.learned.enabled = 0;
+}
+
+void free_particles(Sim& s)
+{
+  destroy_graph(s);
+  free_list(s.part_allocs);
+  s.cap = 0;
+}
+
+void alloc_particles(Sim& s, int64_t cap)
+{
+  free_particles(s);
+  DevParticles& S = s.S;
+  std::vector<void*>& L = s.part_allocs;
+  const int64_t nl = MAX_LEVELS;
+  S.cap = cap;
+  size_t lv = static_cast<size_t>(nl * cap);
+  S.rx = dev_alloc<double>(L, lv);
+  S.ry = dev_alloc<double>(L, lv);
+  S.rz = dev_alloc<double>(L, lv);
+  S.ux = dev_alloc<double>(L, lv);
+  S.uy = dev_alloc<double>(L, lv);
+  S.uz = dev_alloc<double>(L, lv);
+  S.cell = dev_alloc<int>(L, lv);
+  S.univ = dev_alloc<int>(L, lv);
+  S.lat = dev_alloc<int>(L, lv);
+  S.lix = dev_alloc<int>(L, lv);
+  S.liy = dev_alloc<int>(L, lv);
+  S.liz = dev_alloc<int>(L, lv);
+  S.n_coord = dev_alloc<int>(L, cap);
+  S.E = dev_alloc<double>(L, cap);
+  S.wgt = dev_alloc<double>(L, cap);
+  S.time = dev_alloc<double>(L, cap);
+  S.seed = dev_alloc<uint64_t>(L, cap);
+  S.surface = dev_alloc<int>(L, cap);
+  S.material = dev_alloc<int>(L, cap);
+  S.material_last = dev_alloc<int>(L, cap);
+  S.xs_t = dev_alloc<double>(L, cap);
+  S.xs_a = dev_alloc<double>(L, cap);
+  S.micro_t = dev_alloc<double>(L, static_cast<size_t>(std::max(1, s.P.max_mat_nuc)) * cap);
+  S.b_surface = dev_alloc<int>(L, cap);
+  S.b_info = dev_alloc<int>(L, cap);
+  S.n_event = dev_alloc<int>(L, cap);
+  S.lost = dev_alloc<int>(L, cap);
+  S.hash = dev_alloc<uint64_t>(L, cap);
+  S.leak = dev_alloc<double>(L, 1);
+  S.q_xs = dev_alloc<int>(L, cap);
+  S.q_surf = dev_alloc<int>(L, cap);
+  S.q_coll = dev_alloc<int>(L, cap);
+  S.qcount = dev_alloc<int>(L, QC_N);
+  S.counters = dev_alloc<unsigned long long>(L, CNT_N);
+  s.cap = cap;
+  // launch shape: enough workgroups to fill 256 CUs several times over
+  int64_t b = (cap + 255) / 256;
+  s.blocks = static_cast<int>(std::min<int64_t>(std::max<int64_t>(b, 1), 4096));
+}
+
+int64_t bytes_per_particle(const Sim& s)
+{
+  int64_t lv = MAX_LEVELS * (6 * 8 + 6 * 4);
+  return lv + 4 + 3 * 8 + 8 + 3 * 4 + 2 * 8 + std::max(1, s.P.max_mat_nuc) * 8 + 4 * 4 + 8 + 3 * 4;
+}
+
+void choose_capacity(Sim& s)
+{
+  int64_t want = s.m.n_particles;
+  if (s.max_in_flight_opt > 0)
+    want = std::min(want, s.max_in_flight_opt);
+  else if (s.m.max_in_flight > 0 && s.m.max_in_flight < want)
+    want = s.m.max_in_flight;
+  size_t free_b = 0, total_b = 0;
+  hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  int64_t fit = static_cast<int64_t>(free_b / 2) / bytes_per_particle(s);
+  want = std::min<int64_t>(want, std::max<int64_t>(fit, 1));
+  want = std::min<int64_t>(want, static_cast<int64_t>(1) << 30);
+  if (want != s.cap)
+    alloc_particles(s, want);
+}
+
+void record_kernel(Sim& s, int kind, bool begin)
+{
+  KernelTimer& kt = s.kt;
+  hipEvent_t e;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipEventRecord(e, s.stream), "hipEventRecord");
+  if (begin) {
+    kt.start.push_back(e);
+    kt.kind.push_back(kind);
+  } else {
+    kt.stop.push_back(e);
+  }
+}
+
+void flush_kernel_timers(Sim& s)
+{
+  KernelTimer& kt = s.kt;
+  for (size_t k = 0; k < kt.stop.size(); ++k) {
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, kt.start[k], kt.stop[k]), "hipEventElapsedTime");
+    kt.ms[kt.kind[k]] += ms;
+    kt.launches[kt.kind[k]] += 1;
+  }
+  for (auto e : kt.start)
+    (void)hipEventDestroy(e);
+  for (auto e : kt.stop)
+    (void)hipEventDestroy(e);
+  kt.start.clear();
+  kt.stop.clear();
+  kt.kind.clear();
+}
+
+void run_cycles(Sim& s, int n_cycles)
+{
+  if (s.event_timing) {
+    for (int c = 0; c < n_cycles; ++c) {
+      for (int k = 0; k < 4; ++k) {
+        record_kernel(s, k, true);
+        launch_kernel(k, s.P, s.S, s.blocks, s.stream);
+        record_kernel(s, k, false);
+      }
+    }
+  } else if (s.use_graph) {
+    if (!s.graph_valid) {
+      destroy_graph(s);
+      hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      for (int c = 0; c < s.cycles_per_check; ++c)
+        launch_cycle(s.P, s.S, s.blocks, s.stream);
+      hip_check(hipStreamEndCapture(s.stream, &s.graph), "hipStreamEndCapture");
+      hip_check(hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      s.graph_valid = true;
+    }
+    hip_check(hipGraphLaunch(s.graph_exec, s.stream), "hipGraphLaunch");
+  } else {
+    for (int c = 0; c < n_cycles; ++c)
+      launch_cycle(s.P, s.S, s.blocks, s.stream);
+  }
+  s.n_cycles += n_cycles;
+}
+
+void transport_batch(Sim& s)
+{
+  HostModel& m = s.m;
+  const bool active = current_batch > m.n_inactive;
+  bool tally_on = active && !m.tallies.empty();
+  if (s.P.tally.enabled != (tally_on ? 1 : 0)) {
+    s.P.tally.enabled = tally_on ? 1 : 0;
+    s.graph_valid = false;
+  }
+  s.S.trace_hash = s.trace_hash ? 1 : 0;
+  s.S.use_learned = (s.use_learned && s.P.learned.enabled) ? 1 : 0;
+  if (s.trace_hash)
+    s.h_hashes.assign(m.n_particles, 0);
+  hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double), s.stream), "hipMemsetAsync");
+
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  auto w0 = std::chrono::steady_clock::now();
+  hip_check(hipEventRecord(e0, s.stream), "hipEventRecord");
+
+  int64_t remaining = m.n_particles;
+  int64_t offset = 0;
+  while (remaining > 0) {
+    int64_t n = std::min(remaining, s.cap);
+    int64_t id_base = (static_cast<int64_t>(current_batch) - 1) * m.n_particles + offset;
+    if (s.event_timing)
+      record_kernel(s, 4, true);
+    launch_init(s.P, s.S, n, id_base, s.stream);
+    if (s.event_timing)
+      record_kernel(s, 4, false);
+    s.h_qcount[QC_XS] = static_cast<int>(n);
+    s.h_qcount[QC_ADV] = 0;
+    s.h_qcount[QC_SURF] = 0;
+    s.h_qcount[QC_COLL] = 0;
+    hip_check(hipMemcpyAsync(s.S.qcount, s.h_qcount, sizeof(int) * QC_N, hipMemcpyHostToDevice, s.stream),
+      "hipMemcpyAsync");
+    // A batch ends when the calculate-xs queue stays empty after a cycle.
+    // Every history ends within max_particle_events events (one per cycle),
+    // so more cycles than that means a kernel failed to retire particles.
+    long long cycles = 0;
+    while (true) {
+      if (cycles > m.max_events + 16)
+        throw std::runtime_error("event loop did not drain within max_particle_events cycles");
+      run_cycles(s, s.cycles_per_check);
+      cycles += s.cycles_per_check;
+      hip_check(hipMemcpyAsync(s.h_qcount, s.S.qcount, sizeof(int) * QC_N, hipMemcpyDeviceToHost, s.stream),
+        "hipMemcpyAsync");
+      hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+      s.n_checks++;
+      if (s.event_timing)
+        flush_kernel_timers(s);
+      if (s.h_qcount[QC_XS] == 0)
+        break;
+    }
+    if (s.trace_hash)
+      hip_check(hipMemcpy(s.h_hashes.data() + offset, s.S.hash, sizeof(uint64_t) * n, hipMemcpyDeviceToHost),
+        "hipMemcpy");
+    remaining -= n;
+    offset += n;
+  }
+  hip_check(hipEventRecord(e1, s.stream), "hipEventRecord");
+  hip_check(hipMemcpyAsync(s.h_leak, s.S.leak, sizeof(double), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  auto w1 = std::chrono::steady_clock::now();
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (active) {
+    s.t_transport_ms += ms;
+    s.t_batch_wall_s += std::chrono::duration<double>(w1 - w0).count();
+    s.active_batches++;
+  }
+
+  // total_weight: every source particle starts with unit weight
+  s.total_weight = static_cast<double>(m.n_particles);
+
+  // accumulate_tallies (src/tallies/tally.cpp:1059-1104)
+  if (active) {
+    n_realizations += 1;
+    double val = *s.h_leak / s.total_weight;
+    s.gt[3 * GT_LEAKAGE + 1] += val;
+    s.gt[3 * GT_LEAKAGE + 2] += val * val;
+    if (!m.tallies.empty()) {
+      double total_source = 1.0;
+      double norm = total_source / (static_cast<double>(m.n_particles) * 1);
+      launch_accumulate(s.P.tally.value, s.d_results, s.P.tally.n_bins, norm, s.stream);
+      s.n_real++;
+      s.results_dirty = true;
+    }
+  }
+  hip_check(hipMemcpyAsync(s.counters, s.S.counters, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost,
+              s.stream),
+    "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  hip_check(hipGetLastError(), "kernel launch");
+}
+
+void sync_results(Sim& s)
+{
+  if (!s.results_dirty || s.h_results.empty())
+    return;
+  hip_check(hipMemcpy(s.h_results.data(), s.d_results, sizeof(double) * s.h_results.size(), hipMemcpyDeviceToHost),
+    "hipMemcpy");
+  s.results_dirty = false;
+}
+
+template<class F>
+int guarded(F f)
+{
+  try {
+    return f();
+  } catch (const HipError& e) {
+    set_errmsg(e.what());
+    return OPENMC_E_ALLOCATE;
+  } catch (const std::exception& e) {
+    set_errmsg(e.what());
+    return OPENMC_E_INVALID_ARGUMENT;
+  }
+}
+
+} // namespace
+
+Learned& learned_state()
+{
+  return g->learned;
+}
+DevProblem& problem_state()
+{
+  return g->P;
+}
+hipStream_t stream_state()
+{
+  return g->stream;
+}
+const HostModel& model_state()
+{
+  return g->m;
+}
+
+} // namespace omc
+
+using namespace omc;
+
+extern "C" {
+
+int openmc_init(int argc, char* argv[], const void* intracomm)
+{
+  (void)intracomm;
+  return guarded([&]() -> int {
+    if (g && g->initialized)
+      openmc_finalize();
+    std::string dir = ".";
+    for (int i = 1; i < argc; ++i) {
+      std::string a = argv[i] ? argv[i] : "";
+      if (a == "-e" || a == "--event")
+        event_based = true;
+      else if (a == "-s" || a == "--threads")
+        ++i; // host threads: the GPU path ignores them
+      else if (!a.empty() && a[0] != '-')
+        dir = a;
+    }
+    delete g;
+    g = new Sim();
+    read_model(g->m, dir);
+    initialize_data(g->m);
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+      throw std::runtime_error("no GPU visible: the MI355X transport path needs a HIP device");
+    hip_check(hipSetDevice(0), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&g->h_qcount), sizeof(int) * QC_N, hipHostMallocDefault),
+      "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&g->h_leak), sizeof(double), hipHostMallocDefault),
+      "hipHostMalloc");
+    upload_problem(*g);
+    if (g->m.event_based)
+      event_based = true;
+    n_particles = g->m.n_particles;
+    n_inactive = g->m.n_inactive;
+    n_coord_levels = g->m.n_levels;
+    current_batch = 0;
+    n_realizations = 0;
+    g->initialized = true;
+    return 0;
+  });
+}
+
+int openmc_simulation_init(void)
+{
+  if (!g || !g->initialized) {
+    set_errmsg("OpenMC has not been initialized.");
+    return OPENMC_E_ALLOCATE;
+  }
+  return guarded([&]() -> int {
+    if (g->sim_initialized)
+      return 0;
+    g->m.n_particles = n_particles;
+    g->P.n_particles = n_particles;
+    choose_capacity(*g);
+    current_batch = 0;
+    g->sim_initialized = true;
+    return 0;
+  });
+}
+
+int openmc_next_batch(int* status)
+{
+  if (!g || !g->sim_initialized) {
+    set_errmsg("Simulation has not been initialized yet.");
+    return OPENMC_E_ALLOCATE;
+  }
+  return guarded([&]() -> int {
+    if (current_batch >= g->m.n_batches) {
+      if (status)
+        *status = 1;
+      return 0;
+    }
+    ++current_batch;
+    transport_batch(*g);
+    if (status)
+      *status = current_batch >= g->m.n_batches ? 1 : 0;
+    return 0;
+  });
+}
+
+int openmc_simulation_finalize(void)
+{
+  if (!g || !g->sim_initialized)
+    return 0;
+  return guarded([&]() -> int {
+    sync_results(*g);
+    g->sim_initialized = false;
+    if (verbosity >= 6 && g->t_transport_ms > 0) {
+      double rate = static_cast<double>(g->m.n_particles) * g->active_batches / (g->t_transport_ms * 1e-3);
+      std::printf(" Calculation Rate (active)         = %g particles/second\n", rate);
+    }
+    return 0;
+  });
+}
+
+int openmc_run(void)
+{
+  int err = openmc_simulation_init();
+  if (err)
+    return err;
+  int status = 0;
+  while (status == 0) {
+    err = openmc_next_batch(&status);
+    if (err)
+      return err;
+  }
+  return openmc_simulation_finalize();
+}
+
+int openmc_reset(void)
+{
+  if (!g)
+    return 0;
+  return guarded([&]() -> int {
+    if (!g->h_results.empty()) {
+      hip_check(hipMemset(g->d_results, 0, sizeof(double) * g->h_results.size()), "hipMemset");
+      std::fill(g->h_results.begin(), g->h_results.end(), 0.0);
+    }
+    std::fill(g->gt.begin(), g->gt.end(), 0.0);
+    g->n_real = 0;
+    n_realizations = 0;
+    g->results_dirty = false;
+    return 0;
+  });
+}
+
+int openmc_hard_reset(void)
+{
+  int err = openmc_reset();
+  if (g) {
+    current_batch = 0;
+    g->m.seed = 1;
+    g->m.stride = 152917ULL;
+    g->P.master_seed = 1;
+    g->P.stride = 152917ULL;
+    g->graph_valid = false;
+  }
+  return err;
+}
+
+int openmc_reset_timers(void)
+{
+  if (g) {
+    g->t_transport_ms = 0;
+    g->t_batch_wall_s = 0;
+    g->active_batches = 0;
+    for (int k = 0; k < 8; ++k) {
+      g->kt.ms[k] = 0;
+      g->kt.launches[k] = 0;
+    }
+  }
+  return 0;
+}
+
+int openmc_finalize(void)
+{
+  if (!g)
+    return 0;
+  if (g->initialized) {
+    (void)hipStreamSynchronize(g->stream);
+    free_particles(*g);
+    free_learned(g->learned);
+    free_list(g->dev_allocs);
+    if (g->h_qcount)
+      (void)hipHostFree(g->h_qcount);
+    if (g->h_leak)
+      (void)hipHostFree(g->h_leak);
+    if (g->stream)
+      (void)hipStreamDestroy(g->stream);
+  }
+  delete g;
+  g = nullptr;
+  current_batch = 0;
+  n_realizations = 0;
+  return 0;
+}
+
+int openmc_get_tally_index(int32_t id, int32_t* index)
+{
+  if (!g)
+    return OPENMC_E_ALLOCATE;
+  for (size_t k = 0; k < g->m.tallies.size(); ++k) {
+    if (g->m.tallies[k].id == id) {
+      *index = static_cast<int32_t>(k);
+      return 0;
+    }
+  }
+  set_errmsg("No tally exists with ID=" + std::to_string(id) + ".");
+  return OPENMC_E_INVALID_ID;
+}
+
+int openmc_tally_get_id(int32_t index, int32_t* id)
+{
+  if (!g || index < 0 || index >= static_cast<int32_t>(g->m.tallies.size())) {
+    set_errmsg("Index in tallies array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  *id = g->m.tallies[index].id;
+  return 0;
+}
+
+int openmc_tally_get_n_realizations(int32_t index, int32_t* n)
+{
+  if (!g || index < 0 || index >= static_cast<int32_t>(g->m.tallies.size())) {
+    set_errmsg("Index in tallies array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  *n = g->n_real;
+  return 0;
+}
+
+int openmc_tally_get_estimator(int32_t index, int* estimator)
+{
+  if (!g || index < 0 || index >= static_cast<int32_t>(g->m.tallies.size())) {
+    set_errmsg("Index in tallies array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  *estimator = 2; // TallyEstimator::TRACKLENGTH (include/openmc/constants.h)
+  return 0;
+}
+
+int openmc_tally_get_active(int32_t index, bool* active)
+{
+  if (!g || index < 0 || index >= static_cast<int32_t>(g->m.tallies.size())) {
+    set_errmsg("Index in tallies array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  *active = current_batch > g->m.n_inactive;
+  return 0;
+}
+
+int openmc_tally_results(int32_t index, double** results, size_t* shape)
+{
+  if (!g || index < 0 || index >= static_cast<int32_t>(g->m.tallies.size())) {
+    set_errmsg("Index in tallies array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  if (g->h_results.empty()) {
+    set_errmsg("Tally results have not been allocated yet.");
+    return OPENMC_E_ALLOCATE;
+  }
+  return guarded([&]() -> int {
+    sync_results(*g);
+    *results = g->h_results.data();
+    shape[0] = static_cast<size_t>(g->m.tallies[index].n_filter_bins);
+    shape[1] = 1;
+    shape[2] = 3;
+    return 0;
+  });
+}
+
+int openmc_global_tallies(double** ptr)
+{
+  if (!g)
+    return OPENMC_E_ALLOCATE;
+  *ptr = g->gt.data();
+  return 0;
+}
+
+size_t tallies_size(void)
+{
+  return g ? g->m.tallies.size() : 0;
+}
+
+int64_t openmc_get_seed(void)
+{
+  return g ? static_cast<int64_t>(g->P.master_seed) : 1;
+}
+
+void openmc_set_seed(int64_t new_seed)
+{
+  if (g) {
+    g->m.seed = static_cast<uint64_t>(new_seed);
+    g->P.master_seed = static_cast<uint64_t>(new_seed);
+    g->graph_valid = false;
+  }
+}
+
+uint64_t openmc_get_stride(void)
+{
+  return g ? g->P.stride : 152917ULL;
+}
+
+void openmc_set_stride(uint64_t new_stride)
+{
+  if (g) {
+    g->m.stride = new_stride;
+    g->P.stride = new_stride;
+    g->graph_valid = false;
+  }
+}
+
+int omc_set_option(const char* name, double value)
+{
+  if (!g) {
+    set_errmsg("OpenMC has not been initialized.");
+    return OPENMC_E_ALLOCATE;
+  }
+  std::string n = name ? name : "";
+  g->graph_valid = false;
+  if (n == "trace_hash")
+    g->trace_hash = value != 0;
+  else if (n == "learned_index")
+    g->use_learned = value != 0;
+  else if (n == "event_timing")
+    g->event_timing = value != 0;
+  else if (n == "use_graph")
+    g->use_graph = value != 0;
+  else if (n == "cycles_per_check")
+    g->cycles_per_check = std::max(1, static_cast<int>(value));
+  else if (n == "max_in_flight") {
+    g->max_in_flight_opt = static_cast<int64_t>(value);
+    if (g->sim_initialized)
+      return guarded([&]() -> int {
+        choose_capacity(*g);
+        return 0;
+      });
+  } else {
+    set_errmsg("unknown option '" + n + "'");
+    return OPENMC_E_INVALID_ARGUMENT;
+  }
+  return 0;
+}
+
+static const char* k_stat_names =
+  "transport_ms,active_batches,lookups,collisions,segments,lost,xs_events,adv_events,surf_events,"
+  "coll_events,search_probes,cycles,checks,capacity,t_xs_ms,t_adv_ms,t_surf_ms,t_coll_ms,t_init_ms,"
+  "n_xs,n_adv,n_surf,n_coll,n_init,wall_s,blocks";
+
+const char* omc_stat_names(void)
+{
+  return k_stat_names;
+}
+
+int omc_get_stats(double* out, int n)
+{
+  if (!g)
+    return OPENMC_E_ALLOCATE;
+  std::vector<double> v;
+  v.push_back(g->t_transport_ms);
+  v.push_back(g->active_batches);
+  v.push_back(static_cast<double>(g->counters[CNT_LOOKUPS]));
+  v.push_back(static_cast<double>(g->counters[CNT_COLLISIONS]));
+  v.push_back(static_cast<double>(g->counters[CNT_SEGMENTS]));
+  v.push_back(static_cast<double>(g->counters[CNT_LOST]));
+  v.push_back(static_cast<double>(g->counters[CNT_XS_EVENTS]));
+  v.push_back(static_cast<double>(g->counters[CNT_ADV_EVENTS]));
+  v.push_back(static_cast<double>(g->counters[CNT_SURF_EVENTS]));
+  v.push_back(static_cast<double>(g->counters[CNT_COLL_EVENTS]));
+  v.push_back(static_cast<double>(g->counters[CNT_SEARCH_PROBES]));
+  v.push_back(static_cast<double>(g->n_cycles));
+  v.push_back(static_cast<double>(g->n_checks));
+  v.push_back(static_cast<double>(g->cap));
+  for (int k = 0; k < 5; ++k)
+    v.push_back(g->kt.ms[k]);
+  for (int k = 0; k < 5; ++k)
+    v.push_back(static_cast<double>(g->kt.launches[k]));
+  v.push_back(g->t_batch_wall_s);
+  v.push_back(g->blocks);
+  for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
+    out[k] = v[k];
+  return static_cast<int>(v.size());
+}
+
+int omc_particle_hashes(const uint64_t** ptr, int64_t* n)
+{
+  if (!g)
+    return OPENMC_E_ALLOCATE;
+  *ptr = g->h_hashes.data();
+  *n = static_cast<int64_t>(g->h_hashes.size());
+  return 0;
+}
+
+int omc_energy_grid(double* out3)
+{
+  if (!g)
+    return OPENMC_E_ALLOCATE;
+  out3[0] = g->m.energy_min;
+  out3[1] = g->m.energy_max;
+  out3[2] = g->m.log_spacing;
+  return 0;
+}
+
+int omc_n_nuclides(void)
+{
+  return g ? static_cast<int>(g->m.nuclides.size()) : 0;
+}
+
+const int* omc_grid_index(int i_nuc)
+{
+  if (!g || i_nuc < 0 || i_nuc >= static_cast<int>(g->m.nuclides.size()))
+    return nullptr;
+  return g->m.grid_index.data() + static_cast<size_t>(i_nuc) * (g->m.n_log_bins + 1);
+}
+
+int omc_lookup_batch(int i_nuc, const double* E, int64_t n, int learned, int* i_grid, double* xs, uint64_t* probes)
+{
+  if (!g || !g->initialized)
+    return OPENMC_E_ALLOCATE;
+  if (i_nuc < 0 || i_nuc >= g->P.n_nuclides) {
+    set_errmsg("nuclide index out of bounds");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  if (learned && !g->P.learned.enabled) {
+    set_errmsg("learned index has not been built");
+    return OPENMC_E_ALLOCATE;
+  }
+  return guarded([&]() -> int {
+    std::vector<void*> tmp;
+    double* dE = dev_alloc<double>(tmp, n);
+    int* dI = dev_alloc<int>(tmp, n);
+    double* dX = dev_alloc<double>(tmp, 2 * n);
+    unsigned long long* dP = dev_alloc<unsigned long long>(tmp, 1);
+    hip_check(hipMemcpy(dE, E, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    launch_lookup_batch(g->P, i_nuc, dE, n, learned, dI, dX, dP, g->stream);
+    hip_check(hipStreamSynchronize(g->stream), "lookup kernel");
+    hip_check(hipMemcpy(i_grid, dI, sizeof(int) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+    hip_check(hipMemcpy(xs, dX, sizeof(double) * 2 * n, hipMemcpyDeviceToHost), "hipMemcpy");
+    unsigned long long pr = 0;
+    hip_check(hipMemcpy(&pr, dP, sizeof(pr), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (probes)
+      *probes = pr;
+    free_list(tmp);
+    return 0;
+  });
+}
+
+int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat, double* macro, double* ms)
+{
+  if (!g || !g->initialized)
+    return OPENMC_E_ALLOCATE;
+  if (mat < 0 || mat >= g->P.n_materials) {
+    set_errmsg("material index out of bounds");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  if (learned && !g->P.learned.enabled) {
+    set_errmsg("learned index has not been built");
+    return OPENMC_E_ALLOCATE;
+  }
+  return guarded([&]() -> int {
+    std::vector<void*> tmp;
+    double* dE = dev_alloc<double>(tmp, n);
+    double* dM = dev_alloc<double>(tmp, 2 * n);
+    unsigned long long* dP = dev_alloc<unsigned long long>(tmp, 1);
+    hip_check(hipMemcpy(dE, E, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    launch_macro_batch(g->P, mat, dE, n, learned, dM, dP, g->stream); // warm-up
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    int reps = std::max(1, repeat);
+    hip_check(hipEventRecord(e0, g->stream), "hipEventRecord");
+    for (int r = 0; r < reps; ++r)
+      launch_macro_batch(g->P, mat, dE, n, learned, dM, nullptr, g->stream);
+    hip_check(hipEventRecord(e1, g->stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "macro kernel");
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+    if (ms)
+      *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hip_check(hipMemcpy(macro, dM, sizeof(double) * 2 * n, hipMemcpyDeviceToHost), "hipMemcpy");
+    free_list(tmp);
+    return 0;
+  });
+}
+
+int omc_check_model(const char* dir, double* out, int n_out, int* grid_index_out, int64_t n_gi)
+{
+  return guarded([&]() -> int {
+    HostModel m;
+    read_model(m, dir ? dir : ".");
+    initialize_data(m);
+    std::vector<double> v = {static_cast<double>(m.nuclides.size()), static_cast<double>(m.mat_ids.size()),
+      static_cast<double>(m.cells.size()), static_cast<double>(m.surfaces.size()),
+      static_cast<double>(m.lattices.size()), static_cast<double>(m.n_levels), m.energy_min, m.energy_max,
+      m.log_spacing, static_cast<double>(m.n_particles), static_cast<double>(m.n_batches),
+      m.tallies.empty() ? 0.0 : static_cast<double>(m.tallies[0].n_filter_bins),
+      static_cast<double>(m.root_universe)};
+    for (double d : m.mat_dens)
+      v.push_back(d);
+    for (int k = 0; k < n_out && k < static_cast<int>(v.size()); ++k)
+      out[k] = v[k];
+    if (grid_index_out)
+      for (int64_t k = 0; k < n_gi && k < static_cast<int64_t>(m.grid_index.size()); ++k)
+        grid_index_out[k] = m.grid_index[k];
+    return static_cast<int>(v.size());
+  });
+}
+
+int omc_math_batch(int fn, const double* x, const double* y, int64_t n, double* out)
+{
+  return guarded([&]() -> int {
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+      throw std::runtime_error("no GPU visible");
+    std::vector<void*> tmp;
+    double* dx = dev_alloc<double>(tmp, n);
+    double* dy = dev_alloc<double>(tmp, n);
+    double* dout = dev_alloc<double>(tmp, n);
+    hip_check(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    if (y)
+      hip_check(hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    launch_math_batch(fn, dx, dy, n, dout, nullptr);
+    hip_check(hipDeviceSynchronize(), "math kernel");
+    hip_check(hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+    free_list(tmp);
+    return 0;
+  });
+}
+
+int omc_learned_build(int n_seg, int epochs, int* max_err)
+{
+  if (!g || !g->initialized)
+    return OPENMC_E_ALLOCATE;
+  return guarded([&]() -> int {
+    int err = build_learned_index(g->learned, g->m, g->P, n_seg, epochs, g->stream);
+    if (max_err)
+      *max_err = err;
+    g->graph_valid = false;
+    return 0;
+  });
+}
+
+} // extern "C"

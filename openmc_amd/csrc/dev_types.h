@@ -1,0 +1,200 @@
+// Device-resident problem description and particle state (structure of arrays).
+//
+// Everything a transport kernel reads is reached from one DevProblem passed by
+// value as a kernel argument; all pointers are device pointers into HBM.
+// Particle state is a structure of arrays indexed by slot p in [0, cap);
+// per-coordinate-level fields are stored level-major ([level * cap + p]) and
+// per-material-nuclide microscopic totals nuclide-major ([j * cap + p]), so a
+// wavefront's 64 lanes touch 64 consecutive elements of every field.
+#pragma once
+
+#include <stdint.h>
+
+namespace omc {
+
+constexpr int MAX_LEVELS = 4;
+
+// Constants of the reference (include/openmc/constants.h:49-113)
+constexpr double TINY_BIT = 1e-8;
+constexpr double FP_PRECISION = 1e-14;
+constexpr double FP_REL_PRECISION = 1e-5;
+constexpr double FP_COINCIDENT = 1e-12;
+constexpr double PI = 3.141592653589793238462643383279502884;
+constexpr double INFTY = 1.7976931348623157e308;
+constexpr double MASS_NEUTRON = 1.00866491595;
+constexpr double MASS_NEUTRON_EV = 939.56542052e6;
+constexpr double C_LIGHT = 2.99792458e10;
+constexpr double N_AVOGADRO = 0.602214076;
+constexpr double K_BOLTZMANN = 8.617333262e-5;
+constexpr int C_NONE = -1;
+constexpr int MATERIAL_VOID = -1;
+constexpr int SURFACE_NONE = 0;
+
+enum SurfType : int { S_XPLANE = 1, S_YPLANE, S_ZPLANE, S_PLANE, S_ZCYL, S_XCYL, S_YCYL, S_SPHERE };
+enum BcType : int { BC_TRANSMIT = 0, BC_VACUUM = 1, BC_REFLECT = 2 };
+enum FillType : int { FILL_MATERIAL = 0, FILL_UNIVERSE = 1, FILL_LATTICE = 2 };
+
+struct DevSurface {
+  int type;
+  int bc;
+  double c[4];
+};
+
+struct DevCell {
+  int universe;
+  int fill_type;
+  int fill;
+  int material;
+  int tok_begin;
+  int tok_end;
+  int id;
+  int pad;
+};
+
+struct DevUniverse {
+  int cell_begin; // into univ_cells
+  int cell_end;
+};
+
+struct DevLattice {
+  double ll[3];
+  double pitch[3];
+  int n[3];
+  int is_3d;
+  int outer;
+  int univ_offset; // into lat_univ
+};
+
+struct DevNuclide {
+  int64_t off; // first grid point in the concatenated arrays
+  int n;       // grid points
+  int pad;
+  double awr;
+  double kT;
+};
+
+// Learned grid index: piecewise-linear model of i(log E) per nuclide.
+struct DevLearned {
+  const float* seg;    // [n_nuc][n_seg][2] slope, intercept (i ~= slope*u + icpt)
+  const int* err;      // [n_nuc][n_seg] max |i_true - i_pred| over the segment's grid points
+  const double* u_lo;  // [n_nuc] log(E_first) of each nuclide grid
+  const double* inv_w; // [n_nuc] n_seg / (log(E_last) - log(E_first))
+  int n_seg;
+  int enabled;
+};
+
+struct DevSource {
+  int space_type; // 0 point, 1 box
+  int angle_type; // 0 isotropic, 1 monodirectional
+  int energy_type; // 0 discrete, 1 watt
+  int n_discrete;
+  double p0[3], p1[3];
+  double u_ref[3];
+  double watt_a, watt_b;
+  const double* disc_x;
+  const double* disc_prob;
+  const int* disc_alias;
+  const int* domain_ids;
+  int n_domain;
+  int pad;
+};
+
+struct DevTally {
+  int enabled;
+  int n_dim;
+  int shape[3];
+  int n_energy; // energy bins (0 = no energy filter)
+  double ll[3];
+  double width[3];
+  const double* ebins; // n_energy + 1 edges
+  double* value;       // [n_bins] VALUE accumulator for the current batch
+  int64_t n_bins;
+};
+
+struct DevProblem {
+  // cross sections
+  const DevNuclide* nuc;
+  const double* E;   // concatenated energy grids
+  const double2* XS; // {total, absorption} per grid point
+  const double* EL;  // elastic per grid point
+  const int* grid_index; // [n_nuc][n_log_bins + 1]
+  int n_nuclides;
+  int n_log_bins;
+  double energy_min, energy_max, log_spacing;
+  DevLearned learned;
+
+  // materials
+  const int* mat_off; // [n_mat + 1]
+  const int* mat_nuc;
+  const double* mat_dens;
+  int n_materials;
+  int max_mat_nuc;
+
+  // geometry
+  const DevSurface* surf;
+  const DevCell* cell;
+  const int* tokens;
+  const DevUniverse* univ;
+  const int* univ_cells;
+  const DevLattice* lat;
+  const int* lat_univ;
+  int root_universe;
+  int n_levels;
+  int n_surfaces;
+  int pad_geo;
+
+  DevSource src;
+  DevTally tally;
+
+  // settings
+  int64_t n_particles; // per batch (id arithmetic)
+  int64_t max_events;
+  uint64_t master_seed;
+  uint64_t stride;
+  double free_gas_threshold;
+  double energy_cutoff;
+};
+
+// Counters accumulated by the kernels (device, int64)
+enum Counter : int {
+  CNT_LOOKUPS = 0,
+  CNT_COLLISIONS,
+  CNT_SEGMENTS,
+  CNT_LOST,
+  CNT_XS_EVENTS,
+  CNT_ADV_EVENTS,
+  CNT_SURF_EVENTS,
+  CNT_COLL_EVENTS,
+  CNT_SEARCH_PROBES,
+  CNT_N
+};
+
+// Queue counters (device, int32)
+enum QueueCount : int { QC_XS = 0, QC_ADV, QC_SURF, QC_COLL, QC_N };
+
+struct DevParticles {
+  int64_t cap;
+  // per level [MAX_LEVELS * cap]
+  double *rx, *ry, *rz, *ux, *uy, *uz;
+  int *cell, *univ, *lat, *lix, *liy, *liz;
+  // per particle
+  int* n_coord;
+  double *E, *wgt, *time;
+  uint64_t* seed;
+  int *surface, *material, *material_last;
+  double *xs_t, *xs_a;
+  double* micro_t; // [max_mat_nuc * cap]
+  int *b_surface, *b_info;
+  int* n_event;
+  int* lost;
+  uint64_t* hash;
+  double* leak; // per-batch global leakage accumulator [1]
+  // queues
+  int *q_xs, *q_surf, *q_coll;
+  int* qcount;        // [QC_N]
+  unsigned long long* counters; // [CNT_N]
+  int trace_hash;
+  int use_learned;
+};
+
+} // namespace omc

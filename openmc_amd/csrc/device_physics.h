@@ -1,0 +1,813 @@
+// Device-side physics of the event kernels: RNG, CSG geometry, cross-section
+// lookup, collision kinematics and the mesh ray-march.
+//
+// Each function restates the reference function cited beside it in the same
+// floating-point operation order (compiled with -ffp-contract=off), so that a
+// particle's history on the GPU is bit-identical to the reference algorithm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dev_types.h"
+#include "openmc_amd/detmath.h"
+
+namespace omc {
+
+//==============================================================================
+// RNG (src/random_lcg.cpp:32-120)
+//==============================================================================
+
+__device__ __forceinline__ double prn(uint64_t* seed)
+{
+  *seed = (6364136223846793005ULL * (*seed) + 1442695040888963407ULL);
+  uint64_t word = ((*seed >> ((*seed >> 59u) + 5u)) ^ *seed) * 12605985483714917081ull;
+  uint64_t result = (word >> 43u) ^ word;
+  return __builtin_ldexp(static_cast<double>(result), -64);
+}
+
+__host__ __device__ inline uint64_t future_seed(uint64_t n, uint64_t seed)
+{
+  uint64_t g = 6364136223846793005ULL;
+  uint64_t c = 1442695040888963407ULL;
+  uint64_t g_new = 1;
+  uint64_t c_new = 0;
+  while (n > 0) {
+    if (n & 1) {
+      g_new *= g;
+      c_new = c_new * g + c;
+    }
+    c *= (g + 1);
+    g *= g;
+    n >>= 1;
+  }
+  return g_new * seed + c_new;
+}
+
+__device__ __forceinline__ double uniform_distribution(double a, double b, uint64_t* seed)
+{
+  return a + (b - a) * prn(seed);
+}
+
+//==============================================================================
+// 3-vectors with the reference's Position semantics (include/openmc/position.h)
+//==============================================================================
+
+struct V3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ V3 v_add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 v_sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+// double * Position: each component times the scalar
+__device__ __forceinline__ V3 v_scale(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 v_div(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double v_dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double v_norm(V3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ double v_get(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// src/math_functions.cpp:763-792
+__device__ inline V3 rotate_angle(V3 u, double mu, uint64_t* seed)
+{
+  double phi_ = 2.0 * PI * prn(seed);
+  double sinphi, cosphi;
+  detm::sincos(phi_, &sinphi, &cosphi);
+  double a = sqrt(fmax(0., 1. - mu * mu));
+  double b = sqrt(fmax(0., 1. - u.z * u.z));
+  if (b > 1e-10) {
+    return {mu * u.x + a * (u.x * u.z * cosphi - u.y * sinphi) / b,
+      mu * u.y + a * (u.y * u.z * cosphi + u.x * sinphi) / b, mu * u.z - a * b * cosphi};
+  } else {
+    b = sqrt(1. - u.y * u.y);
+    return {mu * u.x + a * (-u.x * u.y * sinphi + u.z * cosphi) / b, mu * u.y + a * b * sinphi,
+      mu * u.z - a * (u.y * u.z * sinphi + u.x * cosphi) / b};
+  }
+}
+
+//==============================================================================
+// Geometry state of one particle held in registers
+//==============================================================================
+
+struct Geo {
+  V3 r[MAX_LEVELS];
+  V3 u[MAX_LEVELS];
+  int cell[MAX_LEVELS];
+  int univ[MAX_LEVELS];
+  int lat[MAX_LEVELS];
+  int li[MAX_LEVELS][3];
+  int n_coord;
+  int surface;
+  int material;
+  int material_last;
+  int lost;
+};
+
+__device__ __forceinline__ void coord_reset(Geo& g, int i)
+{
+  g.cell[i] = C_NONE;
+  g.univ[i] = C_NONE;
+  g.lat[i] = C_NONE;
+  g.li[i][0] = g.li[i][1] = g.li[i][2] = 0;
+}
+
+//==============================================================================
+// Surfaces (src/surface.cpp)
+//==============================================================================
+
+__device__ inline double surf_evaluate(const DevSurface& s, V3 r)
+{
+  switch (s.type) {
+  case S_XPLANE: return r.x - s.c[0];
+  case S_YPLANE: return r.y - s.c[0];
+  case S_ZPLANE: return r.z - s.c[0];
+  case S_PLANE: return s.c[0] * r.x + s.c[1] * r.y + s.c[2] * r.z - s.c[3];
+  case S_ZCYL: {
+    const double r1 = r.x - s.c[0];
+    const double r2 = r.y - s.c[1];
+    return r1 * r1 + r2 * r2 - s.c[2] * s.c[2];
+  }
+  case S_XCYL: {
+    const double r1 = r.y - s.c[0];
+    const double r2 = r.z - s.c[1];
+    return r1 * r1 + r2 * r2 - s.c[2] * s.c[2];
+  }
+  case S_YCYL: {
+    const double r1 = r.x - s.c[0];
+    const double r2 = r.z - s.c[1];
+    return r1 * r1 + r2 * r2 - s.c[2] * s.c[2];
+  }
+  default: {
+    const double x = r.x - s.c[0];
+    const double y = r.y - s.c[1];
+    const double z = r.z - s.c[2];
+    return x * x + y * y + z * z - s.c[3] * s.c[3];
+  }
+  }
+}
+
+// axis_aligned_plane_distance (src/surface.cpp:208-218)
+__device__ __forceinline__ double plane_distance(double ri, double ui, bool coincident, double offset)
+{
+  const double f = offset - ri;
+  if (coincident || fabs(f) < FP_COINCIDENT || ui == 0.0)
+    return INFTY;
+  const double d = f / ui;
+  if (d < 0.0)
+    return INFTY;
+  return d;
+}
+
+// axis_aligned_cylinder_distance (src/surface.cpp:403-442)
+__device__ inline double cyl_distance(double u1, double r2p, double u2, double r3p, double u3,
+  bool coincident, double off1, double off2, double radius)
+{
+  const double a = 1.0 - u1 * u1;
+  if (a == 0.0)
+    return INFTY;
+  const double r2 = r2p - off1;
+  const double r3 = r3p - off2;
+  const double k = r2 * u2 + r3 * u3;
+  const double c = r2 * r2 + r3 * r3 - radius * radius;
+  const double quad = k * k - a * c;
+  if (quad < 0.0) {
+    return INFTY;
+  } else if (coincident || fabs(c) < FP_COINCIDENT) {
+    if (k >= 0.0)
+      return INFTY;
+    return (-k + sqrt(quad)) / a;
+  } else if (c < 0.0) {
+    return (-k + sqrt(quad)) / a;
+  } else {
+    const double d = (-k - sqrt(quad)) / a;
+    if (d < 0.0)
+      return INFTY;
+    return d;
+  }
+}
+
+__device__ inline double surf_distance(const DevSurface& s, V3 r, V3 u, bool coincident)
+{
+  switch (s.type) {
+  case S_XPLANE: return plane_distance(r.x, u.x, coincident, s.c[0]);
+  case S_YPLANE: return plane_distance(r.y, u.y, coincident, s.c[0]);
+  case S_ZPLANE: return plane_distance(r.z, u.z, coincident, s.c[0]);
+  case S_PLANE: {
+    const double f = s.c[0] * r.x + s.c[1] * r.y + s.c[2] * r.z - s.c[3];
+    const double projection = s.c[0] * u.x + s.c[1] * u.y + s.c[2] * u.z;
+    if (coincident || fabs(f) < FP_COINCIDENT || projection == 0.0)
+      return INFTY;
+    const double d = -f / projection;
+    if (d < 0.0)
+      return INFTY;
+    return d;
+  }
+  case S_ZCYL: return cyl_distance(u.z, r.x, u.x, r.y, u.y, coincident, s.c[0], s.c[1], s.c[2]);
+  case S_XCYL: return cyl_distance(u.x, r.y, u.y, r.z, u.z, coincident, s.c[0], s.c[1], s.c[2]);
+  case S_YCYL: return cyl_distance(u.y, r.x, u.x, r.z, u.z, coincident, s.c[0], s.c[1], s.c[2]);
+  default: {
+    const double x = r.x - s.c[0];
+    const double y = r.y - s.c[1];
+    const double z = r.z - s.c[2];
+    const double k = x * u.x + y * u.y + z * u.z;
+    const double c = x * x + y * y + z * z - s.c[3] * s.c[3];
+    const double quad = k * k - c;
+    if (quad < 0.0) {
+      return INFTY;
+    } else if (coincident || fabs(c) < FP_COINCIDENT) {
+      if (k >= 0.0)
+        return INFTY;
+      return -k + sqrt(quad);
+    } else if (c < 0.0) {
+      return -k + sqrt(quad);
+    } else {
+      const double d = -k - sqrt(quad);
+      if (d < 0.0)
+        return INFTY;
+      return d;
+    }
+  }
+  }
+}
+
+__device__ inline V3 surf_normal(const DevSurface& s, V3 r)
+{
+  switch (s.type) {
+  case S_XPLANE: return {1., 0., 0.};
+  case S_YPLANE: return {0., 1., 0.};
+  case S_ZPLANE: return {0., 0., 1.};
+  case S_PLANE: return {s.c[0], s.c[1], s.c[2]};
+  case S_ZCYL: return {2.0 * (r.x - s.c[0]), 2.0 * (r.y - s.c[1]), 0.0};
+  case S_XCYL: return {0.0, 2.0 * (r.y - s.c[0]), 2.0 * (r.z - s.c[1])};
+  case S_YCYL: return {2.0 * (r.x - s.c[0]), 0.0, 2.0 * (r.z - s.c[1])};
+  default: return {2.0 * (r.x - s.c[0]), 2.0 * (r.y - s.c[1]), 2.0 * (r.z - s.c[2])};
+  }
+}
+
+// Surface::sense (src/surface.cpp:116-130)
+__device__ inline bool surf_sense(const DevSurface& s, V3 r, V3 u)
+{
+  const double f = surf_evaluate(s, r);
+  if (fabs(f) < FP_COINCIDENT)
+    return v_dot(u, surf_normal(s, r)) > 0.0;
+  return f > 0.0;
+}
+
+//==============================================================================
+// Cells, universes, lattices (src/cell.cpp, src/universe.cpp, src/lattice.cpp)
+//==============================================================================
+
+// Region::contains_simple (src/cell.cpp:951-970)
+__device__ inline bool cell_contains(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface)
+{
+  for (int k = c.tok_begin; k < c.tok_end; ++k) {
+    int token = P.tokens[k];
+    if (token == on_surface) {
+    } else if (-token == on_surface) {
+      return false;
+    } else {
+      bool sense = surf_sense(P.surf[abs(token) - 1], r, u);
+      if (sense != (token > 0))
+        return false;
+    }
+  }
+  return true;
+}
+
+// Region::distance (src/cell.cpp:910-936)
+__device__ inline double cell_distance(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface,
+  int* i_surf)
+{
+  double min_dist = INFTY;
+  int isurf = 2147483647;
+  for (int k = c.tok_begin; k < c.tok_end; ++k) {
+    int token = P.tokens[k];
+    bool coincident = abs(token) == abs(on_surface);
+    double d = surf_distance(P.surf[abs(token) - 1], r, u, coincident);
+    if (d < min_dist) {
+      if (min_dist - d >= FP_PRECISION * min_dist) {
+        min_dist = d;
+        isurf = -token;
+      }
+    }
+  }
+  *i_surf = isurf;
+  return min_dist;
+}
+
+// RectLattice::distance (src/lattice.cpp:252-306)
+__device__ inline double lattice_distance(const DevLattice& L, V3 r, V3 u, int* trans)
+{
+  double x0 = copysign(0.5 * L.pitch[0], u.x);
+  double y0 = copysign(0.5 * L.pitch[1], u.y);
+  double d = INFTY;
+  trans[0] = trans[1] = trans[2] = 0;
+  if ((fabs(r.x - x0) > FP_PRECISION) && u.x != 0) {
+    d = (x0 - r.x) / u.x;
+    trans[0] = u.x > 0 ? 1 : -1;
+    trans[1] = 0;
+    trans[2] = 0;
+  }
+  if ((fabs(r.y - y0) > FP_PRECISION) && u.y != 0) {
+    double this_d = (y0 - r.y) / u.y;
+    if (this_d < d) {
+      d = this_d;
+      trans[0] = 0;
+      trans[1] = u.y > 0 ? 1 : -1;
+      trans[2] = 0;
+    }
+  }
+  if (L.is_3d) {
+    double z0 = copysign(0.5 * L.pitch[2], u.z);
+    if ((fabs(r.z - z0) > FP_PRECISION) && u.z != 0) {
+      double this_d = (z0 - r.z) / u.z;
+      if (this_d < d) {
+        d = this_d;
+        trans[0] = 0;
+        trans[1] = 0;
+        trans[2] = u.z > 0 ? 1 : -1;
+      }
+    }
+  }
+  return d;
+}
+
+// RectLattice::get_indices (src/lattice.cpp:310-342)
+__device__ inline void lattice_indices(const DevLattice& L, V3 r, V3 u, int* res)
+{
+  double rr[3] = {r.x, r.y, r.z};
+  double uu[3] = {u.x, u.y, u.z};
+  res[2] = 0;
+  int nd = L.is_3d ? 3 : 2;
+  for (int k = 0; k < nd; ++k) {
+    double i_ = (rr[k] - L.ll[k]) / L.pitch[k];
+    long i_close = detm::lround(i_);
+    if (fabs(i_ - static_cast<double>(i_close)) < FP_COINCIDENT) {
+      res[k] = (uu[k] > 0) ? static_cast<int>(i_close) : static_cast<int>(i_close - 1);
+    } else {
+      res[k] = static_cast<int>(floor(i_));
+    }
+  }
+}
+
+// RectLattice::get_local_position (src/lattice.cpp:352-361)
+__device__ inline V3 lattice_local(const DevLattice& L, V3 r, const int* i)
+{
+  r.x -= (L.ll[0] + (i[0] + 0.5) * L.pitch[0]);
+  r.y -= (L.ll[1] + (i[1] + 0.5) * L.pitch[1]);
+  if (L.is_3d)
+    r.z -= (L.ll[2] + (i[2] + 0.5) * L.pitch[2]);
+  return r;
+}
+
+__device__ __forceinline__ bool lattice_valid(const DevLattice& L, const int* i)
+{
+  return (i[0] >= 0) && (i[0] < L.n[0]) && (i[1] >= 0) && (i[1] < L.n[1]) && (i[2] >= 0) &&
+         (i[2] < L.n[2]);
+}
+
+__device__ __forceinline__ int lattice_universe(const DevProblem& P, const DevLattice& L, const int* i)
+{
+  return P.lat_univ[L.univ_offset + L.n[0] * L.n[1] * i[2] + L.n[0] * i[1] + i[0]];
+}
+
+// Universe::find_cell (src/universe.cpp:40-60)
+__device__ inline bool universe_find_cell(const DevProblem& P, Geo& g)
+{
+  int lvl = g.n_coord - 1;
+  int i_univ = g.univ[lvl];
+  const DevUniverse U = P.univ[i_univ];
+  V3 r = g.r[lvl];
+  V3 u = g.u[lvl];
+  for (int k = U.cell_begin; k < U.cell_end; ++k) {
+    int i_cell = P.univ_cells[k];
+    if (cell_contains(P, P.cell[i_cell], r, u, g.surface)) {
+      g.cell[lvl] = i_cell;
+      return true;
+    }
+  }
+  return false;
+}
+
+// find_cell_inner (src/geometry.cpp:102-252), searching the whole universe at
+// each level (the neighbor lists of the reference visit a subset of the same
+// universe and, for universes whose cells partition space, find the same cell)
+__device__ inline bool find_cell_inner(const DevProblem& P, Geo& g)
+{
+  for (;; ++g.n_coord) {
+    if (g.n_coord > MAX_LEVELS)
+      return false;
+    if (!universe_find_cell(P, g))
+      return false;
+    int lvl = g.n_coord - 1;
+    const DevCell c = P.cell[g.cell[lvl]];
+    if (c.fill_type == FILL_MATERIAL) {
+      g.material_last = g.material;
+      g.material = c.material;
+      return true;
+    }
+    if (g.n_coord >= MAX_LEVELS)
+      return false;
+    int nl = g.n_coord;
+    if (c.fill_type == FILL_UNIVERSE) {
+      g.univ[nl] = c.fill;
+      g.r[nl] = g.r[lvl];
+      g.u[nl] = g.u[lvl];
+    } else {
+      const DevLattice L = P.lat[c.fill];
+      g.r[nl] = g.r[lvl];
+      g.u[nl] = g.u[lvl];
+      int idx[3];
+      lattice_indices(L, g.r[nl], g.u[nl], idx);
+      g.li[nl][0] = idx[0];
+      g.li[nl][1] = idx[1];
+      g.li[nl][2] = idx[2];
+      g.r[nl] = lattice_local(L, g.r[nl], idx);
+      g.lat[nl] = c.fill;
+      if (lattice_valid(L, idx)) {
+        g.univ[nl] = lattice_universe(P, L, idx);
+      } else if (L.outer != C_NONE) {
+        g.univ[nl] = L.outer;
+      } else {
+        g.lost = 1;
+        return false;
+      }
+    }
+  }
+}
+
+// exhaustive_find_cell (src/geometry.cpp:284-297)
+__device__ inline bool exhaustive_find_cell(const DevProblem& P, Geo& g)
+{
+  if (g.univ[g.n_coord - 1] == C_NONE) {
+    g.univ[0] = P.root_universe;
+    g.n_coord = 1;
+  }
+  for (int i = g.n_coord; i < MAX_LEVELS; ++i)
+    coord_reset(g, i);
+  return find_cell_inner(P, g);
+}
+
+// neighbor_list_find_cell (src/geometry.cpp:256-282)
+__device__ inline bool neighbor_list_find_cell(const DevProblem& P, Geo& g)
+{
+  for (int i = g.n_coord; i < MAX_LEVELS; ++i)
+    coord_reset(g, i);
+  return find_cell_inner(P, g);
+}
+
+struct Boundary {
+  double distance;
+  int surface;
+  int coord_level;
+  int trans[3];
+};
+
+// distance_to_boundary (src/geometry.cpp:361-453)
+__device__ inline Boundary distance_to_boundary(const DevProblem& P, Geo& g)
+{
+  Boundary info;
+  info.distance = __builtin_inf();
+  info.surface = SURFACE_NONE;
+  info.coord_level = 0;
+  info.trans[0] = info.trans[1] = info.trans[2] = 0;
+  double d_lat = __builtin_inf();
+  int level_lat_trans[3] = {0, 0, 0};
+  for (int i = 0; i < g.n_coord; i++) {
+    V3 r = g.r[i];
+    V3 u = g.u[i];
+    int level_surf_cross;
+    double d_surf = cell_distance(P, P.cell[g.cell[i]], r, u, g.surface, &level_surf_cross);
+    if (g.lat[i] != C_NONE) {
+      d_lat = lattice_distance(P.lat[g.lat[i]], r, u, level_lat_trans);
+      if (d_lat < 0)
+        g.lost = 1;
+    }
+    double& d = info.distance;
+    if (d_surf < d_lat - FP_COINCIDENT) {
+      if (d == __builtin_inf() || (d - d_surf) / d >= FP_REL_PRECISION) {
+        d = d_surf;
+        info.surface = level_surf_cross;
+        info.trans[0] = info.trans[1] = info.trans[2] = 0;
+        info.coord_level = i + 1;
+      }
+    } else {
+      if (d == __builtin_inf() || (d - d_lat) / d >= FP_REL_PRECISION) {
+        d = d_lat;
+        info.surface = SURFACE_NONE;
+        info.trans[0] = level_lat_trans[0];
+        info.trans[1] = level_lat_trans[1];
+        info.trans[2] = level_lat_trans[2];
+        info.coord_level = i + 1;
+      }
+    }
+  }
+  return info;
+}
+
+// cross_lattice (src/geometry.cpp:301-357)
+__device__ inline void cross_lattice(const DevProblem& P, Geo& g, const int* trans)
+{
+  int lvl = g.n_coord - 1;
+  if (lvl < 1 || g.lat[lvl] == C_NONE) {
+    // a lattice translation selected at a level without a lattice; the
+    // reference would index its lattice array with -1 here
+    g.lost = 1;
+    return;
+  }
+  const DevLattice L = P.lat[g.lat[lvl]];
+  g.li[lvl][0] += trans[0];
+  g.li[lvl][1] += trans[1];
+  g.li[lvl][2] += trans[2];
+  V3 r = g.r[lvl - 1];
+  int idx[3] = {g.li[lvl][0], g.li[lvl][1], g.li[lvl][2]};
+  g.r[lvl] = lattice_local(L, r, idx);
+  if (!lattice_valid(L, idx)) {
+    g.n_coord = 1;
+    if (!exhaustive_find_cell(P, g))
+      g.lost = 1;
+  } else {
+    g.univ[lvl] = lattice_universe(P, L, idx);
+    if (!exhaustive_find_cell(P, g)) {
+      g.n_coord = 1;
+      if (!exhaustive_find_cell(P, g))
+        g.lost = 1;
+    }
+  }
+}
+
+// Position::reflect (include/openmc/position.h:207-213)
+__device__ __forceinline__ V3 reflect(V3 u, V3 n)
+{
+  const double projection = v_dot(n, u);
+  const double magnitude = v_dot(n, n);
+  double s = 2.0 * projection / magnitude;
+  n = v_scale(n, s);
+  return v_sub(u, n);
+}
+
+//==============================================================================
+// Cross sections (src/nuclide.cpp:616-839, src/material.cpp:829-901)
+//==============================================================================
+
+// Bucket bounds of the log-grid hash, clamped to the table.
+__device__ __forceinline__ int log_union_index(const DevProblem& P, double E)
+{
+  int i = static_cast<int>(detm::log(E / P.energy_min) / P.log_spacing);
+  return i;
+}
+
+// std::lower_bound over E[first, first + count): first index with !(E[i] < v)
+__device__ __forceinline__ int lower_bound_idx(const double* E, int first, int count, double v, int& probes)
+{
+  while (count > 0) {
+    int step = count >> 1;
+    int it = first + step;
+    ++probes;
+    if (E[it] < v) {
+      first = it + 1;
+      count -= step + 1;
+    } else {
+      count = step;
+    }
+  }
+  return first;
+}
+
+// Learned bounded search.  The nuclide's piecewise-linear model maps
+// x = log(E/E_min) to a predicted interval index j; the recorded bound err of
+// the segment (max |i - pred(x_i)| over the grid points the segment can see,
+// evaluated with this same FP32 arithmetic) guarantees that the global
+// lower_bound of E lies in [j - err - 1, j + err + 3), which is then searched
+// exactly.  Returns the global lower_bound index (first i with E[i] >= E).
+__device__ inline int learned_lower_bound(const DevProblem& P, int i_nuc, const double* Eg, int n,
+  double E, double lx, int& probes)
+{
+  const DevLearned& Lm = P.learned;
+  double uu = (lx - Lm.u_lo[i_nuc]) * Lm.inv_w[i_nuc];
+  int s = uu < 0.0 ? 0 : static_cast<int>(uu);
+  s = s >= Lm.n_seg ? Lm.n_seg - 1 : s;
+  const int64_t k = static_cast<int64_t>(i_nuc) * Lm.n_seg + s;
+  const float2 sg = reinterpret_cast<const float2*>(Lm.seg)[k];
+  const int err = Lm.err[k];
+  float fr = static_cast<float>(uu - static_cast<double>(s));
+  float pred = sg.x * fr + sg.y;
+  int j = static_cast<int>(pred);
+  int lo = j - err - 1;
+  int hi = j + err + 3;
+  lo = lo < 0 ? 0 : (lo > n ? n : lo);
+  hi = hi > n ? n : (hi < lo ? lo : hi);
+  return lower_bound_idx(Eg, lo, hi - lo, E, probes);
+}
+
+struct Micro {
+  int i_grid;
+  double f;
+  double total;
+  double absorption;
+};
+
+// Nuclide::calculate_xs, NEAREST temperature with one temperature
+__device__ inline Micro nuclide_xs(const DevProblem& P, int i_nuc, double E, int i_log, double logE,
+  bool learned, int& probes)
+{
+  const DevNuclide nd = P.nuc[i_nuc];
+  const double* Eg = P.E + nd.off;
+  const int n = nd.n;
+  int i_grid;
+  if (E < Eg[0]) {
+    i_grid = 0;
+  } else if (E > Eg[n - 1]) {
+    i_grid = n - 2;
+  } else {
+    const int* gi = P.grid_index + static_cast<int64_t>(i_nuc) * (P.n_log_bins + 1);
+    if (!learned) {
+      int i_low = gi[i_log];
+      int i_high = gi[i_log + 1] + 1;
+      // lower_bound_index (include/openmc/search.h:13-20)
+      if (Eg[i_low] == E) {
+        i_grid = i_low;
+      } else {
+        i_grid = lower_bound_idx(Eg, i_low, i_high - i_low, E, probes) - 1;
+      }
+    } else {
+      int lb = learned_lower_bound(P, i_nuc, Eg, n, E, logE, probes);
+      // The bucket search equals lower_bound - 1 except when E equals the
+      // first point of its bucket (search.h:17-18), which needs E to be a grid
+      // energy; only then is the bucket start read.
+      if (lb < n && Eg[lb] == E) {
+        int il = gi[i_log];
+        i_grid = (Eg[il] == E) ? il : lb - 1;
+      } else {
+        i_grid = lb - 1;
+      }
+    }
+  }
+  // keep every read inside the grid (a no-op whenever the search is correct,
+  // which the parity tests check index for index)
+  i_grid = i_grid < 0 ? 0 : (i_grid > n - 2 ? n - 2 : i_grid);
+  // check for rare case where two energy points are the same
+  if (i_grid + 2 < n && Eg[i_grid] == Eg[i_grid + 1])
+    ++i_grid;
+  double f = (E - Eg[i_grid]) / (Eg[i_grid + 1] - Eg[i_grid]);
+  const double2 x0 = P.XS[nd.off + i_grid];
+  const double2 x1 = P.XS[nd.off + i_grid + 1];
+  Micro m;
+  m.i_grid = i_grid;
+  m.f = f;
+  m.total = (1.0 - f) * x0.x + f * x1.x;
+  m.absorption = (1.0 - f) * x0.y + f * x1.y;
+  return m;
+}
+
+//==============================================================================
+// Tally: RegularMesh ray-march (src/mesh.cpp:1000-1146, 1328-1369)
+//==============================================================================
+
+struct MeshDist {
+  int next_index;
+  bool max_surface;
+  double distance;
+};
+
+__device__ inline MeshDist mesh_distance(const DevTally& T, const int* ijk, int i, V3 r0, V3 u)
+{
+  MeshDist d;
+  d.next_index = ijk[i];
+  d.max_surface = true;
+  d.distance = INFTY;
+  double ui = v_get(u, i);
+  if (fabs(ui) < FP_PRECISION)
+    return d;
+  d.max_surface = (ui > 0);
+  if (d.max_surface && (ijk[i] <= T.shape[i])) {
+    d.next_index++;
+    d.distance = ((T.ll[i] + ijk[i] * T.width[i]) - v_get(r0, i)) / ui;
+  } else if (!d.max_surface && (ijk[i] >= 1)) {
+    d.next_index--;
+    d.distance = ((T.ll[i] + (ijk[i] - 1) * T.width[i]) - v_get(r0, i)) / ui;
+  }
+  return d;
+}
+
+__device__ inline void mesh_indices(const DevTally& T, V3 r, int* ijk, bool& in_mesh)
+{
+  in_mesh = true;
+  for (int i = 0; i < T.n_dim; ++i) {
+    ijk[i] = static_cast<int>(ceil((v_get(r, i) - T.ll[i]) / T.width[i]));
+    if (ijk[i] < 1 || ijk[i] > T.shape[i])
+      in_mesh = false;
+  }
+}
+
+__device__ __forceinline__ int64_t mesh_bin(const DevTally& T, const int* ijk)
+{
+  switch (T.n_dim) {
+  case 1: return ijk[0] - 1;
+  case 2: return static_cast<int64_t>(ijk[1] - 1) * T.shape[0] + ijk[0] - 1;
+  default:
+    return (static_cast<int64_t>(ijk[2] - 1) * T.shape[1] + (ijk[1] - 1)) * T.shape[0] + ijk[0] - 1;
+  }
+}
+
+__device__ __forceinline__ void tally_add(const DevTally& T, int64_t bin, double score)
+{
+  unsafeAtomicAdd(&T.value[bin], score);
+}
+
+// score_tracklength_tally with [mesh] or [mesh, energy] filters
+// (src/tallies/tally_scoring.cpp:2416-2541); returns segments scored
+__device__ inline int score_tracklength(const DevTally& T, V3 r0, V3 r1, V3 u, double E, double wgt,
+  double distance)
+{
+  double flux = wgt * distance;
+  int n_e = 1;
+  int e_bin = 0;
+  if (T.n_energy > 0) {
+    const double* eb = T.ebins;
+    int nb = T.n_energy + 1;
+    if (!(E >= eb[0] && E <= eb[nb - 1]))
+      return 0;
+    if (eb[0] == E) {
+      e_bin = 0;
+    } else {
+      int dummy = 0;
+      e_bin = lower_bound_idx(eb, 0, nb, E, dummy) - 1;
+    }
+    n_e = T.n_energy;
+  }
+  int segs = 0;
+  double total_distance = v_norm(v_sub(r1, r0));
+  if (total_distance == 0.0)
+    return 0;
+  bool in_mesh;
+  double traveled = 0.0;
+  int ijk[3] = {0, 0, 0};
+  mesh_indices(T, v_add(r0, v_scale(u, TINY_BIT)), ijk, in_mesh);
+  if (total_distance < 2 * TINY_BIT) {
+    if (in_mesh) {
+      double score = flux * (1.0 * 1.0);
+      tally_add(T, mesh_bin(T, ijk) * n_e + e_bin, score);
+      segs++;
+    }
+    return segs;
+  }
+  MeshDist dist[3];
+  for (int k = 0; k < 3; ++k) {
+    dist[k].next_index = -1;
+    dist[k].max_surface = true;
+    dist[k].distance = INFTY;
+  }
+  for (int k = 0; k < T.n_dim; ++k)
+    dist[k] = mesh_distance(T, ijk, k, r0, u);
+  // a straight track crosses each mesh plane at most once; the cap only
+  // guarantees termination on non-finite input
+  const int max_steps = 2 * (T.shape[0] + T.shape[1] + T.shape[2]) + 16;
+  for (int step = 0; step < max_steps; ++step) {
+    if (in_mesh) {
+      int k = 0;
+      if (dist[1].distance < dist[k].distance)
+        k = 1;
+      if (dist[2].distance < dist[k].distance)
+        k = 2;
+      double w = (fmin(dist[k].distance, total_distance) - traveled) / total_distance;
+      double weight = 1.0;
+      weight *= w;
+      weight *= 1.0;
+      tally_add(T, mesh_bin(T, ijk) * n_e + e_bin, flux * weight);
+      segs++;
+      traveled = dist[k].distance;
+      if (traveled >= total_distance)
+        return segs;
+      ijk[k] = dist[k].next_index;
+      dist[k] = mesh_distance(T, ijk, k, r0, u);
+      in_mesh = ((ijk[k] >= 1) && (ijk[k] <= T.shape[k]));
+    } else {
+      int k_max = -1;
+      for (int k = 0; k < T.n_dim; ++k) {
+        if ((ijk[k] < 1 || ijk[k] > T.shape[k]) && (dist[k].distance > traveled)) {
+          traveled = dist[k].distance;
+          k_max = k;
+        }
+      }
+      if (k_max == -1)
+        traveled += TINY_BIT;
+      if (traveled >= total_distance)
+        return segs;
+      mesh_indices(T, v_add(r0, v_scale(u, traveled + TINY_BIT)), ijk, in_mesh);
+      for (int k = 0; k < T.n_dim; ++k)
+        dist[k] = mesh_distance(T, ijk, k, r0, u);
+    }
+  }
+  return segs;
+}
+
+__device__ __forceinline__ void hmix(uint64_t& h, uint64_t v)
+{
+  h = (h ^ v) * 0x100000001b3ULL;
+}
+
+__device__ __forceinline__ uint64_t dbits(double x)
+{
+  return static_cast<uint64_t>(__double_as_longlong(x));
+}
+
+} // namespace omc

@@ -1,0 +1,810 @@
+#include "host_model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <stdexcept>
+#include <unordered_set>
+
+#include "openmc_amd/detmath.h"
+#include "xml.h"
+
+namespace omc {
+
+namespace {
+
+std::string join_path(const std::string& dir, const std::string& f)
+{
+  if (f.empty() || f[0] == '/' || dir.empty())
+    return f;
+  return dir + (dir.back() == '/' ? "" : "/") + f;
+}
+
+std::string dirname_of(const std::string& p)
+{
+  size_t k = p.find_last_of('/');
+  return k == std::string::npos ? std::string(".") : p.substr(0, k);
+}
+
+bool file_exists(const std::string& p)
+{
+  std::ifstream f(p);
+  return static_cast<bool>(f);
+}
+
+// Nuclide file reader; derived columns as Nuclide::create_derived
+// (src/nuclide.cpp:359-500): total = 0 + MT2 + MT102, absorption = 0 + MT102.
+void read_nuclide_file(HostNuclide& nuc, const std::string& path)
+{
+  std::ifstream f(path, std::ios::binary);
+  if (!f)
+    throw std::runtime_error("cannot open nuclide data file " + path);
+  char magic[8];
+  f.read(magic, 8);
+  if (std::memcmp(magic, "OMCNUC1", 7) != 0)
+    throw std::runtime_error("not a nuclide data file: " + path);
+  int32_t nlen = 0;
+  f.read(reinterpret_cast<char*>(&nlen), 4);
+  std::string name(nlen, '\0');
+  f.read(&name[0], nlen);
+  f.read(reinterpret_cast<char*>(&nuc.awr), 8);
+  int32_t nt = 0;
+  f.read(reinterpret_cast<char*>(&nt), 4);
+  if (nt < 1)
+    throw std::runtime_error("nuclide file without temperatures: " + path);
+  std::vector<double> kts(nt);
+  f.read(reinterpret_cast<char*>(kts.data()), 8 * nt);
+  nuc.kT = kts[0];
+  int32_t n = 0;
+  f.read(reinterpret_cast<char*>(&n), 4);
+  if (n < 2)
+    throw std::runtime_error("nuclide grid too short: " + path);
+  nuc.energy.resize(n);
+  std::vector<double> el(n), cap(n);
+  f.read(reinterpret_cast<char*>(nuc.energy.data()), 8 * static_cast<size_t>(n));
+  f.read(reinterpret_cast<char*>(el.data()), 8 * static_cast<size_t>(n));
+  f.read(reinterpret_cast<char*>(cap.data()), 8 * static_cast<size_t>(n));
+  if (!f)
+    throw std::runtime_error("truncated nuclide data file " + path);
+  nuc.total.assign(n, 0.0);
+  nuc.absorption.assign(n, 0.0);
+  for (int k = 0; k < n; ++k)
+    nuc.total[k] += el[k];
+  for (int k = 0; k < n; ++k) {
+    nuc.total[k] += cap[k];
+    nuc.absorption[k] += cap[k];
+  }
+  nuc.elastic = std::move(el);
+}
+
+int surface_type_code(const std::string& t)
+{
+  if (t == "x-plane")
+    return S_XPLANE;
+  if (t == "y-plane")
+    return S_YPLANE;
+  if (t == "z-plane")
+    return S_ZPLANE;
+  if (t == "plane")
+    return S_PLANE;
+  if (t == "z-cylinder")
+    return S_ZCYL;
+  if (t == "x-cylinder")
+    return S_XCYL;
+  if (t == "y-cylinder")
+    return S_YCYL;
+  if (t == "sphere")
+    return S_SPHERE;
+  throw std::runtime_error("surface type '" + t + "' is not supported by the GPU path");
+}
+
+int surface_n_coeffs(int type)
+{
+  switch (type) {
+  case S_XPLANE:
+  case S_YPLANE:
+  case S_ZPLANE:
+    return 1;
+  case S_ZCYL:
+  case S_XCYL:
+  case S_YCYL:
+    return 3;
+  default:
+    return 4;
+  }
+}
+
+// DiscreteIndex::init_alias / normalize (src/distribution.cpp:36-116)
+void init_alias(const std::vector<double>& p_in, std::vector<double>& prob, std::vector<int>& alias)
+{
+  prob = p_in;
+  double integral = 0.0;
+  for (double v : prob)
+    integral += v;
+  for (double& v : prob)
+    v /= integral;
+  size_t n = prob.size();
+  alias.assign(n, 0);
+  std::vector<size_t> large, small;
+  for (size_t i = 0; i < n; i++) {
+    prob[i] *= n;
+    if (prob[i] > 1.0)
+      large.push_back(i);
+    else
+      small.push_back(i);
+  }
+  while (!large.empty() && !small.empty()) {
+    size_t j = small.back();
+    size_t k = large.back();
+    small.pop_back();
+    prob[k] += prob[j] - 1.0;
+    alias[j] = static_cast<int>(k);
+    if (prob[k] < 1.0) {
+      small.push_back(k);
+      large.pop_back();
+    }
+  }
+}
+
+void read_settings(HostModel& m, const XmlNode& s)
+{
+  if (s.has("run_mode")) {
+    m.run_mode = s.value("run_mode");
+    if (m.run_mode != "fixed source")
+      throw std::runtime_error("run mode '" + m.run_mode + "' is outside the GPU path (fixed source only)");
+  }
+  m.n_particles = std::stoll(s.value("particles"));
+  m.n_batches = std::stoi(s.value("batches"));
+  if (s.has("inactive"))
+    m.n_inactive = std::stoi(s.value("inactive"));
+  if (s.has("seed"))
+    m.seed = static_cast<uint64_t>(std::stoll(s.value("seed")));
+  if (s.has("event_based")) {
+    std::string v = s.value("event_based");
+    m.event_based = (v == "true" || v == "1");
+  }
+  if (s.has("max_particles_in_flight"))
+    m.max_in_flight = std::stoll(s.value("max_particles_in_flight"));
+  if (s.has("log_grid_bins"))
+    m.n_log_bins = std::stoi(s.value("log_grid_bins"));
+  if (s.has("max_particle_events"))
+    m.max_events = std::stoll(s.value("max_particle_events"));
+  if (const XmlNode* c = s.child("cutoff")) {
+    if (c->has("energy_neutron"))
+      m.energy_cutoff = std::stod(c->value("energy_neutron"));
+  }
+  auto srcs = s.children_named("source");
+  if (srcs.size() != 1)
+    throw std::runtime_error("exactly one <source> is supported by the GPU path");
+  const XmlNode& src = *srcs[0];
+  if (src.attr("type") && *src.attr("type") != "independent")
+    throw std::runtime_error("only independent sources are supported by the GPU path");
+  DevSource& d = m.src;
+  std::memset(&d, 0, sizeof(d));
+  if (const XmlNode* sp = src.child("space")) {
+    std::string t = sp->value("type");
+    auto par = to_doubles(sp->value("parameters"));
+    if (t == "point") {
+      if (par.size() != 3)
+        throw std::runtime_error("point source needs three parameters");
+      d.space_type = 0;
+      for (int k = 0; k < 3; ++k)
+        d.p0[k] = par[k];
+    } else if (t == "box") {
+      if (par.size() != 6)
+        throw std::runtime_error("box source needs six parameters");
+      d.space_type = 1;
+      for (int k = 0; k < 3; ++k) {
+        d.p0[k] = par[k];
+        d.p1[k] = par[3 + k];
+      }
+    } else {
+      throw std::runtime_error("spatial distribution '" + t + "' is not supported by the GPU path");
+    }
+  } else {
+    throw std::runtime_error("source without <space> is not supported");
+  }
+  d.angle_type = 0;
+  if (const XmlNode* an = src.child("angle")) {
+    std::string t = an->value("type");
+    if (t == "isotropic") {
+      d.angle_type = 0;
+    } else if (t == "monodirectional") {
+      d.angle_type = 1;
+      auto u = to_doubles(an->value("reference_uvw"));
+      for (int k = 0; k < 3; ++k)
+        d.u_ref[k] = u[k];
+    } else {
+      throw std::runtime_error("angular distribution '" + t + "' is not supported by the GPU path");
+    }
+  }
+  d.energy_type = 1;
+  d.watt_a = 0.988e6;
+  d.watt_b = 2.249e-6;
+  if (const XmlNode* en = src.child("energy")) {
+    std::string t = en->value("type");
+    auto par = to_doubles(en->value("parameters"));
+    if (t == "discrete") {
+      d.energy_type = 0;
+      size_t n = par.size() / 2;
+      m.disc_x.assign(par.begin(), par.begin() + n);
+      m.disc_p.assign(par.begin() + n, par.end());
+      init_alias(m.disc_p, m.disc_prob, m.disc_alias);
+      d.n_discrete = static_cast<int>(n);
+    } else if (t == "watt") {
+      d.energy_type = 1;
+      d.watt_a = par.at(0);
+      d.watt_b = par.at(1);
+    } else {
+      throw std::runtime_error("energy distribution '" + t + "' is not supported by the GPU path");
+    }
+  }
+  if (const XmlNode* cn = src.child("constraints")) {
+    if (cn->has("domain_ids")) {
+      std::string dt = cn->has("domain_type") ? cn->value("domain_type") : "cell";
+      if (dt != "cell")
+        throw std::runtime_error("only cell domain constraints are supported by the GPU path");
+      for (long long v : to_ints(cn->value("domain_ids")))
+        m.domain_ids.push_back(static_cast<int>(v));
+    }
+  }
+}
+
+void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_dir)
+{
+  std::string xs_path;
+  if (mats.has("cross_sections"))
+    xs_path = join_path(model_dir, mats.value("cross_sections"));
+  else if (const char* env = std::getenv("OPENMC_CROSS_SECTIONS"))
+    xs_path = env;
+  else
+    throw std::runtime_error("no cross_sections.xml given (materials <cross_sections> or OPENMC_CROSS_SECTIONS)");
+
+  struct RawMat {
+    int units;
+    double value;
+    std::vector<int> nuc;
+    std::vector<double> pct;
+  };
+  std::vector<RawMat> raw;
+  for (const XmlNode* mn : mats.children_named("material")) {
+    RawMat r {};
+    m.mat_ids.push_back(std::stoi(mn->value("id")));
+    const XmlNode* dn = mn->child("density");
+    if (!dn)
+      throw std::runtime_error("material without <density>");
+    std::string units = dn->value("units");
+    if (units == "sum") {
+      r.units = 0;
+    } else {
+      r.value = std::stod(dn->value("value"));
+      if (r.value <= 0.0)
+        throw std::runtime_error("need a positive density");
+      if (units == "atom/b-cm")
+        r.units = 1;
+      else if (units == "atom/cc" || units == "atom/cm3")
+        r.units = 2;
+      else if (units == "g/cc" || units == "g/cm3")
+        r.units = 3;
+      else if (units == "kg/m3")
+        r.units = 4;
+      else
+        throw std::runtime_error("unsupported density units '" + units + "'");
+    }
+    for (const XmlNode* nn : mn->children_named("nuclide")) {
+      std::string name = nn->value("name");
+      auto it = m.nuclide_map.find(name);
+      int idx;
+      if (it == m.nuclide_map.end()) {
+        idx = static_cast<int>(m.nuclide_map.size());
+        m.nuclide_map[name] = idx;
+      } else {
+        idx = it->second;
+      }
+      r.nuc.push_back(idx);
+      if (nn->attr("ao"))
+        r.pct.push_back(std::stod(*nn->attr("ao")));
+      else if (nn->attr("wo"))
+        r.pct.push_back(-std::stod(*nn->attr("wo")));
+      else
+        throw std::runtime_error("nuclide without ao/wo in material");
+    }
+    if (r.nuc.empty())
+      throw std::runtime_error("material without nuclides");
+    raw.push_back(std::move(r));
+  }
+
+  // cross_sections.xml (src/cross_sections.cpp:45-89)
+  auto lib = parse_xml_file(xs_path);
+  std::string lib_dir = dirname_of(xs_path);
+  if (lib->has("directory"))
+    lib_dir = join_path(lib_dir, lib->value("directory"));
+  std::unordered_map<std::string, std::string> paths;
+  for (const XmlNode* ln : lib->children_named("library")) {
+    std::string type = ln->has("type") ? ln->value("type") : "neutron";
+    if (type != "neutron")
+      continue;
+    std::string p = join_path(lib_dir, ln->value("path"));
+    for (const auto& name : split_ws(ln->value("materials")))
+      paths[name] = p;
+  }
+  m.nuclides.resize(m.nuclide_map.size());
+  for (const auto& kv : m.nuclide_map) {
+    auto it = paths.find(kv.first);
+    if (it == paths.end())
+      throw std::runtime_error("nuclide " + kv.first + " is not in the cross section library");
+    read_nuclide_file(m.nuclides[kv.second], it->second);
+    m.nuclides[kv.second].name = kv.first;
+  }
+
+  // Material::normalize_density (src/material.cpp:417-467)
+  m.mat_off.push_back(0);
+  for (auto& r : raw) {
+    double density;
+    double sum = 0.0;
+    for (double a : r.pct)
+      sum += a;
+    switch (r.units) {
+    case 0: density = sum; break;
+    case 1: density = r.value; break;
+    case 2: density = 1.0e-24 * r.value; break;
+    case 3: density = -r.value; break;
+    default: density = -1.0e-3 * r.value; break;
+    }
+    std::vector<double>& a = r.pct;
+    bool percent_in_atom = (a[0] >= 0.0);
+    bool density_in_atom = (density >= 0.0);
+    for (size_t i = 0; i < a.size(); ++i)
+      if (!percent_in_atom)
+        a[i] = -a[i] / m.nuclides[r.nuc[i]].awr;
+    double s = 0.0;
+    for (double v : a)
+      s += v;
+    for (double& v : a)
+      v /= s;
+    if (!density_in_atom) {
+      double sum_percent = 0.0;
+      for (size_t i = 0; i < a.size(); ++i)
+        sum_percent += a[i] * m.nuclides[r.nuc[i]].awr;
+      sum_percent = 1.0 / sum_percent;
+      density = -density * N_AVOGADRO / MASS_NEUTRON * sum_percent;
+    }
+    for (double& v : a)
+      v *= density;
+    for (size_t i = 0; i < a.size(); ++i) {
+      m.mat_nuc.push_back(r.nuc[i]);
+      m.mat_dens.push_back(a[i]);
+    }
+    m.mat_off.push_back(static_cast<int>(m.mat_nuc.size()));
+  }
+}
+
+void read_geometry(HostModel& m, const XmlNode& g)
+{
+  std::unordered_map<int, int> surf_map, mat_map, univ_map, lat_map;
+  for (size_t k = 0; k < m.mat_ids.size(); ++k)
+    mat_map[m.mat_ids[k]] = static_cast<int>(k);
+
+  for (const XmlNode* sn : g.children_named("surface")) {
+    DevSurface s {};
+    int id = std::stoi(sn->value("id"));
+    s.type = surface_type_code(sn->value("type"));
+    auto c = to_doubles(sn->value("coeffs"));
+    if (static_cast<int>(c.size()) != surface_n_coeffs(s.type))
+      throw std::runtime_error("wrong number of coefficients on surface " + std::to_string(id));
+    for (size_t k = 0; k < c.size(); ++k)
+      s.c[k] = c[k];
+    s.bc = BC_TRANSMIT;
+    if (sn->has("boundary")) {
+      std::string b = sn->value("boundary");
+      if (b == "vacuum")
+        s.bc = BC_VACUUM;
+      else if (b == "reflective" || b == "reflect" || b == "reflecting")
+        s.bc = BC_REFLECT;
+      else if (b == "transmission" || b == "transmit")
+        s.bc = BC_TRANSMIT;
+      else
+        throw std::runtime_error("boundary condition '" + b + "' is not supported by the GPU path");
+    }
+    surf_map[id] = static_cast<int>(m.surfaces.size());
+    m.surf_ids.push_back(id);
+    m.surfaces.push_back(s);
+  }
+
+  struct RawLat {
+    int id;
+    std::vector<int> n;
+    std::vector<double> ll, pitch;
+    int outer;
+    std::vector<long long> uids;
+  };
+  std::vector<RawLat> raw_lats;
+  for (const XmlNode* ln : g.children_named("lattice")) {
+    RawLat r;
+    r.id = std::stoi(ln->value("id"));
+    for (long long v : to_ints(ln->value("dimension")))
+      r.n.push_back(static_cast<int>(v));
+    if (r.n.size() != 2 && r.n.size() != 3)
+      throw std::runtime_error("rectangular lattice must be 2-D or 3-D");
+    r.ll = to_doubles(ln->value("lower_left"));
+    r.pitch = to_doubles(ln->value("pitch"));
+    r.outer = ln->has("outer") ? std::stoi(ln->value("outer")) : -1;
+    r.uids = to_ints(ln->value("universes"));
+    lat_map[r.id] = static_cast<int>(raw_lats.size());
+    raw_lats.push_back(std::move(r));
+  }
+  if (!g.children_named("hex_lattice").empty())
+    throw std::runtime_error("hexagonal lattices are outside the GPU path");
+
+  struct RawCell {
+    int fill_id;
+    bool has_fill;
+  };
+  std::vector<RawCell> raw_cells;
+  for (const XmlNode* cn : g.children_named("cell")) {
+    DevCell c {};
+    c.id = std::stoi(cn->value("id"));
+    int uid = cn->has("universe") ? std::stoi(cn->value("universe")) : 0;
+    auto it = univ_map.find(uid);
+    if (it == univ_map.end()) {
+      univ_map[uid] = static_cast<int>(m.univ_ids.size());
+      m.univ_ids.push_back(uid);
+    }
+    c.universe = univ_map[uid];
+    RawCell rc {0, false};
+    if (cn->has("fill")) {
+      rc.has_fill = true;
+      rc.fill_id = std::stoi(cn->value("fill"));
+      c.material = C_NONE;
+    } else {
+      std::string ms = cn->value("material");
+      auto words = split_ws(ms);
+      if (words.size() != 1)
+        throw std::runtime_error("distributed materials are outside the GPU path");
+      if (words[0] == "void") {
+        c.material = MATERIAL_VOID;
+      } else {
+        int mid = std::stoi(words[0]);
+        auto mt = mat_map.find(mid);
+        if (mt == mat_map.end())
+          throw std::runtime_error("cell " + std::to_string(c.id) + " uses unknown material");
+        c.material = mt->second;
+      }
+      c.fill_type = FILL_MATERIAL;
+      c.fill = C_NONE;
+    }
+    if (cn->has("translation") || cn->has("rotation"))
+      throw std::runtime_error("cell translation/rotation is outside the GPU path");
+    c.tok_begin = static_cast<int>(m.tokens.size());
+    if (cn->has("region")) {
+      std::string reg = cn->value("region");
+      if (reg.find_first_of("|~()") != std::string::npos)
+        throw std::runtime_error("only intersection regions are supported by the GPU path");
+      for (long long t : to_ints(reg)) {
+        int sid = static_cast<int>(t < 0 ? -t : t);
+        auto st = surf_map.find(sid);
+        if (st == surf_map.end())
+          throw std::runtime_error("region refers to unknown surface " + std::to_string(sid));
+        int idx = st->second + 1;
+        m.tokens.push_back(t > 0 ? idx : -idx);
+      }
+    }
+    c.tok_end = static_cast<int>(m.tokens.size());
+    m.cells.push_back(c);
+    raw_cells.push_back(rc);
+  }
+
+  // Universe cell lists in cell order (src/geometry_aux.cpp)
+  m.universes.assign(m.univ_ids.size(), DevUniverse {0, 0});
+  {
+    std::vector<std::vector<int>> lists(m.univ_ids.size());
+    for (size_t k = 0; k < m.cells.size(); ++k)
+      lists[m.cells[k].universe].push_back(static_cast<int>(k));
+    for (size_t u = 0; u < lists.size(); ++u) {
+      m.universes[u].cell_begin = static_cast<int>(m.univ_cells.size());
+      for (int c : lists[u])
+        m.univ_cells.push_back(c);
+      m.universes[u].cell_end = static_cast<int>(m.univ_cells.size());
+    }
+  }
+
+  // Resolve fills
+  for (size_t k = 0; k < m.cells.size(); ++k) {
+    if (!raw_cells[k].has_fill)
+      continue;
+    int fid = raw_cells[k].fill_id;
+    if (univ_map.count(fid)) {
+      m.cells[k].fill_type = FILL_UNIVERSE;
+      m.cells[k].fill = univ_map[fid];
+    } else if (lat_map.count(fid)) {
+      m.cells[k].fill_type = FILL_LATTICE;
+      m.cells[k].fill = lat_map[fid];
+    } else {
+      throw std::runtime_error("cell fill " + std::to_string(fid) + " is neither a universe nor a lattice");
+    }
+  }
+
+  // Lattices (src/lattice.cpp:160-245)
+  for (const auto& r : raw_lats) {
+    DevLattice L {};
+    L.is_3d = r.n.size() == 3 ? 1 : 0;
+    L.n[0] = r.n[0];
+    L.n[1] = r.n[1];
+    L.n[2] = L.is_3d ? r.n[2] : 1;
+    for (size_t k = 0; k < r.n.size(); ++k) {
+      L.ll[k] = r.ll.at(k);
+      L.pitch[k] = r.pitch.at(k);
+    }
+    int cnt = L.n[0] * L.n[1] * L.n[2];
+    if (static_cast<int>(r.uids.size()) != cnt)
+      throw std::runtime_error("lattice " + std::to_string(r.id) + " has the wrong number of universes");
+    L.univ_offset = static_cast<int>(m.lat_univ.size());
+    m.lat_univ.resize(m.lat_univ.size() + cnt, C_NONE);
+    for (int iz = 0; iz < L.n[2]; iz++)
+      for (int iy = L.n[1] - 1; iy > -1; iy--)
+        for (int ix = 0; ix < L.n[0]; ix++) {
+          int indx1 = L.n[0] * L.n[1] * iz + L.n[0] * (L.n[1] - iy - 1) + ix;
+          int indx2 = L.n[0] * L.n[1] * iz + L.n[0] * iy + ix;
+          int uid = static_cast<int>(r.uids[indx2]);
+          if (!univ_map.count(uid))
+            throw std::runtime_error("lattice refers to unknown universe " + std::to_string(uid));
+          m.lat_univ[L.univ_offset + indx1] = univ_map[uid];
+        }
+    L.outer = C_NONE;
+    if (r.outer >= 0) {
+      if (!univ_map.count(r.outer))
+        throw std::runtime_error("lattice outer universe unknown");
+      L.outer = univ_map[r.outer];
+    }
+    m.lat_ids.push_back(r.id);
+    m.lattices.push_back(L);
+  }
+
+  // find_root_universe (src/geometry_aux.cpp:284-320)
+  std::unordered_set<int> filled;
+  for (const auto& c : m.cells)
+    if (c.fill_type == FILL_UNIVERSE)
+      filled.insert(c.fill);
+  for (const auto& L : m.lattices) {
+    for (int k = 0; k < L.n[0] * L.n[1] * L.n[2]; ++k)
+      filled.insert(m.lat_univ[L.univ_offset + k]);
+    if (L.outer != C_NONE)
+      filled.insert(L.outer);
+  }
+  int root = -1;
+  for (int u = 0; u < static_cast<int>(m.universes.size()); ++u) {
+    if (!filled.count(u)) {
+      if (root >= 0)
+        throw std::runtime_error("two or more universes are not used as fills");
+      root = u;
+    }
+  }
+  if (root < 0)
+    throw std::runtime_error("could not find a root universe");
+  m.root_universe = root;
+
+  // maximum_levels (src/geometry_aux.cpp:591-625)
+  std::vector<int> memo(m.universes.size(), -1);
+  std::function<int(int)> levels = [&](int u) -> int {
+    if (memo[u] >= 0)
+      return memo[u];
+    int below = 0;
+    for (int k = m.universes[u].cell_begin; k < m.universes[u].cell_end; ++k) {
+      const DevCell& c = m.cells[m.univ_cells[k]];
+      if (c.fill_type == FILL_UNIVERSE) {
+        below = std::max(below, levels(c.fill));
+      } else if (c.fill_type == FILL_LATTICE) {
+        const DevLattice& L = m.lattices[c.fill];
+        for (int j = 0; j < L.n[0] * L.n[1] * L.n[2]; ++j)
+          below = std::max(below, levels(m.lat_univ[L.univ_offset + j]));
+        if (L.outer != C_NONE)
+          below = std::max(below, levels(L.outer));
+      }
+    }
+    memo[u] = 1 + below;
+    return memo[u];
+  };
+  m.n_levels = levels(root);
+  if (m.n_levels > MAX_LEVELS)
+    throw std::runtime_error("geometry deeper than the GPU path's coordinate levels");
+
+  // cell-domain ids must exist
+  for (int id : m.domain_ids) {
+    bool ok = false;
+    for (const auto& c : m.cells)
+      ok = ok || c.id == id;
+    if (!ok)
+      throw std::runtime_error("source domain refers to unknown cell " + std::to_string(id));
+  }
+}
+
+void read_tallies(HostModel& m, const XmlNode& t)
+{
+  struct Mesh {
+    int n_dim;
+    int shape[3];
+    double ll[3], ur[3], width[3];
+  };
+  std::unordered_map<int, Mesh> meshes;
+  for (const XmlNode* mn : t.children_named("mesh")) {
+    if (mn->attr("type") && *mn->attr("type") != "regular")
+      throw std::runtime_error("only regular meshes are supported by the GPU path");
+    Mesh me {};
+    auto dim = to_ints(mn->value("dimension"));
+    me.n_dim = static_cast<int>(dim.size());
+    if (me.n_dim < 1 || me.n_dim > 3)
+      throw std::runtime_error("mesh must be one, two or three dimensional");
+    auto ll = to_doubles(mn->value("lower_left"));
+    std::vector<double> ur;
+    if (mn->has("upper_right")) {
+      ur = to_doubles(mn->value("upper_right"));
+    } else {
+      auto w = to_doubles(mn->value("width"));
+      for (int k = 0; k < me.n_dim; ++k)
+        ur.push_back(ll[k] + static_cast<double>(dim[k]) * w[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      me.shape[k] = k < me.n_dim ? static_cast<int>(dim[k]) : 1;
+      me.ll[k] = k < me.n_dim ? ll[k] : 0.0;
+      me.ur[k] = k < me.n_dim ? ur[k] : 0.0;
+      // RegularMesh::set_grid: width = (upper_right - lower_left) / shape
+      me.width[k] = k < me.n_dim ? (me.ur[k] - me.ll[k]) / static_cast<double>(me.shape[k]) : 0.0;
+    }
+    meshes[std::stoi(mn->value("id"))] = me;
+  }
+  struct Filt {
+    std::string type;
+    std::vector<double> bins;
+    int mesh;
+  };
+  std::unordered_map<int, Filt> filters;
+  for (const XmlNode* fn : t.children_named("filter")) {
+    Filt f;
+    f.type = fn->value("type");
+    f.mesh = -1;
+    if (f.type == "mesh") {
+      f.mesh = std::stoi(fn->value("bins"));
+    } else if (f.type == "energy") {
+      f.bins = to_doubles(fn->value("bins"));
+    } else {
+      throw std::runtime_error("filter type '" + f.type + "' is outside the GPU path");
+    }
+    filters[std::stoi(fn->value("id"))] = f;
+  }
+  for (const XmlNode* tn : t.children_named("tally")) {
+    HostTally ht;
+    ht.id = std::stoi(tn->value("id"));
+    auto scores = split_ws(tn->value("scores"));
+    if (scores.size() != 1 || scores[0] != "flux")
+      throw std::runtime_error("only the flux score is supported by the GPU path");
+    if (tn->has("nuclides")) {
+      auto nn = split_ws(tn->value("nuclides"));
+      if (!(nn.size() == 1 && nn[0] == "total"))
+        throw std::runtime_error("per-nuclide tallies are outside the GPU path");
+    }
+    if (tn->has("estimator")) {
+      ht.estimator = tn->value("estimator");
+      if (ht.estimator != "tracklength")
+        throw std::runtime_error("only the tracklength estimator is supported by the GPU path");
+    }
+    int n_mesh = 0, n_energy = 0;
+    if (tn->has("filters")) {
+      for (long long fid : to_ints(tn->value("filters"))) {
+        auto it = filters.find(static_cast<int>(fid));
+        if (it == filters.end())
+          throw std::runtime_error("tally refers to unknown filter");
+        const Filt& f = it->second;
+        ht.filter_ids.push_back(static_cast<int>(fid));
+        ht.filter_types.push_back(f.type);
+        if (f.type == "mesh") {
+          if (n_mesh++ || !ht.filter_types.empty() && ht.filter_types.size() != 1)
+            throw std::runtime_error("the GPU path supports filters [mesh] or [mesh, energy]");
+          auto mt = meshes.find(f.mesh);
+          if (mt == meshes.end())
+            throw std::runtime_error("mesh filter refers to unknown mesh");
+          const Mesh& me = mt->second;
+          ht.has_mesh = true;
+          ht.mesh_id = f.mesh;
+          ht.n_dim = me.n_dim;
+          for (int k = 0; k < 3; ++k) {
+            ht.shape[k] = me.shape[k];
+            ht.ll[k] = me.ll[k];
+            ht.ur[k] = me.ur[k];
+            ht.width[k] = me.width[k];
+          }
+          int64_t nb = 1;
+          for (int k = 0; k < me.n_dim; ++k)
+            nb *= me.shape[k];
+          ht.filter_nbins.push_back(static_cast<int>(nb));
+        } else {
+          if (n_energy++ || n_mesh != 1)
+            throw std::runtime_error("the GPU path supports filters [mesh] or [mesh, energy]");
+          ht.ebins = f.bins;
+          ht.filter_nbins.push_back(static_cast<int>(f.bins.size()) - 1);
+        }
+      }
+    }
+    if (!ht.has_mesh)
+      throw std::runtime_error("the GPU path needs a mesh filter on every tally");
+    ht.n_filter_bins = 1;
+    for (int nb : ht.filter_nbins)
+      ht.n_filter_bins *= nb;
+    m.tallies.push_back(ht);
+  }
+  if (m.tallies.size() > 1)
+    throw std::runtime_error("the GPU path supports one tally per model");
+}
+
+} // namespace
+
+void read_model(HostModel& m, const std::string& dir)
+{
+  std::string model_path = join_path(dir, "model.xml");
+  std::unique_ptr<XmlNode> model_doc, set_doc, mat_doc, geo_doc, tal_doc;
+  const XmlNode *settings = nullptr, *materials = nullptr, *geometry = nullptr, *tallies = nullptr;
+  if (file_exists(model_path)) {
+    model_doc = parse_xml_file(model_path);
+    settings = model_doc->child("settings");
+    materials = model_doc->child("materials");
+    geometry = model_doc->child("geometry");
+    tallies = model_doc->child("tallies");
+  } else {
+    set_doc = parse_xml_file(join_path(dir, "settings.xml"));
+    mat_doc = parse_xml_file(join_path(dir, "materials.xml"));
+    geo_doc = parse_xml_file(join_path(dir, "geometry.xml"));
+    settings = set_doc.get();
+    materials = mat_doc.get();
+    geometry = geo_doc.get();
+    if (file_exists(join_path(dir, "tallies.xml"))) {
+      tal_doc = parse_xml_file(join_path(dir, "tallies.xml"));
+      tallies = tal_doc.get();
+    }
+  }
+  if (!settings || !materials || !geometry)
+    throw std::runtime_error("model needs <settings>, <materials> and <geometry>");
+  read_settings(m, *settings);
+  read_materials(m, *materials, dir);
+  read_geometry(m, *geometry);
+  if (tallies)
+    read_tallies(m, *tallies);
+}
+
+void initialize_data(HostModel& m)
+{
+  m.energy_max = INFTY;
+  m.energy_min = 0.0;
+  for (const auto& nuc : m.nuclides) {
+    m.energy_min = std::max(m.energy_min, nuc.energy.front());
+    m.energy_max = std::min(m.energy_max, nuc.energy.back());
+  }
+  const int M = m.n_log_bins;
+  double spacing = detm::log(m.energy_max / m.energy_min) / M;
+  // xt::linspace(0, M*spacing, M+1) element k: 0 + k * ((stop - 0) / M)
+  double stop = M * spacing;
+  double step = (stop - 0.0) / static_cast<double>(M);
+  m.grid_index.assign(m.nuclides.size() * static_cast<size_t>(M + 1), 0);
+  for (size_t i = 0; i < m.nuclides.size(); ++i) {
+    const auto& E = m.nuclides[i].energy;
+    int* gi = &m.grid_index[i * static_cast<size_t>(M + 1)];
+    int j = 0;
+    for (int k = 0; k <= M; ++k) {
+      double umesh_k = 0.0 + step * static_cast<double>(k);
+      while (detm::log(E[j + 1] / m.energy_min) <= umesh_k) {
+        if (j + 2 == static_cast<int>(E.size()))
+          break;
+        ++j;
+      }
+      gi[k] = j;
+    }
+  }
+  m.log_spacing = detm::log(m.energy_max / m.energy_min) / m.n_log_bins;
+
+  for (double e : m.disc_x)
+    if (e > m.energy_max)
+      throw std::runtime_error("Source energy above range of energies of at least one cross section table");
+}
+
+} // namespace omc

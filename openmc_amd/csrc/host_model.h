@@ -1,0 +1,93 @@
+// Host-side model: the result of openmc_init reading model.xml, the
+// cross-section library and the nuclide files, flattened into the arrays the
+// device copies of DevProblem point at.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "dev_types.h"
+
+namespace omc {
+
+struct HostNuclide {
+  std::string name;
+  double awr = 0;
+  double kT = 0;
+  std::vector<double> energy, total, absorption, elastic;
+};
+
+struct HostTally {
+  int id = 0;
+  bool has_mesh = false;
+  int mesh_id = 0;
+  int n_dim = 3;
+  int shape[3] = {1, 1, 1};
+  double ll[3] = {0, 0, 0}, ur[3] = {0, 0, 0}, width[3] = {0, 0, 0};
+  std::vector<double> ebins;
+  std::vector<int> filter_ids;
+  std::vector<std::string> filter_types;
+  std::vector<int> filter_nbins;
+  int64_t n_filter_bins = 1;
+  std::string estimator = "tracklength";
+};
+
+struct HostModel {
+  // settings
+  int64_t n_particles = 0;
+  int n_batches = 0;
+  int n_inactive = 0;
+  uint64_t seed = 1;
+  uint64_t stride = 152917ULL;
+  bool event_based = false;
+  int64_t max_in_flight = 0; // 0 = let the GPU size it
+  int n_log_bins = 8000;
+  int64_t max_events = 1000000;
+  double free_gas_threshold = 400.0;
+  double energy_cutoff = 0.0;
+  std::string run_mode = "fixed source";
+
+  std::vector<HostNuclide> nuclides;
+  std::unordered_map<std::string, int> nuclide_map;
+
+  std::vector<int> mat_ids;
+  std::vector<int> mat_off;
+  std::vector<int> mat_nuc;
+  std::vector<double> mat_dens;
+
+  std::vector<int> surf_ids;
+  std::vector<DevSurface> surfaces;
+  std::vector<DevCell> cells;
+  std::vector<int> tokens;
+  std::vector<int> univ_ids;
+  std::vector<DevUniverse> universes;
+  std::vector<int> univ_cells;
+  std::vector<int> lat_ids;
+  std::vector<DevLattice> lattices;
+  std::vector<int> lat_univ;
+  int root_universe = 0;
+  int n_levels = 1;
+
+  // source
+  DevSource src {};
+  std::vector<double> disc_x, disc_p, disc_prob;
+  std::vector<int> disc_alias, domain_ids;
+
+  std::vector<HostTally> tallies;
+
+  double energy_min = 0, energy_max = 0, log_spacing = 0;
+  std::vector<int> grid_index; // [n_nuc * (n_log_bins + 1)]
+};
+
+// Reads <cwd>/model.xml (or the separate settings/materials/geometry/tallies
+// files) exactly as openmc_init's read_model_xml does (src/initialize.cpp:360-470).
+void read_model(HostModel& m, const std::string& dir);
+
+// initialize_data + Nuclide::init_grid (src/simulation.cpp:682-761, src/nuclide.cpp:502-533)
+void initialize_data(HostModel& m);
+
+} // namespace omc

@@ -1,0 +1,122 @@
+// Batched cross-section lookups outside the transport loop.
+//
+// k_lookup_batch: one nuclide, many energies -> grid index and (total,
+// absorption), i.e. Nuclide::calculate_xs (src/nuclide.cpp:616-839) applied
+// to an array; used by the parity tests and the lookup-rate measurement.
+// k_macro_batch: one material, many energies -> macroscopic (total,
+// absorption), i.e. Material::calculate_neutron_xs (src/material.cpp:829-901).
+#include <hip/hip_runtime.h>
+
+#include "device_physics.h"
+#include "transport.h"
+
+namespace omc {
+
+constexpr int LB_BLOCK = 256;
+
+__device__ __forceinline__ int clamp_log_index(const DevProblem& P, double lx)
+{
+  int i_log = static_cast<int>(lx / P.log_spacing);
+  return i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+}
+
+__global__ void __launch_bounds__(LB_BLOCK) k_lookup_batch(DevProblem P, int i_nuc, const double* E, int64_t n,
+  int learned, int* i_grid, double* xs, unsigned long long* probes_out)
+{
+  unsigned long long probes_sum = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double e = E[i];
+    double lx = detm::log(e / P.energy_min);
+    int i_log = clamp_log_index(P, lx);
+    int probes = 0;
+    Micro m = nuclide_xs(P, i_nuc, e, i_log, lx, learned != 0, probes);
+    i_grid[i] = m.i_grid;
+    xs[2 * i] = m.total;
+    xs[2 * i + 1] = m.absorption;
+    probes_sum += probes;
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    probes_sum += __shfl_xor(probes_sum, o);
+  if (probes_out && __lane_id() == 0 && probes_sum)
+    atomicAdd(probes_out, probes_sum);
+}
+
+__global__ void __launch_bounds__(LB_BLOCK) k_macro_batch(DevProblem P, int material, const double* E, int64_t n,
+  int learned, double* macro, unsigned long long* probes_out)
+{
+  unsigned long long probes_sum = 0;
+  const int j0 = P.mat_off[material];
+  const int j1 = P.mat_off[material + 1];
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double e = E[i];
+    double lx = detm::log(e / P.energy_min);
+    int i_log = clamp_log_index(P, lx);
+    int probes = 0;
+    double xs_t = 0.0, xs_a = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      Micro m = nuclide_xs(P, P.mat_nuc[j], e, i_log, lx, learned != 0, probes);
+      const double dens = P.mat_dens[j] * 1.0;
+      xs_t += dens * m.total;
+      xs_a += dens * m.absorption;
+    }
+    macro[2 * i] = xs_t;
+    macro[2 * i + 1] = xs_a;
+    probes_sum += probes;
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    probes_sum += __shfl_xor(probes_sum, o);
+  if (probes_out && __lane_id() == 0 && probes_sum)
+    atomicAdd(probes_out, probes_sum);
+}
+
+// Elementwise primitives of the histories, for bitwise host/device checks:
+// 0 log, 1 sin, 2 cos, 3 sqrt, 4 x / y, 5 prn(seed = bits of x), 6 u64 -> f64 of bits of x
+__global__ void k_math_batch(int fn, const double* x, const double* y, int64_t n, double* out)
+{
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double v = x[i];
+    double r;
+    switch (fn) {
+    case 0: r = detm::log(v); break;
+    case 1: r = detm::sin(v); break;
+    case 2: r = detm::cos(v); break;
+    case 3: r = sqrt(v); break;
+    case 4: r = v / y[i]; break;
+    case 5: {
+      uint64_t sd = static_cast<uint64_t>(__double_as_longlong(v));
+      r = prn(&sd);
+      break;
+    }
+    default: r = static_cast<double>(static_cast<uint64_t>(__double_as_longlong(v))); break;
+    }
+    out[i] = r;
+  }
+}
+
+void launch_math_batch(int fn, const double* x, const double* y, int64_t n, double* out, hipStream_t st)
+{
+  int64_t b = (n + LB_BLOCK - 1) / LB_BLOCK;
+  int blocks = static_cast<int>(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+  hipLaunchKernelGGL(k_math_batch, dim3(blocks), dim3(LB_BLOCK), 0, st, fn, x, y, n, out);
+}
+
+void launch_lookup_batch(const DevProblem& P, int i_nuc, const double* E, int64_t n, int learned, int* i_grid,
+  double* xs, unsigned long long* probes, hipStream_t st)
+{
+  int64_t b = (n + LB_BLOCK - 1) / LB_BLOCK;
+  int blocks = static_cast<int>(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+  hipLaunchKernelGGL(k_lookup_batch, dim3(blocks), dim3(LB_BLOCK), 0, st, P, i_nuc, E, n, learned, i_grid, xs, probes);
+}
+
+void launch_macro_batch(const DevProblem& P, int material, const double* E, int64_t n, int learned, double* macro,
+  unsigned long long* probes, hipStream_t st)
+{
+  int64_t b = (n + LB_BLOCK - 1) / LB_BLOCK;
+  int blocks = static_cast<int>(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+  hipLaunchKernelGGL(k_macro_batch, dim3(blocks), dim3(LB_BLOCK), 0, st, P, material, E, n, learned, macro, probes);
+}
+
+} // namespace omc

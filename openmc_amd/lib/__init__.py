@@ -1,0 +1,423 @@
+"""ctypes bindings to this build's libopenmc.so, mirroring ``openmc.lib``.
+
+Same entry points, argument meanings and error behaviour as the reference's
+bindings (openmc/lib/__init__.py, core.py, tally.py, settings.py, error.py):
+``init``/``run``/``next_batch``/``finalize``, ``settings.event_based`` and
+friends as data descriptors over the exported globals, ``tallies[id].mean`` /
+``.std_dev`` computed from the zero-copy results array, and error codes mapped
+to exceptions.  Extra functions specific to the GPU path (options, run
+statistics, history hashes, batched lookups, learned index) are prefixed
+nothing special but documented as build-specific.
+
+Importing this module loads libopenmc.so and fails loudly if it is missing;
+nothing here falls back to a CPU implementation.
+"""
+import os
+from contextlib import contextmanager
+from ctypes import (CDLL, POINTER, c_bool, c_char, c_char_p, c_double, c_float, c_int, c_int32,
+                    c_int64, c_size_t, c_uint64, c_void_p, create_string_buffer)
+
+import numpy as np
+
+_filename = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libopenmc.so")
+if not os.path.exists(_filename):
+    raise ImportError("libopenmc.so has not been built (run __graft_entry__.build())")
+_dll = CDLL(_filename)
+
+
+class OpenMCError(Exception):
+    pass
+
+
+class AllocationError(OpenMCError):
+    pass
+
+
+class OutOfBoundsError(OpenMCError):
+    pass
+
+
+class InvalidArgumentError(OpenMCError):
+    pass
+
+
+class InvalidTypeError(OpenMCError):
+    pass
+
+
+class InvalidIDError(OpenMCError):
+    pass
+
+
+class GeometryError(OpenMCError):
+    pass
+
+
+class DataError(OpenMCError):
+    pass
+
+
+class PhysicsError(OpenMCError):
+    pass
+
+
+def _errcode(s):
+    return c_int.in_dll(_dll, s).value
+
+
+def _error_handler(err, func, args):
+    """Raise the exception matching the error code (openmc/lib/error.py:8-41)."""
+    msg = (c_char * 256).in_dll(_dll, "openmc_err_msg").value.decode()
+    table = [("OPENMC_E_ALLOCATE", AllocationError), ("OPENMC_E_OUT_OF_BOUNDS", OutOfBoundsError),
+             ("OPENMC_E_INVALID_ARGUMENT", InvalidArgumentError), ("OPENMC_E_INVALID_TYPE", InvalidTypeError),
+             ("OPENMC_E_INVALID_ID", InvalidIDError), ("OPENMC_E_GEOMETRY", GeometryError),
+             ("OPENMC_E_DATA", DataError), ("OPENMC_E_PHYSICS", PhysicsError)]
+    for name, exc in table:
+        if err == _errcode(name):
+            raise exc(msg)
+    if err < 0:
+        raise OpenMCError(msg or "Unknown error encountered (code %d)." % err)
+    return err
+
+
+def _bind(name, argtypes, restype=c_int, check=True):
+    f = getattr(_dll, name)
+    f.argtypes = argtypes
+    f.restype = restype
+    if check and restype is c_int:
+        f.errcheck = _error_handler
+    return f
+
+
+_bind("openmc_init", [c_int, POINTER(POINTER(c_char)), c_void_p])
+_bind("openmc_run", [])
+_bind("openmc_simulation_init", [])
+_bind("openmc_simulation_finalize", [])
+_bind("openmc_next_batch", [POINTER(c_int)])
+_bind("openmc_finalize", [])
+_bind("openmc_reset", [])
+_bind("openmc_hard_reset", [])
+_bind("openmc_reset_timers", [])
+_bind("openmc_get_tally_index", [c_int32, POINTER(c_int32)])
+_bind("openmc_tally_get_id", [c_int32, POINTER(c_int32)])
+_bind("openmc_tally_get_n_realizations", [c_int32, POINTER(c_int32)])
+_bind("openmc_tally_results", [c_int32, POINTER(POINTER(c_double)), POINTER(c_size_t)])
+_bind("openmc_global_tallies", [POINTER(POINTER(c_double))])
+_bind("tallies_size", [], c_size_t, check=False)
+_bind("openmc_get_seed", [], c_int64, check=False)
+_bind("openmc_set_seed", [c_int64], None, check=False)
+_bind("openmc_get_stride", [], c_uint64, check=False)
+_bind("openmc_set_stride", [c_uint64], None, check=False)
+_bind("omc_set_option", [c_char_p, c_double])
+_bind("omc_get_stats", [POINTER(c_double), c_int], c_int, check=False)
+_bind("omc_stat_names", [], c_char_p, check=False)
+_bind("omc_particle_hashes", [POINTER(POINTER(c_uint64)), POINTER(c_int64)])
+_bind("omc_energy_grid", [POINTER(c_double)])
+_bind("omc_n_nuclides", [], c_int, check=False)
+_bind("omc_grid_index", [c_int], POINTER(c_int), check=False)
+_bind("omc_lookup_batch", [c_int, POINTER(c_double), c_int64, c_int, POINTER(c_int), POINTER(c_double),
+                           POINTER(c_uint64)])
+_bind("omc_macro_batch", [c_int, POINTER(c_double), c_int64, c_int, c_int, POINTER(c_double),
+                          POINTER(c_double)])
+_bind("omc_math_batch", [c_int, POINTER(c_double), POINTER(c_double), c_int64, POINTER(c_double)])
+_bind("omc_check_model", [c_char_p, POINTER(c_double), c_int, POINTER(c_int), c_int64], c_int, check=False)
+_bind("omc_learned_build", [c_int, c_int, POINTER(c_int)])
+_bind("omc_learned_loss_grad", [c_int, c_int, POINTER(c_float), POINTER(c_double), POINTER(c_double)])
+_bind("omc_learned_model", [c_int, POINTER(c_float), POINTER(c_int)], c_int, check=False)
+_bind("omc_learned_stats", [POINTER(c_double)], c_int, check=False)
+
+is_initialized = False
+
+
+class _DLLGlobal:
+    """Data descriptor over an exported global (openmc/lib/core.py:724-734)."""
+
+    def __init__(self, ctype, name):
+        self.ctype = ctype
+        self.name = name
+
+    def __get__(self, instance, owner):
+        return self.ctype.in_dll(_dll, self.name).value
+
+    def __set__(self, instance, value):
+        self.ctype.in_dll(_dll, self.name).value = value
+
+
+class _Settings:
+    event_based = _DLLGlobal(c_bool, "event_based")
+    particles = _DLLGlobal(c_int64, "n_particles")
+    inactive = _DLLGlobal(c_int32, "n_inactive")
+    generations_per_batch = _DLLGlobal(c_int32, "gen_per_batch")
+    run_CE = _DLLGlobal(c_bool, "run_CE")
+    verbosity = _DLLGlobal(c_int, "verbosity")
+
+    @property
+    def run_mode(self):
+        return {1: "fixed source"}.get(c_int.in_dll(_dll, "run_mode").value)
+
+    @property
+    def seed(self):
+        return _dll.openmc_get_seed()
+
+    @seed.setter
+    def seed(self, seed):
+        _dll.openmc_set_seed(seed)
+
+    @property
+    def stride(self):
+        return _dll.openmc_get_stride()
+
+    @stride.setter
+    def stride(self, stride):
+        _dll.openmc_set_stride(stride)
+
+
+settings = _Settings()
+
+
+def init(args=None, intracomm=None, output=True):
+    """Initialize from model.xml in the current directory (openmc/lib/core.py:309-350)."""
+    global is_initialized
+    args = ["openmc"] + (list(args) if args is not None else [])
+    argv = (POINTER(c_char) * (len(args) + 1))()
+    bufs = [create_string_buffer(a.encode()) for a in args]
+    for i, b in enumerate(bufs):
+        argv[i] = b
+    if not output:
+        c_int.in_dll(_dll, "verbosity").value = 1
+    _dll.openmc_init(len(args), argv, None)
+    is_initialized = True
+
+
+def run(output=True):
+    _dll.openmc_run()
+
+
+def simulation_init():
+    _dll.openmc_simulation_init()
+
+
+def simulation_finalize():
+    _dll.openmc_simulation_finalize()
+
+
+def next_batch():
+    status = c_int()
+    _dll.openmc_next_batch(status)
+    return status.value
+
+
+def finalize():
+    global is_initialized
+    _dll.openmc_finalize()
+    is_initialized = False
+
+
+def reset():
+    _dll.openmc_reset()
+
+
+def hard_reset():
+    _dll.openmc_hard_reset()
+
+
+def reset_timers():
+    _dll.openmc_reset_timers()
+
+
+def current_batch():
+    return c_int.in_dll(_dll, "current_batch").value
+
+
+def num_realizations():
+    return c_int32.in_dll(_dll, "n_realizations").value
+
+
+@contextmanager
+def run_in_memory(**kwargs):
+    init(**kwargs)
+    try:
+        yield
+    finally:
+        finalize()
+
+
+class Tally:
+    """Read access to one tally's results (openmc/lib/tally.py)."""
+
+    def __init__(self, tally_id):
+        index = c_int32()
+        _dll.openmc_get_tally_index(tally_id, index)
+        self._index = index.value
+        self.id = tally_id
+
+    @property
+    def num_realizations(self):
+        n = c_int32()
+        _dll.openmc_tally_get_n_realizations(self._index, n)
+        return n.value
+
+    @property
+    def results(self):
+        data = POINTER(c_double)()
+        shape = (c_size_t * 3)()
+        _dll.openmc_tally_results(self._index, data, shape)
+        return np.ctypeslib.as_array(data, tuple(shape))
+
+    @property
+    def mean(self):
+        n = self.num_realizations
+        sum_ = self.results[:, :, 1]
+        return sum_ / n if n > 0 else sum_.copy()
+
+    @property
+    def std_dev(self):
+        results = self.results
+        std_dev = np.empty(results.shape[:2])
+        std_dev.fill(np.inf)
+        n = self.num_realizations
+        if n > 1:
+            sum_ = results[:, :, 1]
+            sum_sq = results[:, :, 2]
+            mean = sum_ / n
+            nonzero = np.abs(mean) > 0
+            std_dev[nonzero] = np.sqrt((sum_sq[nonzero] / n - mean[nonzero] ** 2) / (n - 1))
+        return std_dev
+
+
+class _TallyMapping:
+    def __getitem__(self, key):
+        return Tally(key)
+
+    def __iter__(self):
+        for i in range(len(self)):
+            tid = c_int32()
+            _dll.openmc_tally_get_id(i, tid)
+            yield tid.value
+
+    def __len__(self):
+        return _dll.tallies_size()
+
+    def keys(self):
+        return list(iter(self))
+
+
+tallies = _TallyMapping()
+
+
+def global_tallies():
+    ptr = POINTER(c_double)()
+    _dll.openmc_global_tallies(ptr)
+    return np.ctypeslib.as_array(ptr, (4, 3)).copy()
+
+
+# --- build-specific ---------------------------------------------------------
+
+def set_option(name, value):
+    _dll.omc_set_option(name.encode(), float(value))
+
+
+def stats():
+    names = _dll.omc_stat_names().decode().split(",")
+    out = (c_double * len(names))()
+    _dll.omc_get_stats(out, len(names))
+    return dict(zip(names, list(out)))
+
+
+def particle_hashes():
+    ptr = POINTER(c_uint64)()
+    n = c_int64()
+    _dll.omc_particle_hashes(ptr, n)
+    if n.value == 0:
+        return np.zeros(0, dtype=np.uint64)
+    return np.ctypeslib.as_array(ptr, (n.value,)).copy()
+
+
+def energy_grid():
+    out = (c_double * 3)()
+    _dll.omc_energy_grid(out)
+    return tuple(out)
+
+
+def grid_index(i_nuc, n_log_bins):
+    ptr = _dll.omc_grid_index(i_nuc)
+    return np.ctypeslib.as_array(ptr, (n_log_bins + 1,)).copy()
+
+
+def lookup_batch(i_nuc, energies, learned=False):
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    ig = np.zeros(E.size, dtype=np.int32)
+    xs = np.zeros((E.size, 2), dtype=np.float64)
+    probes = c_uint64()
+    _dll.omc_lookup_batch(i_nuc, E.ctypes.data_as(POINTER(c_double)), E.size, int(learned),
+                          ig.ctypes.data_as(POINTER(c_int)), xs.ctypes.data_as(POINTER(c_double)), probes)
+    return ig, xs, probes.value
+
+
+def macro_batch(material, energies, learned=False, repeat=1):
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros((E.size, 2), dtype=np.float64)
+    ms = c_double()
+    _dll.omc_macro_batch(material, E.ctypes.data_as(POINTER(c_double)), E.size, int(learned), int(repeat),
+                         out.ctypes.data_as(POINTER(c_double)), ms)
+    return out, ms.value
+
+
+def check_model(directory, n_gi=0):
+    """Parse a model and initialize its data on the host only (no GPU).
+
+    Returns (summary dict, normalized atom densities, first n_gi grid-index entries).
+    """
+    out = (c_double * 4096)()
+    gi = np.zeros(max(n_gi, 1), dtype=np.int32)
+    n = _dll.omc_check_model(directory.encode(), out, 4096, gi.ctypes.data_as(POINTER(c_int)), n_gi)
+    if n < 0:
+        _error_handler(n, None, None)
+    keys = ["n_nuclides", "n_materials", "n_cells", "n_surfaces", "n_lattices", "n_levels", "energy_min",
+            "energy_max", "log_spacing", "n_particles", "n_batches", "n_tally_bins", "root_universe"]
+    vals = list(out)[:n]
+    return dict(zip(keys, vals[:len(keys)])), np.array(vals[len(keys):]), gi[:n_gi]
+
+
+def math_batch(fn, x, y=None):
+    """fn: 0 log, 1 sin, 2 cos, 3 sqrt, 4 x/y, 5 prn(seed bits of x), 6 u64->f64 of bits of x."""
+    X = np.ascontiguousarray(x, dtype=np.float64)
+    Y = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(X.size)
+    _dll.omc_math_batch(fn, X.ctypes.data_as(POINTER(c_double)),
+                        None if Y is None else Y.ctypes.data_as(POINTER(c_double)), X.size,
+                        out.ctypes.data_as(POINTER(c_double)))
+    return out
+
+
+def learned_build(n_seg=256, epochs=200):
+    e = c_int()
+    _dll.omc_learned_build(n_seg, epochs, e)
+    return e.value
+
+
+def learned_loss_grad(i_nuc, params):
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1, 2)
+    n_seg = p.shape[0]
+    loss = np.zeros(n_seg)
+    grad = np.zeros((n_seg, 2))
+    _dll.omc_learned_loss_grad(i_nuc, n_seg, p.ctypes.data_as(POINTER(c_float)),
+                               loss.ctypes.data_as(POINTER(c_double)), grad.ctypes.data_as(POINTER(c_double)))
+    return loss, grad
+
+
+def learned_stats():
+    out = (c_double * 6)()
+    if _dll.omc_learned_stats(out) < 0:
+        raise AllocationError("learned index has not been built")
+    keys = ["n_seg", "max_err", "mean_err", "loss_init", "loss_trained", "loss_final"]
+    return dict(zip(keys, list(out)))
+
+
+def learned_model(i_nuc, n_seg):
+    p = np.zeros((n_seg, 2), dtype=np.float32)
+    e = np.zeros(n_seg, dtype=np.int32)
+    r = _dll.omc_learned_model(i_nuc, p.ctypes.data_as(POINTER(c_float)), e.ctypes.data_as(POINTER(c_int)))
+    if r < 0:
+        raise AllocationError("learned index has not been built")
+    return p, e

@@ -1,0 +1,138 @@
+"""ORACLE -- test infrastructure only: ctypes wrapper of oracle/_build/liboracle.so.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker and the CPU baseline; never by the product.
+"""
+import os
+from ctypes import CDLL, POINTER, c_char_p, c_double, c_int, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+from . import build as _build
+from .problem import write_oracle_problem
+
+_libs = {}
+
+
+def load_library(glibc_math=False):
+    path = _build.LIB_GLIBC if glibc_math else _build.LIB
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        _build.build()
+    lib = CDLL(path)
+    lib.oracle_load.argtypes = [c_char_p]
+    lib.oracle_load.restype = c_void_p
+    lib.oracle_last_error.restype = c_char_p
+    lib.oracle_free.argtypes = [c_void_p]
+    lib.oracle_run.argtypes = [c_void_p, c_int, c_int, c_int64, c_int, c_int]
+    lib.oracle_run.restype = c_int
+    lib.oracle_n_tally_bins.argtypes = [c_void_p]
+    lib.oracle_n_tally_bins.restype = c_int64
+    lib.oracle_results.argtypes = [c_void_p]
+    lib.oracle_results.restype = POINTER(c_double)
+    lib.oracle_batch_values.argtypes = [c_void_p]
+    lib.oracle_batch_values.restype = POINTER(c_double)
+    lib.oracle_hashes.argtypes = [c_void_p]
+    lib.oracle_hashes.restype = POINTER(c_uint64)
+    lib.oracle_n_realizations.argtypes = [c_void_p]
+    lib.oracle_counters.argtypes = [c_void_p, POINTER(c_int64)]
+    lib.oracle_leakage.argtypes = [c_void_p, POINTER(c_double)]
+    lib.oracle_energy_grid.argtypes = [c_void_p, POINTER(c_double)]
+    lib.oracle_grid_index.argtypes = [c_void_p, c_int]
+    lib.oracle_grid_index.restype = POINTER(c_int)
+    lib.oracle_lookup.argtypes = [c_void_p, c_int, c_double, POINTER(c_int), POINTER(c_double)]
+    lib.oracle_prn.argtypes = [POINTER(c_uint64)]
+    lib.oracle_prn.restype = c_double
+    lib.oracle_future_seed.argtypes = [c_uint64, c_uint64]
+    lib.oracle_future_seed.restype = c_uint64
+    lib.oracle_maxwell.argtypes = [c_double, POINTER(c_uint64)]
+    lib.oracle_maxwell.restype = c_double
+    lib.oracle_watt.argtypes = [c_double, c_double, POINTER(c_uint64)]
+    lib.oracle_watt.restype = c_double
+    lib.oracle_rotate_angle.argtypes = [POINTER(c_double), c_double, POINTER(c_double), POINTER(c_uint64),
+                                        POINTER(c_double)]
+    for fn in ("oracle_log", "oracle_sin", "oracle_cos"):
+        getattr(lib, fn).argtypes = [c_double]
+        getattr(lib, fn).restype = c_double
+    _libs[path] = lib
+    return lib
+
+
+class Oracle:
+    """One loaded problem of the CPU restatement."""
+
+    def __init__(self, problem_file, glibc_math=False):
+        self.lib = load_library(glibc_math)
+        h = self.lib.oracle_load(problem_file.encode())
+        if not h:
+            raise RuntimeError("oracle: " + self.lib.oracle_last_error().decode())
+        self.h = h
+
+    @classmethod
+    def from_model(cls, model, nuclides, directory, glibc_math=False):
+        path = write_oracle_problem(model, nuclides, directory)
+        return cls(path, glibc_math)
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, event_based=False, threads=1, n_particles=0, n_batches=0, trace_hash=True):
+        r = self.lib.oracle_run(self.h, 1 if event_based else 0, threads, n_particles, n_batches,
+                                1 if trace_hash else 0)
+        if r != 0:
+            raise RuntimeError("oracle: " + self.lib.oracle_last_error().decode())
+
+    @property
+    def n_bins(self):
+        return self.lib.oracle_n_tally_bins(self.h)
+
+    def results(self):
+        n = self.n_bins
+        return np.ctypeslib.as_array(self.lib.oracle_results(self.h), (n, 1, 3)).copy()
+
+    def hashes(self, n):
+        return np.ctypeslib.as_array(self.lib.oracle_hashes(self.h), (n,)).copy()
+
+    def n_realizations(self):
+        return self.lib.oracle_n_realizations(self.h)
+
+    def counters(self):
+        out = (c_int64 * 8)()
+        self.lib.oracle_counters(self.h, out)
+        keys = ["lookups", "collisions", "segments", "lost", "xs_events", "adv_events", "surf_events",
+                "coll_events"]
+        return dict(zip(keys, list(out)))
+
+    def leakage(self):
+        out = (c_double * 4)()
+        self.lib.oracle_leakage(self.h, out)
+        return list(out)
+
+    def energy_grid(self):
+        out = (c_double * 3)()
+        self.lib.oracle_energy_grid(self.h, out)
+        return tuple(out)
+
+    def grid_index(self, i_nuc, n_log_bins=8000):
+        return np.ctypeslib.as_array(self.lib.oracle_grid_index(self.h, i_nuc), (n_log_bins + 1,)).copy()
+
+    def lookup(self, i_nuc, energies):
+        E = np.asarray(energies, dtype=np.float64)
+        ig = np.zeros(E.size, dtype=np.int32)
+        xs = np.zeros((E.size, 4))
+        g = c_int()
+        o = (c_double * 4)()
+        for k, e in enumerate(E):
+            self.lib.oracle_lookup(self.h, i_nuc, float(e), g, o)
+            ig[k] = g.value
+            xs[k] = list(o)
+        return ig, xs

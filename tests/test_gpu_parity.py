@@ -1,0 +1,194 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle.
+
+Bit-exact for everything integer- or history-shaped (grid indices, per-
+particle history hashes that fold in every event's type, surface, nuclide,
+grid index, distance, energy and direction bits); FP64 tallies within 1e-12
+relative per bin (only the order of the atomic additions differs).
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_tallies_close, gpu_run, oracle_run, prepare
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from openmc_amd import lib as _lib
+    return _lib
+
+
+def _oracle_lib():
+    from oracle.oracle import load_library
+    return load_library()
+
+
+# ---------------------------------------------------------------- primitives
+
+def test_math_primitives_bitwise(lib):
+    rng = np.random.default_rng(7)
+    ol = _oracle_lib()
+    n = 200000
+    x_log = np.concatenate([rng.random(n // 4), np.exp(rng.uniform(-40, 40, n // 4)),
+                            1.0 + (rng.random(n // 4) - 0.5) * 1e-6,
+                            np.ldexp(rng.integers(1, 2**53, n // 4).astype(np.float64), -53)])
+    x_ang = np.concatenate([rng.random(n // 2) * 2 * np.pi, rng.random(n // 2) * np.pi / 2])
+    for fn, xs, ref in [(0, x_log, ol.oracle_log), (1, x_ang, ol.oracle_sin), (2, x_ang, ol.oracle_cos)]:
+        g = lib.math_batch(fn, xs)
+        h = np.array([ref(float(v)) for v in xs])
+        assert np.array_equal(g.view(np.uint64), h.view(np.uint64)), "fn %d differs" % fn
+    xs = rng.random(n) * 1e6
+    ys = rng.random(n) * 3 + 1e-3
+    assert np.array_equal(lib.math_batch(3, xs).view(np.uint64), np.sqrt(xs).view(np.uint64))
+    assert np.array_equal(lib.math_batch(4, xs, ys).view(np.uint64), (xs / ys).view(np.uint64))
+    seeds = rng.integers(0, 2**63, 5000, dtype=np.int64).view(np.float64)
+    g = lib.math_batch(5, seeds)
+    from ctypes import c_uint64
+    h = []
+    for s in seeds.view(np.uint64):
+        c = c_uint64(int(s))
+        h.append(ol.oracle_prn(c))
+    assert np.array_equal(g, np.array(h))
+    big = rng.integers(0, 2**63, 5000, dtype=np.int64).view(np.uint64)
+    big[::2] |= np.uint64(1) << np.uint64(63)
+    g = lib.math_batch(6, big.view(np.float64))
+    assert np.array_equal(g, big.astype(np.float64))
+
+
+# ---------------------------------------------------------------- lookups
+
+def test_lookup_plain_and_learned_bitwise(lib):
+    d, model, nucs = prepare(0, n_particles=100, n_batches=1)
+    from oracle.oracle import Oracle
+    import os
+    o = Oracle(os.path.join(d, "oracle_problem.txt"))
+    from helpers import chdir
+    rng = np.random.default_rng(11)
+    with chdir(d):
+        lib.init(output=False)
+        try:
+            max_err = lib.learned_build(256, 200)
+            assert max_err <= 64
+            for i_nuc in range(len(nucs)):
+                pts = nucs[i_nuc].energy
+                e_min, e_max, spacing = o.energy_grid()
+                E = np.concatenate([np.exp(rng.uniform(np.log(e_min), np.log(e_max), 20000)), pts[1:-1],
+                                    0.5 * (pts[:-1] + pts[1:]), e_min * np.exp(np.arange(8000) * spacing),
+                                    [pts[0] * 0.5, pts[-1] * 1.5]])
+                ig_o, xs_o = o.lookup(i_nuc, E)
+                ig_g, xs_g, pr_plain = lib.lookup_batch(i_nuc, E, learned=False)
+                assert np.array_equal(ig_o, ig_g)
+                assert np.array_equal(xs_o[:, 1:3].view(np.uint64), xs_g.view(np.uint64))
+                ig_l, xs_l, pr_learned = lib.lookup_batch(i_nuc, E, learned=True)
+                assert np.array_equal(ig_o, ig_l)
+                assert np.array_equal(xs_g.view(np.uint64), xs_l.view(np.uint64))
+        finally:
+            lib.finalize()
+
+
+# ---------------------------------------------------------------- transport
+
+@pytest.mark.parametrize("mode", ["history", "event"])
+def test_transport_cfg0_histories_and_tallies(mode):
+    d, model, nucs = prepare(0, n_particles=3000, n_batches=2)
+    o = oracle_run(d, event_based=(mode == "event"))
+    g = gpu_run(d, options={"trace_hash": 1})
+    assert g["n_real"] == 2
+    # last batch histories, bit for bit
+    h_o = o.hashes(3000)
+    assert np.array_equal(g["hashes"], h_o), "%d of 3000 histories differ" % np.sum(g["hashes"] != h_o)
+    assert_tallies_close(g["results"], o.results())
+    c = o.counters()
+    s = g["stats"]
+    assert s["lost"] == 0 and c["lost"] == 0
+    assert s["collisions"] == c["collisions"]
+    assert s["segments"] == c["segments"]
+    assert s["surf_events"] == c["surf_events"]
+
+
+def test_transport_cfg1_lattice_energy_filter():
+    d, model, nucs = prepare(1, n_particles=2000, n_batches=2, grid_lo=2000, grid_hi=6000, n_nuclides=12,
+                             lattice_n=5, mesh=(10, 10, 20), half_height=20.0,
+                             energy_bins=[1e-5, 1.0, 1e3, 1e5, 2e7])
+    o = oracle_run(d)
+    g = gpu_run(d, options={"trace_hash": 1})
+    h_o = o.hashes(2000)
+    assert np.array_equal(g["hashes"], h_o), "%d of 2000 histories differ" % np.sum(g["hashes"] != h_o)
+    assert_tallies_close(g["results"], o.results())
+    assert g["stats"]["surf_events"] == o.counters()["surf_events"]
+    assert g["stats"]["surf_events"] > 0
+
+
+def test_transport_graph_and_direct_launch_identical():
+    d, model, nucs = prepare(0, n_particles=2000, n_batches=1)
+    a = gpu_run(d, options={"trace_hash": 1, "use_graph": 1})
+    b = gpu_run(d, options={"trace_hash": 1, "use_graph": 0, "cycles_per_check": 3})
+    c = gpu_run(d, options={"trace_hash": 1, "max_in_flight": 700})
+    assert np.array_equal(a["hashes"], b["hashes"])
+    assert np.array_equal(a["hashes"], c["hashes"])
+    assert_tallies_close(a["results"], b["results"])
+    assert_tallies_close(a["results"], c["results"])
+
+
+def test_transport_learned_index_same_histories():
+    d, model, nucs = prepare(0, n_particles=2000, n_batches=1)
+    a = gpu_run(d, options={"trace_hash": 1})
+    b = gpu_run(d, options={"trace_hash": 1}, learned=(256, 200))
+    assert np.array_equal(a["hashes"], b["hashes"])
+    assert_tallies_close(a["results"], b["results"])
+
+
+def test_learned_loss_and_gradient_match_cpu(lib):
+    """Forward/backward kernels of the learned index vs a numpy restatement."""
+    d, model, nucs = prepare(0, n_particles=100, n_batches=1)
+    from helpers import chdir
+    ol = _oracle_lib()
+    n_seg = 64
+    with chdir(d):
+        lib.init(output=False)
+        try:
+            e_min, e_max, _ = lib.energy_grid()
+            for i_nuc in range(2):
+                E = nucs[i_nuc].energy
+                x = np.array([ol.oracle_log(float(v / e_min)) for v in E])
+                u_lo = ol.oracle_log(float(E[0] / e_min))
+                u_hi = ol.oracle_log(float(E[-1] / e_min))
+                inv_w = n_seg / (u_hi - u_lo)
+                uu = (x - u_lo) * inv_w
+                seg = np.clip(np.where(uu < 0, 0, uu.astype(np.int64)), 0, n_seg - 1)
+                rng = np.random.default_rng(i_nuc)
+                params = np.stack([rng.uniform(100, 300, n_seg), rng.uniform(0, E.size, n_seg)], 1)
+                params = params.astype(np.float32)
+                f = (uu - seg).astype(np.float32)
+                a = params[seg, 0]
+                b = params[seg, 1]
+                pred = (a * f).astype(np.float32) + b
+                r = pred.astype(np.float64) - np.arange(E.size)
+                loss_ref = np.bincount(seg, weights=r * r, minlength=n_seg)
+                ga_ref = np.bincount(seg, weights=2 * r * f.astype(np.float64), minlength=n_seg)
+                gb_ref = np.bincount(seg, weights=2 * r, minlength=n_seg)
+                loss, grad = lib.learned_loss_grad(i_nuc, params)
+                assert np.allclose(loss, loss_ref, rtol=1e-10, atol=1e-6)
+                assert np.allclose(grad[:, 0], ga_ref, rtol=1e-10, atol=1e-6)
+                assert np.allclose(grad[:, 1], gb_ref, rtol=1e-10, atol=1e-6)
+                # a plausible mistake (dropping the factor f) must not pass
+                assert not np.allclose(grad[:, 0], gb_ref, rtol=1e-3)
+        finally:
+            lib.finalize()
+
+
+def test_learned_training_reduces_loss_and_bounds_hold(lib):
+    d, model, nucs = prepare(0, n_particles=100, n_batches=1)
+    from helpers import chdir
+    with chdir(d):
+        lib.init(output=False)
+        try:
+            lib.learned_build(256, 200)
+            st = lib.learned_stats()
+            p1, e1 = lib.learned_model(0, 256)
+            assert e1.max() <= 64 and st["max_err"] <= 64
+            # the Adam steps on the forward/backward kernels lower the squared error
+            assert st["loss_trained"] < st["loss_init"]
+        finally:
+            lib.finalize()
