@@ -1,0 +1,213 @@
+"""Benchmark of the MI355X event-based transport loop.
+
+Workload (default): the GPU-scale lattice problem -- 60 synthetic nuclides
+with 20k-100k-point grids, three 20-nuclide materials in a 17x17 pin lattice
+(1.26 cm pitch, reflective sides, vacuum at z = +-50 cm), Watt source in the
+fuel, 1e7 particles per batch, 100x100x100 track-length flux mesh in FP64.
+A step is one batch.  Inputs (nuclear data, geometry, tallies) are resident in
+HBM before timing starts; the timed region is W warm-up batches excluded, then
+exactly K batches, each ending in a stream synchronization.
+
+Prints one JSON line: particles/s (whole job), XS lookups/s, the roofline of
+the dominant kernel from HIP events recorded on the library's stream around
+every kernel launch in the timed region, the learned-index comparison and a
+CPU baseline (the oracle, history-based, OpenMP over the host cores, on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes (fixed contract): 48 B per XS lookup (E, sigma_t,
+# sigma_a brackets), 16 B per collision (sigma_el bracket), 16 B per scored
+# mesh segment (FP64 read + write), 144 B of particle state per event.
+B_LOOKUP, B_COLL, B_SEG, B_EVENT = 48, 16, 16, 144
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(st):
+    """Algorithmic bytes moved by each event kernel over the measured region."""
+    coll_lookups = st["collisions"]
+    xs_lookups = st["lookups"] - coll_lookups
+    return {
+        "k_xs": B_LOOKUP * xs_lookups + B_EVENT * st["xs_events"],
+        "k_advance": B_SEG * st["segments"] + B_EVENT * st["adv_events"],
+        "k_surface": B_EVENT * st["surf_events"],
+        "k_collision": B_LOOKUP * coll_lookups + B_COLL * st["collisions"] + B_EVENT * st["coll_events"],
+    }
+
+
+def diff(a, b):
+    return {k: b[k] - a[k] for k in b}
+
+
+def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200):
+    from openmc_amd import lib
+    old = os.getcwd()
+    os.chdir(d)
+    try:
+        lib.init(output=False)
+        info = {}
+        if learned:
+            t = time.perf_counter()
+            info["learned_max_err"] = lib.learned_build(n_seg, epochs)
+            info["learned_build_s"] = time.perf_counter() - t
+            info.update({"learned_" + k: v for k, v in lib.learned_stats().items()})
+            lib.set_option("learned_index", 1)
+        lib.set_option("event_timing", 1)
+        lib.simulation_init()
+        for _ in range(warmup):
+            lib.next_batch()
+        lib.reset_timers()
+        s0 = lib.stats()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            lib.next_batch()
+            log("  batch %d done (%.2f s)" % (k + 1, time.perf_counter() - t0))
+        t1 = time.perf_counter()
+        s1 = lib.stats()
+        lib.simulation_finalize()
+        n = lib.settings.particles
+        lib.finalize()
+    finally:
+        os.chdir(old)
+    st = diff(s0, s1)
+    info.update({"elapsed_s": t1 - t0, "stats": st, "n_particles": n})
+    return info
+
+
+def roofline(info):
+    st = info["stats"]
+    times = {"k_xs": st["t_xs_ms"], "k_advance": st["t_adv_ms"], "k_surface": st["t_surf_ms"],
+             "k_collision": st["t_coll_ms"]}
+    launches = {"k_xs": st["n_xs"], "k_advance": st["n_adv"], "k_surface": st["n_surf"],
+                "k_collision": st["n_coll"]}
+    by = kernel_bytes(st)
+    dom = max(times, key=times.get)
+    achieved = by[dom] / (times[dom] * 1e-3) / 1e9
+    per_kernel = {k: {"ms": times[k], "launches": launches[k], "avg_us": 1e3 * times[k] / max(launches[k], 1),
+                      "GB_per_s": by[k] / max(times[k], 1e-9) * 1e-6} for k in times}
+    total_bytes = sum(by.values())
+    whole = total_bytes / (st["transport_ms"] * 1e-3) / 1e9
+    return dom, {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                 "kernel": dom, "whole_loop_GB_per_s": round(whole, 2)}, per_kernel
+
+
+def cpu_baseline(d, target_s=15.0):
+    from oracle.oracle import Oracle
+    threads = os.cpu_count() or 1
+    path = os.path.join(d, "oracle_problem.txt")
+    o = Oracle(path)
+    t = time.perf_counter()
+    o.run(event_based=False, threads=threads, n_particles=2000, n_batches=1, trace_hash=False)
+    probe = time.perf_counter() - t
+    n = int(max(2000, min(2_000_000, 2000 * target_s / max(probe, 1e-3))))
+    o.close()
+    o = Oracle(path)
+    t = time.perf_counter()
+    o.run(event_based=False, threads=threads, n_particles=n, n_batches=1, trace_hash=False)
+    el = time.perf_counter() - t
+    c = o.counters()
+    o.close()
+    return {"value": round(n / el, 2), "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": "%d histories of batch 1 of the same problem, history-based, %d OpenMP threads, "
+                      "%.1f s" % (n, threads, el),
+            "lookups_per_s": round(c["lookups"] / el, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="run a plain device copy of this many bytes first (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
+              "design is a per-rank particle split plus one tally reduction per batch)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from openmc_amd import configs
+    copy_info = None
+    if args.copy_calibration > 0:
+        from openmc_amd import lib
+        ms = lib.copy_kernel(args.copy_calibration, reps=3)
+        copy_info = {"bytes": args.copy_calibration, "ms": ms,
+                     "GB_per_s": round(2 * args.copy_calibration / (ms * 1e-3) / 1e9, 1)}
+        log("copy kernel: %d bytes in %.3f ms (%.0f GB/s read+write)" % (args.copy_calibration, ms,
+                                                                          copy_info["GB_per_s"]))
+    log("preparing problem (config %d, %d particles/batch)" % (args.config, args.particles))
+    kw = {"n_particles": args.particles, "n_batches": args.warmup + args.steps + 1}
+    d, model, nucs = configs.problem_dir(args.config, **kw)
+    from oracle.problem import write_oracle_problem
+    if not args.no_cpu_baseline and not os.path.exists(os.path.join(d, "oracle_problem.txt")):
+        write_oracle_problem(model, nucs, d)
+
+    log("GPU plain binary search: %d warm-up + %d timed batches" % (args.warmup, args.steps))
+    plain = run_gpu(d, args.steps, args.warmup)
+    dom, roof, per_kernel = roofline(plain)
+    n = plain["n_particles"]
+    value = n * args.steps / plain["elapsed_s"]
+    st = plain["stats"]
+    out = {
+        "metric": "particles/s and XS lookups/s on 1 MI355X: learned-index vs binary search",
+        "value": round(value, 2),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * plain["elapsed_s"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic nuclides from fixed seeds; random source histories",
+        "config": {"workload": "lattice-17x17-60nuclides-1e7pps-mesh100^3" if args.config == 1 else
+                   "config%d" % args.config, "particles_per_batch": n, "nuclides": len(nucs),
+                   "grid_points": int(sum(x.n_points for x in nucs)), "mode": "event-based"},
+        "xs_lookups_per_s": round(st["lookups"] / (st["t_xs_ms"] + st["t_coll_ms"]) * 1e3, 1),
+        "xs_lookups_per_s_transport": round(st["lookups"] / (st["transport_ms"] * 1e-3), 1),
+        "events_per_particle": round((st["xs_events"] + st["adv_events"] + st["surf_events"] +
+                                      st["coll_events"]) / (n * args.steps), 2),
+        "kernels": per_kernel,
+        "roofline": roof,
+    }
+    if copy_info:
+        out["copy_calibration"] = copy_info
+    if not args.no_learned:
+        log("GPU learned index: %d warm-up + %d timed batches" % (args.warmup, args.steps))
+        lr = run_gpu(d, args.steps, args.warmup, learned=True)
+        lst = lr["stats"]
+        out["learned"] = {
+            "particles_per_s": round(n * args.steps / lr["elapsed_s"], 2),
+            "xs_lookups_per_s": round(lst["lookups"] / (lst["t_xs_ms"] + lst["t_coll_ms"]) * 1e3, 1),
+            "speedup_vs_binary": round(plain["elapsed_s"] / lr["elapsed_s"], 4),
+            "probes_per_lookup": round(lst["search_probes"] / max(lst["lookups"], 1), 3),
+            "probes_per_lookup_binary": round(st["search_probes"] / max(st["lookups"], 1), 3),
+            "max_err": lr.get("learned_max_err"), "mean_err": lr.get("learned_mean_err"),
+            "build_s": round(lr.get("learned_build_s", 0), 3)}
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle, history-based)")
+        out["cpu_baseline"] = cpu_baseline(d)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

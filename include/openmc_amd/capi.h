@@ -109,6 +109,9 @@ int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat
  * the concatenated log-grid tables to grid_index_out.  Returns the number of
  * values available in out (>= 0) or an error code. */
 int omc_check_model(const char* dir, double* out, int n_out, int* grid_index_out, int64_t n_gi);
+/* Streaming copy of n_bytes, `reps` times: calibrates HBM byte counters and
+ * the achievable copy bandwidth; *ms = time per copy. */
+int omc_copy_kernel(int64_t n_bytes, int reps, double* ms);
 /* Elementwise primitives on the GPU for bitwise host/device checks: fn 0 log,
  * 1 sin, 2 cos, 3 sqrt, 4 x/y, 5 prn(seed = bits of x), 6 u64->f64 of bits of x. */
 int omc_math_batch(int fn, const double* x, const double* y, int64_t n, double* out);

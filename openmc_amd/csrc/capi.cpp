@@ -1032,6 +1032,32 @@ int omc_check_model(const char* dir, double* out, int n_out, int* grid_index_out
   });
 }
 
+int omc_copy_kernel(int64_t n_bytes, int reps, double* ms)
+{
+  return guarded([&]() -> int {
+    std::vector<void*> tmp;
+    char* a = dev_alloc<char>(tmp, n_bytes);
+    char* b = dev_alloc<char>(tmp, n_bytes);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    launch_copy(a, b, n_bytes, nullptr);
+    hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
+    for (int r = 0; r < std::max(1, reps); ++r)
+      launch_copy(a, b, n_bytes, nullptr);
+    hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "copy kernel");
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+    if (ms)
+      *ms = t / std::max(1, reps);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    free_list(tmp);
+    return 0;
+  });
+}
+
 int omc_math_batch(int fn, const double* x, const double* y, int64_t n, double* out)
 {
   return guarded([&]() -> int {

@@ -96,6 +96,22 @@ __global__ void k_math_batch(int fn, const double* x, const double* y, int64_t n
   }
 }
 
+// Plain streaming copy: the calibration of HBM byte counters (a copy of N
+// bytes reads N and writes N).
+__global__ void __launch_bounds__(LB_BLOCK) k_copy(const double4* src, double4* dst, int64_t n)
+{
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy(const void* src, void* dst, int64_t n_bytes, hipStream_t st)
+{
+  int64_t n = n_bytes / static_cast<int64_t>(sizeof(double4));
+  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(LB_BLOCK), 0, st, static_cast<const double4*>(src),
+    static_cast<double4*>(dst), n);
+}
+
 void launch_math_batch(int fn, const double* x, const double* y, int64_t n, double* out, hipStream_t st)
 {
   int64_t b = (n + LB_BLOCK - 1) / LB_BLOCK;

@@ -120,6 +120,7 @@ _bind("omc_lookup_batch", [c_int, POINTER(c_double), c_int64, c_int, POINTER(c_i
 _bind("omc_macro_batch", [c_int, POINTER(c_double), c_int64, c_int, c_int, POINTER(c_double),
                           POINTER(c_double)])
 _bind("omc_math_batch", [c_int, POINTER(c_double), POINTER(c_double), c_int64, POINTER(c_double)])
+_bind("omc_copy_kernel", [c_int64, c_int, POINTER(c_double)])
 _bind("omc_check_model", [c_char_p, POINTER(c_double), c_int, POINTER(c_int), c_int64], c_int, check=False)
 _bind("omc_learned_build", [c_int, c_int, POINTER(c_int)])
 _bind("omc_learned_loss_grad", [c_int, c_int, POINTER(c_float), POINTER(c_double), POINTER(c_double)])
@@ -361,6 +362,13 @@ def macro_batch(material, energies, learned=False, repeat=1):
     _dll.omc_macro_batch(material, E.ctypes.data_as(POINTER(c_double)), E.size, int(learned), int(repeat),
                          out.ctypes.data_as(POINTER(c_double)), ms)
     return out, ms.value
+
+
+def copy_kernel(n_bytes, reps=10):
+    """Time a plain device copy of n_bytes; returns ms per copy."""
+    ms = c_double()
+    _dll.omc_copy_kernel(int(n_bytes), int(reps), ms)
+    return ms.value
 
 
 def check_model(directory, n_gi=0):

@@ -17,8 +17,6 @@
 // NEAREST temperature with one temperature, point/box sources with isotropic
 // or monodirectional angles and discrete/Watt energies, and RegularMesh
 // track-length flux tallies (optionally with an energy filter).
-//
-// Parity status of each function is listed in DESIGN.md.
 
 #include <algorithm>
 #include <cmath>
@@ -586,6 +584,7 @@ struct Boundary {
 struct Counters {
   int64_t lookups = 0, collisions = 0, events = 0, segments = 0, lost = 0;
   int64_t xs_events = 0, advance_events = 0, surface_events = 0, collision_events = 0;
+  double tracklength = 0.0; // sum of wgt * distance over all advances
 };
 
 struct Particle {
@@ -655,7 +654,8 @@ void initialize_data(Problem& P)
   // Nuclide::init_grid (src/nuclide.cpp:502-533)
   double spacing = m_log(P.energy_max / P.energy_min) / M;
   // xt::linspace(0, M*spacing, M+1): start + k*step, step = (stop-start)/(num-1)
-  // (xtensor's generator; exact element formula unpinned)
+  // (xtensor's generator; its exact element formula is not pinned by any
+  // reference test: k*step with step = stop/M is assumed)
   double stop = M * spacing;
   double step = (stop - 0.0) / static_cast<double>(M);
   for (auto& nuc : P.nuclides) {
@@ -1271,6 +1271,7 @@ void event_advance(Sim& S, Particle& p, Counters& cnt)
   double dt = distance / speed;
   p.time += dt;
   score_tracklength_tally(S, p, distance, cnt);
+  cnt.tracklength += p.wgt * distance;
   if (distance == distance_cutoff)
     p.wgt = 0.0;
   if (S.trace_hash) {
@@ -1534,6 +1535,7 @@ void transport_history_based(Sim& S, int64_t n)
       S.cnt.advance_events += c.advance_events;
       S.cnt.surface_events += c.surface_events;
       S.cnt.collision_events += c.collision_events;
+      S.cnt.tracklength += c.tracklength;
     }
   }
 }
@@ -1620,6 +1622,7 @@ void transport_event_based(Sim& S, int64_t n_total)
     S.cnt.advance_events += c.advance_events;
     S.cnt.surface_events += c.surface_events;
     S.cnt.collision_events += c.collision_events;
+    S.cnt.tracklength += c.tracklength;
     remaining -= n;
     source_offset += n;
   }
@@ -2060,6 +2063,7 @@ void oracle_leakage(void* h, double* out)
   out[1] = S.leak_sum_sq;
   out[2] = S.leakage_batch;
   out[3] = S.total_weight;
+  out[4] = S.cnt.tracklength;
 }
 
 void oracle_energy_grid(void* h, double* out)

@@ -113,9 +113,9 @@ class Oracle:
         return dict(zip(keys, list(out)))
 
     def leakage(self):
-        out = (c_double * 4)()
+        out = (c_double * 5)()
         self.lib.oracle_leakage(self.h, out)
-        return list(out)
+        return dict(zip(["leak_sum", "leak_sum_sq", "leak_batch", "total_weight", "tracklength"], list(out)))
 
     def energy_grid(self):
         out = (c_double * 3)()
