@@ -51,13 +51,15 @@ def diff(a, b):
     return {k: b[k] - a[k] for k in b}
 
 
-def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200):
+def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None):
     from openmc_amd import lib
     old = os.getcwd()
     os.chdir(d)
     try:
         lib.init(output=False)
         info = {}
+        for k, v in (options or {}).items():
+            lib.set_option(k, v)
         if learned:
             t = time.perf_counter()
             info["learned_max_err"] = lib.learned_build(n_seg, epochs)
@@ -89,10 +91,12 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200):
 def roofline(info):
     st = info["stats"]
     times = {"k_xs": st["t_xs_ms"], "k_advance": st["t_adv_ms"], "k_surface": st["t_surf_ms"],
-             "k_collision": st["t_coll_ms"]}
+             "k_collision": st["t_coll_ms"], "k_sort": st["t_sort_ms"], "k_init": st["t_init_ms"]}
     launches = {"k_xs": st["n_xs"], "k_advance": st["n_adv"], "k_surface": st["n_surf"],
-                "k_collision": st["n_coll"]}
+                "k_collision": st["n_coll"], "k_sort": st["n_sort"], "k_init": st["n_init"]}
     by = kernel_bytes(st)
+    by["k_sort"] = 0
+    by["k_init"] = 0
     dom = max(times, key=times.get)
     achieved = by[dom] / (times[dom] * 1e-3) / 1e9
     per_kernel = {k: {"ms": times[k], "launches": launches[k], "avg_us": 1e3 * times[k] / max(launches[k], 1),
@@ -135,6 +139,8 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="transport option passed to omc_set_option (e.g. sort=0, xs_lanes=8)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run a plain device copy of this many bytes first (HBM counter calibration)")
     args = ap.parse_args()
@@ -161,7 +167,8 @@ def main():
         write_oracle_problem(model, nucs, d)
 
     log("GPU plain binary search: %d warm-up + %d timed batches" % (args.warmup, args.steps))
-    plain = run_gpu(d, args.steps, args.warmup)
+    opts = {k: float(v) for k, v in (o.split("=", 1) for o in args.option)}
+    plain = run_gpu(d, args.steps, args.warmup, options=opts)
     dom, roof, per_kernel = roofline(plain)
     n = plain["n_particles"]
     value = n * args.steps / plain["elapsed_s"]
@@ -193,7 +200,7 @@ def main():
         out["copy_calibration"] = copy_info
     if not args.no_learned:
         log("GPU learned index: %d warm-up + %d timed batches" % (args.warmup, args.steps))
-        lr = run_gpu(d, args.steps, args.warmup, learned=True)
+        lr = run_gpu(d, args.steps, args.warmup, learned=True, options=opts)
         lst = lr["stats"]
         out["learned"] = {
             "particles_per_s": round(n * args.steps / lr["elapsed_s"], 2),

@@ -109,6 +109,12 @@ struct Sim {
   bool event_timing = false;
   bool use_graph = true;
   int cycles_per_check = 16;
+  int xs_lanes_opt = 0; // 0 = from the largest material
+  bool sort = true;
+  int sort_blocks_opt = 0; // 0 = one workgroup per CU
+  int blocks_opt = 0;      // 0 = from the capacity
+  int xs_group_opt = 4;
+  LaunchConfig lc {1, 1, 0, 1, 4};
   int64_t max_in_flight_opt = 0;
   // graph
   hipGraphExec_t graph_exec = nullptr;
@@ -258,6 +264,19 @@ void upload_problem(Sim& s)
   P.learned.enabled = 0;
 }
 
+void update_launch_config(Sim& s)
+{
+  // one lane per particle unless asked otherwise: measured fastest on the
+  // 20-nuclide materials of the lattice problem (the lanes of a particle
+  // group cost workgroup barriers and an LDS staging area)
+  int lanes = s.xs_lanes_opt > 0 ? s.xs_lanes_opt : 1;
+  s.lc.blocks = s.blocks_opt > 0 ? s.blocks_opt : s.blocks;
+  s.lc.xs_lanes = lanes;
+  s.lc.sort = s.sort ? 1 : 0;
+  s.lc.sort_blocks = std::min(s.lc.blocks, s.sort_blocks_opt > 0 ? s.sort_blocks_opt : 256);
+  s.lc.xs_group = s.xs_group_opt;
+}
+
 void free_particles(Sim& s)
 {
   destroy_graph(s);
@@ -270,35 +289,13 @@ void alloc_particles(Sim& s, int64_t cap)
   free_particles(s);
   DevParticles& S = s.S;
   std::vector<void*>& L = s.part_allocs;
-  const int64_t nl = MAX_LEVELS;
   S.cap = cap;
-  size_t lv = static_cast<size_t>(nl * cap);
-  S.rx = dev_alloc<double>(L, lv);
-  S.ry = dev_alloc<double>(L, lv);
-  S.rz = dev_alloc<double>(L, lv);
-  S.ux = dev_alloc<double>(L, lv);
-  S.uy = dev_alloc<double>(L, lv);
-  S.uz = dev_alloc<double>(L, lv);
-  S.cell = dev_alloc<int>(L, lv);
-  S.univ = dev_alloc<int>(L, lv);
-  S.lat = dev_alloc<int>(L, lv);
-  S.lix = dev_alloc<int>(L, lv);
-  S.liy = dev_alloc<int>(L, lv);
-  S.liz = dev_alloc<int>(L, lv);
-  S.n_coord = dev_alloc<int>(L, cap);
-  S.E = dev_alloc<double>(L, cap);
-  S.wgt = dev_alloc<double>(L, cap);
-  S.time = dev_alloc<double>(L, cap);
-  S.seed = dev_alloc<uint64_t>(L, cap);
-  S.surface = dev_alloc<int>(L, cap);
-  S.material = dev_alloc<int>(L, cap);
-  S.material_last = dev_alloc<int>(L, cap);
-  S.xs_t = dev_alloc<double>(L, cap);
-  S.xs_a = dev_alloc<double>(L, cap);
-  S.micro_t = dev_alloc<double>(L, static_cast<size_t>(std::max(1, s.P.max_mat_nuc)) * cap);
-  S.b_surface = dev_alloc<int>(L, cap);
-  S.b_info = dev_alloc<int>(L, cap);
-  S.n_event = dev_alloc<int>(L, cap);
+  // levels >= 1 only; a one-level model keeps one unused record per particle
+  const int64_t deep = std::max(1, s.P.n_levels - 1);
+  S.hot = dev_alloc<PHot>(L, cap);
+  S.lev = dev_alloc<PLevel>(L, static_cast<size_t>(deep * cap));
+  S.max_nn = std::max(1, s.P.max_mat_nuc);
+  S.micro_t = dev_alloc<double>(L, static_cast<size_t>(S.max_nn) * cap);
   S.lost = dev_alloc<int>(L, cap);
   S.hash = dev_alloc<uint64_t>(L, cap);
   S.leak = dev_alloc<double>(L, 1);
@@ -307,16 +304,39 @@ void alloc_particles(Sim& s, int64_t cap)
   S.q_coll = dev_alloc<int>(L, cap);
   S.qcount = dev_alloc<int>(L, QC_N);
   S.counters = dev_alloc<unsigned long long>(L, CNT_N);
+  // counting sort by (material, energy bucket): the bucket is the sign,
+  // exponent and top mantissa bits of E, offset by those of energy_min
+  S.q_sorted = dev_alloc<int>(L, cap);
+  S.q_key = dev_alloc<int>(L, cap);
+  // four buckets per binade (sign, exponent and two mantissa bits), fewer
+  // when the materials times buckets exceed the LDS histogram
+  const int n_mat = s.P.n_materials + 1;
+  int shift = 50, eb = 256;
+  while (n_mat * eb > SORT_MAX_BINS && eb > 1) {
+    ++shift;
+    eb >>= 1;
+  }
+  uint64_t emin_bits;
+  std::memcpy(&emin_bits, &s.m.energy_min, 8);
+  S.sort_shift = shift;
+  S.sort_eb = eb;
+  S.sort_base = static_cast<int>(emin_bits >> shift) - 1;
+  S.sort_nb = std::min(n_mat * eb, SORT_MAX_BINS);
+  S.sort_count = dev_alloc<int>(L, static_cast<size_t>(S.sort_nb) * SORT_MAX_BLOCKS);
+  S.sort_total = dev_alloc<int>(L, S.sort_nb);
   s.cap = cap;
   // launch shape: enough workgroups to fill 256 CUs several times over
   int64_t b = (cap + 255) / 256;
   s.blocks = static_cast<int>(std::min<int64_t>(std::max<int64_t>(b, 1), 4096));
+  update_launch_config(s);
 }
 
 int64_t bytes_per_particle(const Sim& s)
 {
-  int64_t lv = MAX_LEVELS * (6 * 8 + 6 * 4);
-  return lv + 4 + 3 * 8 + 8 + 3 * 4 + 2 * 8 + std::max(1, s.P.max_mat_nuc) * 8 + 4 * 4 + 8 + 3 * 4;
+  const int64_t deep = std::max(1, s.P.n_levels - 1);
+  // hot record, deeper levels, nuclide totals, lost flag, hash, four queues
+  return static_cast<int64_t>(sizeof(PHot)) + deep * static_cast<int64_t>(sizeof(PLevel)) +
+         std::max(1, s.P.max_mat_nuc) * 8 + 4 + 8 + 4 * 4;
 }
 
 void choose_capacity(Sim& s)
@@ -371,9 +391,11 @@ void run_cycles(Sim& s, int n_cycles)
 {
   if (s.event_timing) {
     for (int c = 0; c < n_cycles; ++c) {
-      for (int k = 0; k < 4; ++k) {
+      for (int k = KERNEL_SORT; k <= KERNEL_COLLISION; ++k) {
+        if (k == KERNEL_SORT && !s.lc.sort)
+          continue;
         record_kernel(s, k, true);
-        launch_kernel(k, s.P, s.S, s.blocks, s.stream);
+        launch_kernel(k, s.P, s.S, s.lc, s.stream);
         record_kernel(s, k, false);
       }
     }
@@ -382,7 +404,7 @@ void run_cycles(Sim& s, int n_cycles)
       destroy_graph(s);
       hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       for (int c = 0; c < s.cycles_per_check; ++c)
-        launch_cycle(s.P, s.S, s.blocks, s.stream);
+        launch_cycle(s.P, s.S, s.lc, s.stream);
       hip_check(hipStreamEndCapture(s.stream, &s.graph), "hipStreamEndCapture");
       hip_check(hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
       s.graph_valid = true;
@@ -390,7 +412,7 @@ void run_cycles(Sim& s, int n_cycles)
     hip_check(hipGraphLaunch(s.graph_exec, s.stream), "hipGraphLaunch");
   } else {
     for (int c = 0; c < n_cycles; ++c)
-      launch_cycle(s.P, s.S, s.blocks, s.stream);
+      launch_cycle(s.P, s.S, s.lc, s.stream);
   }
   s.n_cycles += n_cycles;
 }
@@ -422,10 +444,10 @@ void transport_batch(Sim& s)
     int64_t n = std::min(remaining, s.cap);
     int64_t id_base = (static_cast<int64_t>(current_batch) - 1) * m.n_particles + offset;
     if (s.event_timing)
-      record_kernel(s, 4, true);
+      record_kernel(s, KERNEL_INIT, true);
     launch_init(s.P, s.S, n, id_base, s.stream);
     if (s.event_timing)
-      record_kernel(s, 4, false);
+      record_kernel(s, KERNEL_INIT, false);
     s.h_qcount[QC_XS] = static_cast<int>(n);
     s.h_qcount[QC_ADV] = 0;
     s.h_qcount[QC_SURF] = 0;
@@ -852,7 +874,34 @@ int omc_set_option(const char* name, double value)
     g->use_graph = value != 0;
   else if (n == "cycles_per_check")
     g->cycles_per_check = std::max(1, static_cast<int>(value));
-  else if (n == "max_in_flight") {
+  else if (n == "xs_lanes") {
+    int v = static_cast<int>(value);
+    if (v != 0 && (v < 1 || v > 64 || (v & (v - 1)) != 0)) {
+      set_errmsg("xs_lanes must be 0 or a power of two in [1, 64]");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    g->xs_lanes_opt = v;
+    update_launch_config(*g);
+  } else if (n == "sort") {
+    g->sort = value != 0;
+    update_launch_config(*g);
+  } else if (n == "xs_group") {
+    int v = static_cast<int>(value);
+    if (v != 1 && v != 2 && v != 4 && v != 8) {
+      set_errmsg("xs_group must be 1, 2, 4 or 8");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    g->xs_group_opt = v;
+    update_launch_config(*g);
+  } else if (n == "sort_blocks" || n == "blocks") {
+    int v = static_cast<int>(value);
+    if (v < 0 || v > 65536) {
+      set_errmsg(n + " must be in [0, 65536]");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    (n == "blocks" ? g->blocks_opt : g->sort_blocks_opt) = v;
+    update_launch_config(*g);
+  } else if (n == "max_in_flight") {
     g->max_in_flight_opt = static_cast<int64_t>(value);
     if (g->sim_initialized)
       return guarded([&]() -> int {
@@ -868,8 +917,8 @@ int omc_set_option(const char* name, double value)
 
 static const char* k_stat_names =
   "transport_ms,active_batches,lookups,collisions,segments,lost,xs_events,adv_events,surf_events,"
-  "coll_events,search_probes,cycles,checks,capacity,t_xs_ms,t_adv_ms,t_surf_ms,t_coll_ms,t_init_ms,"
-  "n_xs,n_adv,n_surf,n_coll,n_init,wall_s,blocks";
+  "coll_events,search_probes,cycles,checks,capacity,t_sort_ms,t_xs_ms,t_adv_ms,t_surf_ms,t_coll_ms,"
+  "t_init_ms,n_sort,n_xs,n_adv,n_surf,n_coll,n_init,wall_s,blocks,xs_lanes,sort";
 
 const char* omc_stat_names(void)
 {
@@ -895,12 +944,14 @@ int omc_get_stats(double* out, int n)
   v.push_back(static_cast<double>(g->n_cycles));
   v.push_back(static_cast<double>(g->n_checks));
   v.push_back(static_cast<double>(g->cap));
-  for (int k = 0; k < 5; ++k)
+  for (int k = 0; k < 6; ++k)
     v.push_back(g->kt.ms[k]);
-  for (int k = 0; k < 5; ++k)
+  for (int k = 0; k < 6; ++k)
     v.push_back(static_cast<double>(g->kt.launches[k]));
   v.push_back(g->t_batch_wall_s);
   v.push_back(g->blocks);
+  v.push_back(g->lc.xs_lanes);
+  v.push_back(g->lc.sort);
   for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
     out[k] = v[k];
   return static_cast<int>(v.size());

@@ -1,11 +1,12 @@
-// Device-resident problem description and particle state (structure of arrays).
+// Device-resident problem description and particle state.
 //
 // Everything a transport kernel reads is reached from one DevProblem passed by
 // value as a kernel argument; all pointers are device pointers into HBM.
-// Particle state is a structure of arrays indexed by slot p in [0, cap);
-// per-coordinate-level fields are stored level-major ([level * cap + p]) and
-// per-material-nuclide microscopic totals nuclide-major ([j * cap + p]), so a
-// wavefront's 64 lanes touch 64 consecutive elements of every field.
+// Particle state is indexed by slot p in [0, cap): one 128-byte hot record
+// per particle, one 64-byte record per deeper coordinate level, and the
+// per-nuclide microscopic totals of the particle's material, contiguous per
+// particle.  Event kernels visit particles in queue order, which is not slot
+// order, so each particle's state is kept in as few cache lines as possible.
 #pragma once
 
 #include <stdint.h>
@@ -13,6 +14,8 @@
 namespace omc {
 
 constexpr int MAX_LEVELS = 4;
+constexpr int SORT_MAX_BINS = 2048;  // (material + 1, energy bucket) keys
+constexpr int SORT_MAX_BLOCKS = 256; // workgroups of the sort kernels
 
 // Constants of the reference (include/openmc/constants.h:49-113)
 constexpr double TINY_BIT = 1e-8;
@@ -172,25 +175,62 @@ enum Counter : int {
 // Queue counters (device, int32)
 enum QueueCount : int { QC_XS = 0, QC_ADV, QC_SURF, QC_COLL, QC_N };
 
+// Hot state of one particle: exactly one 128-byte L2 cache line, so an event
+// kernel touching a particle in any queue order moves one line, not one line
+// per field.  The direction is shared by every coordinate level (no rotated
+// fills on this path: find_cell_inner copies it down unchanged,
+// src/geometry.cpp:196-197, and collisions copy it down, src/particle.cpp:396-406).
+struct alignas(128) PHot {
+  double rx, ry, rz; // level-0 position
+  double ux, uy, uz; // direction
+  double E;
+  double wgt;
+  double xs_t; // macroscopic total
+  double xs_a; // macroscopic absorption
+  double time;
+  uint64_t seed; // tracking stream
+  int cell0;     // level-0 cell (its universe is the root universe)
+  int material;
+  int material_last;
+  int n_coord;
+  int surface;   // signed surface index + 1
+  int b_surface; // boundary of the last advance
+  int b_info;    // coord level | lattice translation << 8
+  int n_event;
+};
+static_assert(sizeof(PHot) == 128, "one cache line per particle");
+
+// Coordinate level >= 1 of one particle (64 bytes).
+struct alignas(64) PLevel {
+  double rx, ry, rz;
+  int cell, univ, lat;
+  int lix, liy, liz;
+  int pad[4];
+};
+static_assert(sizeof(PLevel) == 64, "half a cache line per level");
+
 struct DevParticles {
   int64_t cap;
-  // per level [MAX_LEVELS * cap]
-  double *rx, *ry, *rz, *ux, *uy, *uz;
-  int *cell, *univ, *lat, *lix, *liy, *liz;
-  // per particle
-  int* n_coord;
-  double *E, *wgt, *time;
-  uint64_t* seed;
-  int *surface, *material, *material_last;
-  double *xs_t, *xs_a;
-  double* micro_t; // [max_mat_nuc * cap]
-  int *b_surface, *b_info;
-  int* n_event;
+  PHot* hot;      // [cap]
+  PLevel* lev;    // [cap][MAX_LEVELS - 1]
+  double* micro_t; // [cap][max_mat_nuc]: per-nuclide totals for nuclide sampling
+  int max_nn;
   int* lost;
   uint64_t* hash;
   double* leak; // per-batch global leakage accumulator [1]
   // queues
   int *q_xs, *q_surf, *q_coll;
+  // counting sort of the calculate-xs queue by (material, energy bucket):
+  // producers append each particle's key beside it in q_key; sort_count is
+  // a [sort_nb][SORT_MAX_BLOCKS] matrix of per-workgroup bin counts
+  int* q_key;
+  int* q_sorted;
+  int* sort_count;
+  int* sort_total; // [sort_nb]: bin totals, then bin starts
+  int sort_nb;
+  int sort_shift; // energy bucket = (bits(E) >> sort_shift) - sort_base
+  int sort_base;
+  int sort_eb;    // energy buckets per material
   int* qcount;        // [QC_N]
   unsigned long long* counters; // [CNT_N]
   int trace_hash;

@@ -25,7 +25,7 @@ __device__ __forceinline__ double prn(uint64_t* seed)
   return __builtin_ldexp(static_cast<double>(result), -64);
 }
 
-__host__ __device__ inline uint64_t future_seed(uint64_t n, uint64_t seed)
+__host__ __device__ __forceinline__ uint64_t future_seed(uint64_t n, uint64_t seed)
 {
   uint64_t g = 6364136223846793005ULL;
   uint64_t c = 1442695040888963407ULL;
@@ -66,7 +66,7 @@ __device__ __forceinline__ double v_norm(V3 a) { return sqrt(a.x * a.x + a.y * a
 __device__ __forceinline__ double v_get(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
 // src/math_functions.cpp:763-792
-__device__ inline V3 rotate_angle(V3 u, double mu, uint64_t* seed)
+__device__ __forceinline__ V3 rotate_angle(V3 u, double mu, uint64_t* seed)
 {
   double phi_ = 2.0 * PI * prn(seed);
   double sinphi, cosphi;
@@ -84,36 +84,56 @@ __device__ inline V3 rotate_angle(V3 u, double mu, uint64_t* seed)
 }
 
 //==============================================================================
-// Geometry state of one particle held in registers
+// Geometry state of one particle held in registers: NL coordinate levels,
+// one scalar array per component so that every array is small enough to live
+// in VGPRs (no scratch) once the level loops are unrolled.
 //==============================================================================
 
+// The direction is one vector for all levels: the reference copies it down
+// unchanged when descending (src/geometry.cpp:196-197, 216-217) and after
+// collisions (src/particle.cpp:396-406), and no cell on this path is rotated.
+template<int NL>
 struct Geo {
-  V3 r[MAX_LEVELS];
-  V3 u[MAX_LEVELS];
-  int cell[MAX_LEVELS];
-  int univ[MAX_LEVELS];
-  int lat[MAX_LEVELS];
-  int li[MAX_LEVELS][3];
+  double rx[NL], ry[NL], rz[NL];
+  double ux, uy, uz;
+  int cell[NL], univ[NL], lat[NL];
+  int lix[NL], liy[NL], liz[NL];
   int n_coord;
   int surface;
   int material;
   int material_last;
   int lost;
-};
 
-__device__ __forceinline__ void coord_reset(Geo& g, int i)
-{
-  g.cell[i] = C_NONE;
-  g.univ[i] = C_NONE;
-  g.lat[i] = C_NONE;
-  g.li[i][0] = g.li[i][1] = g.li[i][2] = 0;
-}
+  __device__ __forceinline__ V3 r(int i) const { return {rx[i], ry[i], rz[i]}; }
+  __device__ __forceinline__ V3 u() const { return {ux, uy, uz}; }
+  __device__ __forceinline__ void set_r(int i, V3 v)
+  {
+    rx[i] = v.x;
+    ry[i] = v.y;
+    rz[i] = v.z;
+  }
+  __device__ __forceinline__ void set_u(V3 v)
+  {
+    ux = v.x;
+    uy = v.y;
+    uz = v.z;
+  }
+  __device__ __forceinline__ void reset(int i)
+  {
+    cell[i] = C_NONE;
+    univ[i] = C_NONE;
+    lat[i] = C_NONE;
+    lix[i] = 0;
+    liy[i] = 0;
+    liz[i] = 0;
+  }
+};
 
 //==============================================================================
 // Surfaces (src/surface.cpp)
 //==============================================================================
 
-__device__ inline double surf_evaluate(const DevSurface& s, V3 r)
+__device__ __forceinline__ double surf_evaluate(const DevSurface& s, V3 r)
 {
   switch (s.type) {
   case S_XPLANE: return r.x - s.c[0];
@@ -157,7 +177,7 @@ __device__ __forceinline__ double plane_distance(double ri, double ui, bool coin
 }
 
 // axis_aligned_cylinder_distance (src/surface.cpp:403-442)
-__device__ inline double cyl_distance(double u1, double r2p, double u2, double r3p, double u3,
+__device__ __forceinline__ double cyl_distance(double u1, double r2p, double u2, double r3p, double u3,
   bool coincident, double off1, double off2, double radius)
 {
   const double a = 1.0 - u1 * u1;
@@ -184,7 +204,7 @@ __device__ inline double cyl_distance(double u1, double r2p, double u2, double r
   }
 }
 
-__device__ inline double surf_distance(const DevSurface& s, V3 r, V3 u, bool coincident)
+__device__ __forceinline__ double surf_distance(const DevSurface& s, V3 r, V3 u, bool coincident)
 {
   switch (s.type) {
   case S_XPLANE: return plane_distance(r.x, u.x, coincident, s.c[0]);
@@ -228,7 +248,7 @@ __device__ inline double surf_distance(const DevSurface& s, V3 r, V3 u, bool coi
   }
 }
 
-__device__ inline V3 surf_normal(const DevSurface& s, V3 r)
+__device__ __forceinline__ V3 surf_normal(const DevSurface& s, V3 r)
 {
   switch (s.type) {
   case S_XPLANE: return {1., 0., 0.};
@@ -243,7 +263,7 @@ __device__ inline V3 surf_normal(const DevSurface& s, V3 r)
 }
 
 // Surface::sense (src/surface.cpp:116-130)
-__device__ inline bool surf_sense(const DevSurface& s, V3 r, V3 u)
+__device__ __forceinline__ bool surf_sense(const DevSurface& s, V3 r, V3 u)
 {
   const double f = surf_evaluate(s, r);
   if (fabs(f) < FP_COINCIDENT)
@@ -256,7 +276,7 @@ __device__ inline bool surf_sense(const DevSurface& s, V3 r, V3 u)
 //==============================================================================
 
 // Region::contains_simple (src/cell.cpp:951-970)
-__device__ inline bool cell_contains(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface)
+__device__ __forceinline__ bool cell_contains(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface)
 {
   for (int k = c.tok_begin; k < c.tok_end; ++k) {
     int token = P.tokens[k];
@@ -273,7 +293,7 @@ __device__ inline bool cell_contains(const DevProblem& P, const DevCell& c, V3 r
 }
 
 // Region::distance (src/cell.cpp:910-936)
-__device__ inline double cell_distance(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface,
+__device__ __forceinline__ double cell_distance(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface,
   int* i_surf)
 {
   double min_dist = INFTY;
@@ -294,7 +314,7 @@ __device__ inline double cell_distance(const DevProblem& P, const DevCell& c, V3
 }
 
 // RectLattice::distance (src/lattice.cpp:252-306)
-__device__ inline double lattice_distance(const DevLattice& L, V3 r, V3 u, int* trans)
+__device__ __forceinline__ double lattice_distance(const DevLattice& L, V3 r, V3 u, int* trans)
 {
   double x0 = copysign(0.5 * L.pitch[0], u.x);
   double y0 = copysign(0.5 * L.pitch[1], u.y);
@@ -331,13 +351,15 @@ __device__ inline double lattice_distance(const DevLattice& L, V3 r, V3 u, int* 
 }
 
 // RectLattice::get_indices (src/lattice.cpp:310-342)
-__device__ inline void lattice_indices(const DevLattice& L, V3 r, V3 u, int* res)
+__device__ __forceinline__ void lattice_indices(const DevLattice& L, V3 r, V3 u, int* res)
 {
-  double rr[3] = {r.x, r.y, r.z};
-  double uu[3] = {u.x, u.y, u.z};
+  const double rr[3] = {r.x, r.y, r.z};
+  const double uu[3] = {u.x, u.y, u.z};
   res[2] = 0;
-  int nd = L.is_3d ? 3 : 2;
-  for (int k = 0; k < nd; ++k) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (k == 2 && !L.is_3d)
+      break;
     double i_ = (rr[k] - L.ll[k]) / L.pitch[k];
     long i_close = detm::lround(i_);
     if (fabs(i_ - static_cast<double>(i_close)) < FP_COINCIDENT) {
@@ -349,7 +371,7 @@ __device__ inline void lattice_indices(const DevLattice& L, V3 r, V3 u, int* res
 }
 
 // RectLattice::get_local_position (src/lattice.cpp:352-361)
-__device__ inline V3 lattice_local(const DevLattice& L, V3 r, const int* i)
+__device__ __forceinline__ V3 lattice_local(const DevLattice& L, V3 r, const int* i)
 {
   r.x -= (L.ll[0] + (i[0] + 0.5) * L.pitch[0]);
   r.y -= (L.ll[1] + (i[1] + 0.5) * L.pitch[1]);
@@ -369,16 +391,39 @@ __device__ __forceinline__ int lattice_universe(const DevProblem& P, const DevLa
   return P.lat_univ[L.univ_offset + L.n[0] * L.n[1] * i[2] + L.n[0] * i[1] + i[0]];
 }
 
-// Universe::find_cell (src/universe.cpp:40-60)
-__device__ inline bool universe_find_cell(const DevProblem& P, Geo& g)
+// Selects of the lowest coordinate level with static indices (the level
+// arrays stay in registers).
+template<int NL>
+__device__ __forceinline__ int lowest_cell(const Geo<NL>& g)
 {
-  int lvl = g.n_coord - 1;
-  int i_univ = g.univ[lvl];
-  const DevUniverse U = P.univ[i_univ];
-  V3 r = g.r[lvl];
-  V3 u = g.u[lvl];
+  int c = g.cell[0];
+#pragma unroll
+  for (int l = 1; l < NL; ++l)
+    if (l == g.n_coord - 1)
+      c = g.cell[l];
+  return c;
+}
+
+template<int NL>
+__device__ __forceinline__ int lowest_univ(const Geo<NL>& g)
+{
+  int u = g.univ[0];
+#pragma unroll
+  for (int l = 1; l < NL; ++l)
+    if (l == g.n_coord - 1)
+      u = g.univ[l];
+  return u;
+}
+
+// Universe::find_cell (src/universe.cpp:40-60) at coordinate level lvl
+template<int NL>
+__device__ __forceinline__ bool universe_find_cell(const DevProblem& P, Geo<NL>& g, const int lvl)
+{
+  const DevUniverse U = P.univ[g.univ[lvl]];
+  const V3 r = g.r(lvl);
+  const V3 u = g.u();
   for (int k = U.cell_begin; k < U.cell_end; ++k) {
-    int i_cell = P.univ_cells[k];
+    const int i_cell = P.univ_cells[k];
     if (cell_contains(P, P.cell[i_cell], r, u, g.surface)) {
       g.cell[lvl] = i_cell;
       return true;
@@ -390,37 +435,39 @@ __device__ inline bool universe_find_cell(const DevProblem& P, Geo& g)
 // find_cell_inner (src/geometry.cpp:102-252), searching the whole universe at
 // each level (the neighbor lists of the reference visit a subset of the same
 // universe and, for universes whose cells partition space, find the same cell)
-__device__ inline bool find_cell_inner(const DevProblem& P, Geo& g)
+template<int NL>
+__device__ __forceinline__ bool find_cell_inner(const DevProblem& P, Geo<NL>& g)
 {
-  for (;; ++g.n_coord) {
-    if (g.n_coord > MAX_LEVELS)
+  // Iteration `lvl` handles coordinate level lvl = n_coord - 1 (levels above
+  // the starting n_coord are skipped), so every array index is static.
+#pragma unroll
+  for (int lvl = 0; lvl < NL; ++lvl) {
+    if (lvl < g.n_coord - 1)
+      continue;
+    if (!universe_find_cell(P, g, lvl))
       return false;
-    if (!universe_find_cell(P, g))
-      return false;
-    int lvl = g.n_coord - 1;
     const DevCell c = P.cell[g.cell[lvl]];
     if (c.fill_type == FILL_MATERIAL) {
       g.material_last = g.material;
       g.material = c.material;
       return true;
     }
-    if (g.n_coord >= MAX_LEVELS)
+    if (lvl + 1 >= NL)
       return false;
-    int nl = g.n_coord;
+    const int nl = lvl + 1;
+    const V3 r = g.r(lvl);
+    const V3 u = g.u();
     if (c.fill_type == FILL_UNIVERSE) {
       g.univ[nl] = c.fill;
-      g.r[nl] = g.r[lvl];
-      g.u[nl] = g.u[lvl];
+      g.set_r(nl, r);
     } else {
       const DevLattice L = P.lat[c.fill];
-      g.r[nl] = g.r[lvl];
-      g.u[nl] = g.u[lvl];
       int idx[3];
-      lattice_indices(L, g.r[nl], g.u[nl], idx);
-      g.li[nl][0] = idx[0];
-      g.li[nl][1] = idx[1];
-      g.li[nl][2] = idx[2];
-      g.r[nl] = lattice_local(L, g.r[nl], idx);
+      lattice_indices(L, r, u, idx);
+      g.lix[nl] = idx[0];
+      g.liy[nl] = idx[1];
+      g.liz[nl] = idx[2];
+      g.set_r(nl, lattice_local(L, r, idx));
       g.lat[nl] = c.fill;
       if (lattice_valid(L, idx)) {
         g.univ[nl] = lattice_universe(P, L, idx);
@@ -431,26 +478,34 @@ __device__ inline bool find_cell_inner(const DevProblem& P, Geo& g)
         return false;
       }
     }
+    ++g.n_coord;
   }
+  return false;
 }
 
 // exhaustive_find_cell (src/geometry.cpp:284-297)
-__device__ inline bool exhaustive_find_cell(const DevProblem& P, Geo& g)
+template<int NL>
+__device__ __forceinline__ bool exhaustive_find_cell(const DevProblem& P, Geo<NL>& g)
 {
-  if (g.univ[g.n_coord - 1] == C_NONE) {
+  if (lowest_univ(g) == C_NONE) {
     g.univ[0] = P.root_universe;
     g.n_coord = 1;
   }
-  for (int i = g.n_coord; i < MAX_LEVELS; ++i)
-    coord_reset(g, i);
+#pragma unroll
+  for (int i = 1; i < NL; ++i)
+    if (i >= g.n_coord)
+      g.reset(i);
   return find_cell_inner(P, g);
 }
 
 // neighbor_list_find_cell (src/geometry.cpp:256-282)
-__device__ inline bool neighbor_list_find_cell(const DevProblem& P, Geo& g)
+template<int NL>
+__device__ __forceinline__ bool neighbor_list_find_cell(const DevProblem& P, Geo<NL>& g)
 {
-  for (int i = g.n_coord; i < MAX_LEVELS; ++i)
-    coord_reset(g, i);
+#pragma unroll
+  for (int i = 1; i < NL; ++i)
+    if (i >= g.n_coord)
+      g.reset(i);
   return find_cell_inner(P, g);
 }
 
@@ -462,7 +517,8 @@ struct Boundary {
 };
 
 // distance_to_boundary (src/geometry.cpp:361-453)
-__device__ inline Boundary distance_to_boundary(const DevProblem& P, Geo& g)
+template<int NL>
+__device__ __forceinline__ Boundary distance_to_boundary(const DevProblem& P, Geo<NL>& g)
 {
   Boundary info;
   info.distance = __builtin_inf();
@@ -471,9 +527,12 @@ __device__ inline Boundary distance_to_boundary(const DevProblem& P, Geo& g)
   info.trans[0] = info.trans[1] = info.trans[2] = 0;
   double d_lat = __builtin_inf();
   int level_lat_trans[3] = {0, 0, 0};
-  for (int i = 0; i < g.n_coord; i++) {
-    V3 r = g.r[i];
-    V3 u = g.u[i];
+#pragma unroll
+  for (int i = 0; i < NL; i++) {
+    if (i >= g.n_coord)
+      break;
+    const V3 r = g.r(i);
+    const V3 u = g.u();
     int level_surf_cross;
     double d_surf = cell_distance(P, P.cell[g.cell[i]], r, u, g.surface, &level_surf_cross);
     if (g.lat[i] != C_NONE) {
@@ -504,28 +563,39 @@ __device__ inline Boundary distance_to_boundary(const DevProblem& P, Geo& g)
 }
 
 // cross_lattice (src/geometry.cpp:301-357)
-__device__ inline void cross_lattice(const DevProblem& P, Geo& g, const int* trans)
+template<int NL>
+__device__ __forceinline__ void cross_lattice(const DevProblem& P, Geo<NL>& g, const int* trans)
 {
-  int lvl = g.n_coord - 1;
-  if (lvl < 1 || g.lat[lvl] == C_NONE) {
+  bool valid = false;
+  bool done = false;
+#pragma unroll
+  for (int lvl = 1; lvl < NL; ++lvl) {
+    if (lvl != g.n_coord - 1)
+      continue;
+    if (g.lat[lvl] == C_NONE)
+      break;
+    done = true;
+    const DevLattice L = P.lat[g.lat[lvl]];
+    int idx[3] = {g.lix[lvl] + trans[0], g.liy[lvl] + trans[1], g.liz[lvl] + trans[2]};
+    g.lix[lvl] = idx[0];
+    g.liy[lvl] = idx[1];
+    g.liz[lvl] = idx[2];
+    g.set_r(lvl, lattice_local(L, g.r(lvl - 1), idx));
+    valid = lattice_valid(L, idx);
+    if (valid)
+      g.univ[lvl] = lattice_universe(P, L, idx);
+  }
+  if (!done) {
     // a lattice translation selected at a level without a lattice; the
     // reference would index its lattice array with -1 here
     g.lost = 1;
     return;
   }
-  const DevLattice L = P.lat[g.lat[lvl]];
-  g.li[lvl][0] += trans[0];
-  g.li[lvl][1] += trans[1];
-  g.li[lvl][2] += trans[2];
-  V3 r = g.r[lvl - 1];
-  int idx[3] = {g.li[lvl][0], g.li[lvl][1], g.li[lvl][2]};
-  g.r[lvl] = lattice_local(L, r, idx);
-  if (!lattice_valid(L, idx)) {
+  if (!valid) {
     g.n_coord = 1;
     if (!exhaustive_find_cell(P, g))
       g.lost = 1;
   } else {
-    g.univ[lvl] = lattice_universe(P, L, idx);
     if (!exhaustive_find_cell(P, g)) {
       g.n_coord = 1;
       if (!exhaustive_find_cell(P, g))
@@ -578,7 +648,7 @@ __device__ __forceinline__ int lower_bound_idx(const double* E, int first, int c
 // evaluated with this same FP32 arithmetic) guarantees that the global
 // lower_bound of E lies in [j - err - 1, j + err + 3), which is then searched
 // exactly.  Returns the global lower_bound index (first i with E[i] >= E).
-__device__ inline int learned_lower_bound(const DevProblem& P, int i_nuc, const double* Eg, int n,
+__device__ __forceinline__ int learned_lower_bound(const DevProblem& P, int i_nuc, const double* Eg, int n,
   double E, double lx, int& probes)
 {
   const DevLearned& Lm = P.learned;
@@ -606,7 +676,7 @@ struct Micro {
 };
 
 // Nuclide::calculate_xs, NEAREST temperature with one temperature
-__device__ inline Micro nuclide_xs(const DevProblem& P, int i_nuc, double E, int i_log, double logE,
+__device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, double E, int i_log, double logE,
   bool learned, int& probes)
 {
   const DevNuclide nd = P.nuc[i_nuc];
@@ -658,6 +728,145 @@ __device__ inline Micro nuclide_xs(const DevProblem& P, int i_nuc, double E, int
   return m;
 }
 
+// K lookups of one energy in K nuclides, interleaved: every phase issues the
+// K independent loads before it uses any of them, and the K bounded binary
+// searches run in lockstep, so one lane keeps up to K cache misses in flight
+// instead of one.  Each result equals nuclide_xs for the same nuclide:
+//  - the bucket search's special case (E equal to the first point of its
+//    bucket gives that point, include/openmc/search.h:17-18) is applied after
+//    the search: lower_bound over [i_low, i_high) returns i_low exactly when
+//    E[i_low] >= E, so E[i_low] == E implies the lower bound is i_low, and
+//    that point only needs reading when the lower bound is i_low;
+//  - the duplicate-point bump reads the next point only when it applies.
+// Entries k >= kk are not computed (their outputs are left unchanged).
+template<int K>
+__device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int* nuc_list, int kk, double E,
+  int i_log, double logE, bool learned, int& probes, Micro* out)
+{
+  const double* Eg[K];
+  int64_t off[K];
+  int n[K], fst[K], cnt[K], lo0[K], ig[K];
+  bool search[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i_nuc = nuc_list[k < kk ? k : 0];
+    const DevNuclide nd = P.nuc[i_nuc];
+    off[k] = nd.off;
+    n[k] = nd.n;
+    Eg[k] = P.E + nd.off;
+  }
+  // bracket by the log-grid bucket (or the learned window)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i_nuc = nuc_list[k < kk ? k : 0];
+    const double e_first = Eg[k][0];
+    const double e_last = Eg[k][n[k] - 1];
+    const int* gi = P.grid_index + static_cast<int64_t>(i_nuc) * (P.n_log_bins + 1);
+    int lo, hi;
+    if (!learned) {
+      lo = gi[i_log];
+      hi = gi[i_log + 1] + 1;
+    } else {
+      const DevLearned& Lm = P.learned;
+      double uu = (logE - Lm.u_lo[i_nuc]) * Lm.inv_w[i_nuc];
+      int s = uu < 0.0 ? 0 : static_cast<int>(uu);
+      s = s >= Lm.n_seg ? Lm.n_seg - 1 : s;
+      const int64_t sk = static_cast<int64_t>(i_nuc) * Lm.n_seg + s;
+      const float2 sg = reinterpret_cast<const float2*>(Lm.seg)[sk];
+      const int err = Lm.err[sk];
+      float fr = static_cast<float>(uu - static_cast<double>(s));
+      float pred = sg.x * fr + sg.y;
+      int j = static_cast<int>(pred);
+      lo = j - err - 1;
+      hi = j + err + 3;
+      lo = lo < 0 ? 0 : (lo > n[k] ? n[k] : lo);
+      hi = hi > n[k] ? n[k] : (hi < lo ? lo : hi);
+    }
+    const bool below = E < e_first;
+    const bool above = !below && E > e_last;
+    search[k] = k < kk && !below && !above;
+    ig[k] = below ? 0 : n[k] - 2;
+    lo0[k] = lo;
+    fst[k] = lo;
+    cnt[k] = search[k] ? hi - lo : 0;
+  }
+  // lockstep lower_bound searches
+  while (true) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      any = any || cnt[k] > 0;
+    if (!any)
+      break;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      v[k] = cnt[k] > 0 ? Eg[k][fst[k] + (cnt[k] >> 1)] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (cnt[k] > 0) {
+        ++probes;
+        const int step = cnt[k] >> 1;
+        if (v[k] < E) {
+          fst[k] += step + 1;
+          cnt[k] -= step + 1;
+        } else {
+          cnt[k] = step;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!search[k])
+      continue;
+    const int lb = fst[k];
+    if (!learned) {
+      ig[k] = lb - 1;
+      if (lb == lo0[k] && Eg[k][lb] == E)
+        ig[k] = lb;
+    } else {
+      ig[k] = lb - 1;
+      if (lb < n[k] && Eg[k][lb] == E) {
+        const int i_nuc = nuc_list[k];
+        const int il = P.grid_index[static_cast<int64_t>(i_nuc) * (P.n_log_bins + 1) + i_log];
+        if (Eg[k][il] == E)
+          ig[k] = il;
+      }
+    }
+  }
+  double e0[K], e1[K];
+  double2 x0[K], x1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    int i = ig[k];
+    i = i < 0 ? 0 : (i > n[k] - 2 ? n[k] - 2 : i);
+    ig[k] = i;
+    e0[k] = Eg[k][i];
+    e1[k] = Eg[k][i + 1];
+    x0[k] = P.XS[off[k] + i];
+    x1[k] = P.XS[off[k] + i + 1];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k >= kk)
+      continue;
+    int i = ig[k];
+    if (i + 2 < n[k] && e0[k] == e1[k]) {
+      ++i;
+      e0[k] = e1[k];
+      e1[k] = Eg[k][i + 1];
+      x0[k] = x1[k];
+      x1[k] = P.XS[off[k] + i + 1];
+    }
+    const double f = (E - e0[k]) / (e1[k] - e0[k]);
+    out[k].i_grid = i;
+    out[k].f = f;
+    out[k].total = (1.0 - f) * x0[k].x + f * x1[k].x;
+    out[k].absorption = (1.0 - f) * x0[k].y + f * x1[k].y;
+  }
+}
+
 //==============================================================================
 // Tally: RegularMesh ray-march (src/mesh.cpp:1000-1146, 1328-1369)
 //==============================================================================
@@ -668,7 +877,7 @@ struct MeshDist {
   double distance;
 };
 
-__device__ inline MeshDist mesh_distance(const DevTally& T, const int* ijk, int i, V3 r0, V3 u)
+__device__ __forceinline__ MeshDist mesh_distance(const DevTally& T, const int* ijk, int i, V3 r0, V3 u)
 {
   MeshDist d;
   d.next_index = ijk[i];
@@ -688,7 +897,7 @@ __device__ inline MeshDist mesh_distance(const DevTally& T, const int* ijk, int 
   return d;
 }
 
-__device__ inline void mesh_indices(const DevTally& T, V3 r, int* ijk, bool& in_mesh)
+__device__ __forceinline__ void mesh_indices(const DevTally& T, V3 r, int* ijk, bool& in_mesh)
 {
   in_mesh = true;
   for (int i = 0; i < T.n_dim; ++i) {
@@ -715,7 +924,7 @@ __device__ __forceinline__ void tally_add(const DevTally& T, int64_t bin, double
 
 // score_tracklength_tally with [mesh] or [mesh, energy] filters
 // (src/tallies/tally_scoring.cpp:2416-2541); returns segments scored
-__device__ inline int score_tracklength(const DevTally& T, V3 r0, V3 r1, V3 u, double E, double wgt,
+__device__ __forceinline__ int score_tracklength(const DevTally& T, V3 r0, V3 r1, V3 u, double E, double wgt,
   double distance)
 {
   double flux = wgt * distance;

@@ -1,4 +1,4 @@
-// Host-callable launchers of the event kernels (transport.hip).
+// Host-callable launchers of the event kernels (transport.hip, xs_lookup.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,9 +7,26 @@
 
 namespace omc {
 
+enum KernelKind : int {
+  KERNEL_SORT = 0,
+  KERNEL_XS = 1,
+  KERNEL_ADVANCE = 2,
+  KERNEL_SURFACE = 3,
+  KERNEL_COLLISION = 4,
+  KERNEL_INIT = 5
+};
+
+struct LaunchConfig {
+  int blocks;   // workgroups of the grid-stride event kernels
+  int xs_lanes; // lanes per particle in the calculate-xs kernel (1..64, power of two)
+  int sort;     // counting-sort the calculate-xs queue each cycle
+  int sort_blocks; // workgroups of the histogram and scatter kernels
+  int xs_group;    // nuclide lookups interleaved per lane (1, 2, 4 or 8) when xs_lanes == 1
+};
+
 void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st);
-void launch_cycle(const DevProblem& P, const DevParticles& S, int blocks, hipStream_t st);
-void launch_kernel(int which, const DevProblem& P, const DevParticles& S, int blocks, hipStream_t st);
+void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st);
+void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st);
 void launch_accumulate(double* value, double* results, int64_t n_bins, double norm, hipStream_t st);
 
 // Standalone batched lookups (xs_lookup.hip): E[n] -> i_grid[n], xs[n][2]
@@ -17,7 +34,6 @@ void launch_lookup_batch(const DevProblem& P, int i_nuc, const double* E, int64_
   double* xs, unsigned long long* probes, hipStream_t st);
 void launch_macro_batch(const DevProblem& P, int material, const double* E, int64_t n, int learned,
   double* macro, unsigned long long* probes, hipStream_t st);
-
 void launch_copy(const void* src, void* dst, int64_t n_bytes, hipStream_t st);
 void launch_math_batch(int fn, const double* x, const double* y, int64_t n, double* out, hipStream_t st);
 

@@ -1,22 +1,35 @@
 // Event kernels of the GPU transport loop (gfx950).
 //
 // One batch runs as: k_init (source sampling, src/simulation.cpp:573-649),
-// then cycles of  k_xs -> k_advance -> k_surface -> k_collision  until no
-// particle is alive.  The calculate-xs queue doubles as the advance queue
-// (every particle that computes cross sections advances next,
+// then cycles of
+//     [k_sort_count -> k_sort_scan -> k_sort_scatter] -> k_xs -> k_advance
+//       -> k_surface -> k_collision
+// until no particle is alive.  The calculate-xs queue doubles as the advance
+// queue (every particle that computes cross sections advances next,
 // src/event.cpp:87-101); advance splits it into the surface-crossing and
 // collision queues; both of those refill the calculate-xs queue with the
 // survivors.  Appends are compacted per wavefront: a ballot of the lanes that
 // append, one atomicAdd per wavefront for the base slot, and each lane's rank
 // among the set lanes (popcount of the lower mask bits) as its offset.
 //
+// The optional counting sort orders the calculate-xs queue by (material,
+// energy bucket) -- the order the reference's event.cpp:77-85 comment asks
+// for -- so that neighbouring lanes search neighbouring parts of the same
+// energy grids.  Histories do not depend on the processing order.  Particle
+// state is one 128-byte record per particle (PHot) plus one 64-byte record
+// per deeper coordinate level, so visiting particles in queue order costs one
+// cache line per particle rather than one per field.
+//
 // Queue sizes live in device memory, so a cycle needs no host round trip and
 // a run of cycles can be captured in one hipGraph.  Each counter is reset by
 // a kernel that neither reads nor appends to it:
-//   k_xs       reads XS,   sets ADV = XS, resets SURF and COLL
-//   k_advance  reads ADV,  resets XS, appends SURF / COLL
-//   k_surface  reads SURF, appends XS
+//   k_xs        reads XS,   sets ADV = XS, resets SURF and COLL
+//   k_advance   reads ADV,  resets XS, appends SURF / COLL
+//   k_surface   reads SURF, appends XS
 //   k_collision reads COLL, appends XS
+//
+// All kernels are templated on NL, the number of coordinate levels of the
+// model, so that per-level state stays in registers.
 #include <hip/hip_runtime.h>
 
 #include "device_physics.h"
@@ -50,6 +63,42 @@ __device__ __forceinline__ void wave_push(int* q, int* cnt, bool pred, int val)
   }
 }
 
+// Key of a particle about to calculate cross sections, for the counting sort
+// of the calculate-xs queue: (material + 1, energy bucket), the bucket being
+// the sign, exponent and leading mantissa bits of E offset by those of the
+// grid minimum.  The kernel that appends the particle stores it beside the slot.
+__device__ __forceinline__ int sort_key(const DevParticles& S, int material, double E)
+{
+  const int mat = material + 1; // void -> 0
+  const uint64_t bits = static_cast<uint64_t>(__double_as_longlong(E));
+  int eb = static_cast<int>(bits >> S.sort_shift) - S.sort_base;
+  eb = eb < 0 ? 0 : (eb >= S.sort_eb ? S.sort_eb - 1 : eb);
+  int key = mat * S.sort_eb + eb;
+  key = key < S.sort_nb ? key : S.sort_nb - 1;
+  return key < 0 ? 0 : key;
+}
+
+// The same append to the calculate-xs queue, with the particle's sort key
+// stored at the same position of q_key.
+__device__ __forceinline__ void wave_push_xs(const DevParticles& S, bool pred, int val, int key)
+{
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0ull)
+    return;
+  int lane = lane_id();
+  int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+  int base = 0;
+  if (lane == leader)
+    base = atomicAdd(&S.qcount[QC_XS], __popcll(mask));
+  base = __shfl(base, leader);
+  if (pred) {
+    unsigned long long below = lane == 0 ? 0ull : (mask & ((1ull << lane) - 1ull));
+    const int at = base + __popcll(below);
+    S.q_xs[at] = val;
+    S.q_key[at] = key;
+  }
+}
+
 // Wavefront sum then one atomic per wavefront.  Every lane must call it.
 __device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned long long v)
 {
@@ -59,84 +108,115 @@ __device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned lon
     atomicAdd(ctr, v);
 }
 
-__device__ inline void load_geo(const DevParticles& S, int nl, int64_t p, Geo& g)
+template<int NL>
+__device__ __forceinline__ PLevel* level_rec(const DevParticles& S, int64_t p, int l)
 {
-  const int64_t cap = S.cap;
-  for (int i = 0; i < MAX_LEVELS; ++i) {
-    if (i < nl) {
-      int64_t k = i * cap + p;
-      g.r[i] = {S.rx[k], S.ry[k], S.rz[k]};
-      g.u[i] = {S.ux[k], S.uy[k], S.uz[k]};
-      g.cell[i] = S.cell[k];
-      g.univ[i] = S.univ[k];
-      g.lat[i] = S.lat[k];
-      g.li[i][0] = S.lix[k];
-      g.li[i][1] = S.liy[k];
-      g.li[i][2] = S.liz[k];
-    } else {
-      g.r[i] = {0, 0, 0};
-      g.u[i] = {0, 0, 0};
-      coord_reset(g, i);
-    }
+  return S.lev + p * (NL > 1 ? NL - 1 : 1) + (l - 1);
+}
+
+// Geometry state from a particle's records (h already loaded by the caller).
+template<int NL>
+__device__ __forceinline__ void load_geo(const DevProblem& P, const DevParticles& S, int64_t p, const PHot& h,
+  Geo<NL>& g)
+{
+  g.rx[0] = h.rx;
+  g.ry[0] = h.ry;
+  g.rz[0] = h.rz;
+  g.ux = h.ux;
+  g.uy = h.uy;
+  g.uz = h.uz;
+  g.cell[0] = h.cell0;
+  g.univ[0] = h.cell0 == C_NONE ? C_NONE : P.root_universe; // level 0 is always the root universe
+  g.lat[0] = C_NONE;
+  g.lix[0] = g.liy[0] = g.liz[0] = 0;
+#pragma unroll
+  for (int l = 1; l < NL; ++l) {
+    const PLevel L = *level_rec<NL>(S, p, l);
+    g.rx[l] = L.rx;
+    g.ry[l] = L.ry;
+    g.rz[l] = L.rz;
+    g.cell[l] = L.cell;
+    g.univ[l] = L.univ;
+    g.lat[l] = L.lat;
+    g.lix[l] = L.lix;
+    g.liy[l] = L.liy;
+    g.liz[l] = L.liz;
   }
-  g.n_coord = S.n_coord[p];
-  g.surface = S.surface[p];
-  g.material = S.material[p];
-  g.material_last = S.material_last[p];
+  g.n_coord = h.n_coord;
+  g.surface = h.surface;
+  g.material = h.material;
+  g.material_last = h.material_last;
   g.lost = 0;
 }
 
-__device__ inline void store_geo(const DevParticles& S, int nl, int64_t p, const Geo& g)
+// Write the geometry fields of a particle back (levels >= 1 in full).
+template<int NL>
+__device__ __forceinline__ void store_geo(const DevParticles& S, int64_t p, const Geo<NL>& g)
 {
-  const int64_t cap = S.cap;
-  for (int i = 0; i < MAX_LEVELS; ++i) {
-    if (i < nl) {
-      int64_t k = i * cap + p;
-      S.rx[k] = g.r[i].x;
-      S.ry[k] = g.r[i].y;
-      S.rz[k] = g.r[i].z;
-      S.ux[k] = g.u[i].x;
-      S.uy[k] = g.u[i].y;
-      S.uz[k] = g.u[i].z;
-      S.cell[k] = g.cell[i];
-      S.univ[k] = g.univ[i];
-      S.lat[k] = g.lat[i];
-      S.lix[k] = g.li[i][0];
-      S.liy[k] = g.li[i][1];
-      S.liz[k] = g.li[i][2];
-    }
+  PHot* h = S.hot + p;
+  h->rx = g.rx[0];
+  h->ry = g.ry[0];
+  h->rz = g.rz[0];
+  h->ux = g.ux;
+  h->uy = g.uy;
+  h->uz = g.uz;
+  h->cell0 = g.cell[0];
+  h->material = g.material;
+  h->material_last = g.material_last;
+  h->n_coord = g.n_coord;
+  h->surface = g.surface;
+#pragma unroll
+  for (int l = 1; l < NL; ++l) {
+    PLevel L;
+    L.rx = g.rx[l];
+    L.ry = g.ry[l];
+    L.rz = g.rz[l];
+    L.cell = g.cell[l];
+    L.univ = g.univ[l];
+    L.lat = g.lat[l];
+    L.lix = g.lix[l];
+    L.liy = g.liy[l];
+    L.liz = g.liz[l];
+    L.pad[0] = L.pad[1] = L.pad[2] = L.pad[3] = 0;
+    *level_rec<NL>(S, p, l) = L;
   }
-  S.n_coord[p] = g.n_coord;
-  S.surface[p] = g.surface;
-  S.material[p] = g.material;
-  S.material_last[p] = g.material_last;
+}
+
+template<int NL>
+__device__ __forceinline__ void geo_clear(Geo<NL>& g, V3 r, V3 u)
+{
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    g.set_r(l, V3 {0, 0, 0});
+    g.reset(l);
+  }
+  g.set_r(0, r);
+  g.set_u(u);
+  g.n_coord = 1;
+  g.surface = SURFACE_NONE;
+  g.material = C_NONE;
+  g.material_last = C_NONE;
+  g.lost = 0;
 }
 
 //==============================================================================
 // Source sampling (src/source.cpp:226-271, 355-417; src/simulation.cpp:573-649)
 //==============================================================================
 
-__device__ inline bool satisfies_spatial_constraints(const DevProblem& P, V3 r)
+template<int NL>
+__device__ __forceinline__ bool satisfies_spatial_constraints(const DevProblem& P, V3 r)
 {
-  Geo g;
-  for (int i = 0; i < MAX_LEVELS; ++i) {
-    g.r[i] = {0, 0, 0};
-    g.u[i] = {0, 0, 0};
-    coord_reset(g, i);
-  }
-  g.r[0] = r;
-  g.u[0] = {0.0, 0.0, 1.0};
-  g.n_coord = 1;
-  g.surface = SURFACE_NONE;
-  g.material = C_NONE;
-  g.material_last = C_NONE;
-  g.lost = 0;
+  Geo<NL> g;
+  geo_clear(g, r, V3 {0.0, 0.0, 1.0});
   if (!exhaustive_find_cell(P, g))
     return false;
   bool accepted = true;
   if (P.src.n_domain > 0) {
-    for (int i = 0; i < g.n_coord; i++) {
-      int id = P.cell[g.cell[i]].id;
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      if (i >= g.n_coord)
+        break;
+      const int id = P.cell[g.cell[i]].id;
       accepted = false;
       for (int k = 0; k < P.src.n_domain; ++k)
         accepted = accepted || (P.src.domain_ids[k] == id);
@@ -148,7 +228,7 @@ __device__ inline bool satisfies_spatial_constraints(const DevProblem& P, V3 r)
 }
 
 // maxwell_spectrum / watt_spectrum (src/random_dist.cpp:20-39)
-__device__ inline double watt_spectrum(double a, double b, uint64_t* seed)
+__device__ __forceinline__ double watt_spectrum(double a, double b, uint64_t* seed)
 {
   double r1 = prn(seed);
   double r2 = prn(seed);
@@ -160,9 +240,10 @@ __device__ inline double watt_spectrum(double a, double b, uint64_t* seed)
 
 constexpr int MAX_SOURCE_TRIES = 1 << 20;
 
+template<int NL>
 __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, int64_t n, int64_t id_base)
 {
-  const int nl = P.n_levels;
+  unsigned long long n_lost = 0;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t id = id_base + i + 1;
@@ -183,7 +264,7 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
         V3 ext = v_sub(ur, ll);
         r = v_add(ll, V3 {x1 * ext.x, x2 * ext.y, x3 * ext.z});
       }
-      accepted = satisfies_spatial_constraints(P, r);
+      accepted = satisfies_spatial_constraints<NL>(P, r);
       if (++tries >= MAX_SOURCE_TRIES && !accepted) {
         lost = true;
         break;
@@ -224,43 +305,270 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
         break;
       }
     }
-    // Particle::from_source (src/particle.cpp:114-158) + initialize_history
-    const int64_t cap = S.cap;
-    for (int l = 0; l < nl; ++l) {
-      int64_t k = l * cap + i;
-      S.cell[k] = C_NONE;
-      S.univ[k] = C_NONE;
-      S.lat[k] = C_NONE;
-      S.lix[k] = 0;
-      S.liy[k] = 0;
-      S.liz[k] = 0;
+    // Particle::from_source (src/particle.cpp:114-158) + initialize_history.
+    // The cell search that the first calculate_xs event performs for a new
+    // particle (src/particle.cpp:180-196) is done here instead: it reads only
+    // the position and direction set above, draws no random number, and
+    // leaves the same coordinate levels, material and material_last.
+    Geo<NL> g;
+    geo_clear(g, r, u);
+    if (!lost) {
+      if (!exhaustive_find_cell(P, g) || g.lost)
+        lost = true;
     }
-    S.rx[i] = r.x;
-    S.ry[i] = r.y;
-    S.rz[i] = r.z;
-    S.ux[i] = u.x;
-    S.uy[i] = u.y;
-    S.uz[i] = u.z;
-    S.n_coord[i] = 1;
-    S.E[i] = E;
-    S.wgt[i] = lost ? 0.0 : 1.0;
-    S.time[i] = 0.0;
-    S.seed[i] = future_seed(static_cast<uint64_t>(id) * P.stride, P.master_seed + 0);
-    S.surface[i] = SURFACE_NONE;
-    S.material[i] = C_NONE;
-    S.material_last[i] = C_NONE;
-    S.n_event[i] = 0;
+    PHot h;
+    h.rx = g.rx[0];
+    h.ry = g.ry[0];
+    h.rz = g.rz[0];
+    h.ux = u.x;
+    h.uy = u.y;
+    h.uz = u.z;
+    h.E = E;
+    h.wgt = lost ? 0.0 : 1.0;
+    h.xs_t = 0.0;
+    h.xs_a = 0.0;
+    h.time = 0.0;
+    h.seed = future_seed(static_cast<uint64_t>(id) * P.stride, P.master_seed + 0);
+    h.cell0 = g.cell[0];
+    h.material = g.material;
+    h.material_last = g.material_last;
+    h.n_coord = g.n_coord;
+    h.surface = SURFACE_NONE;
+    h.b_surface = SURFACE_NONE;
+    h.b_info = 0;
+    h.n_event = 0;
+    S.hot[i] = h;
+    store_geo<NL>(S, i, g);
     S.lost[i] = lost ? 1 : 0;
+    n_lost += lost ? 1 : 0;
     S.hash[i] = 0xcbf29ce484222325ULL ^ static_cast<uint64_t>(id);
     S.q_xs[i] = static_cast<int>(i);
+    S.q_key[i] = sort_key(S, g.material, E);
+  }
+  wave_count(&S.counters[CNT_LOST], n_lost);
+}
+
+//==============================================================================
+// Counting sort of the calculate-xs queue by (material, energy bucket)
+//==============================================================================
+
+// The sort of a queue of n particles uses nw(n) = clamp(ceil(n / 2048), 1,
+// max_blocks) workgroups (the launch has max_blocks; the others return at
+// once), so short queues late in a batch scan a small count matrix.
+// Workgroup b owns the contiguous chunk [b * chunk, min(n, (b + 1) * chunk)).
+constexpr int SORT_MIN_PER_BLOCK = 2048;
+
+__device__ __forceinline__ int sort_workgroups(int n, int max_blocks)
+{
+  int w = (n + SORT_MIN_PER_BLOCK - 1) / SORT_MIN_PER_BLOCK;
+  return w < 1 ? 1 : (w > max_blocks ? max_blocks : w);
+}
+
+// Per-workgroup bin counts of its chunk into sort_count[bin][b].
+__global__ void __launch_bounds__(BLOCK) k_sort_count(DevParticles S)
+{
+  __shared__ int sh_cnt[SORT_MAX_BINS];
+  const int n = S.qcount[QC_XS];
+  const int nw = sort_workgroups(n, gridDim.x);
+  if (static_cast<int>(blockIdx.x) >= nw)
+    return;
+  const int nb = S.sort_nb;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x)
+    sh_cnt[k] = 0;
+  __syncthreads();
+  const int chunk = (n + nw - 1) / nw;
+  const int lo = blockIdx.x * chunk;
+  const int hi = min(n, lo + chunk);
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    atomicAdd(&sh_cnt[S.q_key[i]], 1);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nb; k += blockDim.x)
+    S.sort_count[k * SORT_MAX_BLOCKS + blockIdx.x] = sh_cnt[k];
+}
+
+// One thread per bin: exclusive scan of the bin's row over the workgroups
+// (sort_count[bin][b] becomes the offset of workgroup b inside the bin) and
+// the bin total.
+__global__ void __launch_bounds__(BLOCK) k_sort_rows(DevParticles S, int max_blocks)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S.sort_nb)
+    return;
+  const int nw = sort_workgroups(S.qcount[QC_XS], max_blocks);
+  int* row = S.sort_count + static_cast<int64_t>(k) * SORT_MAX_BLOCKS;
+  int run = 0;
+  for (int b = 0; b < nw; ++b) {
+    const int v = row[b];
+    row[b] = run;
+    run += v;
+  }
+  S.sort_total[k] = run;
+}
+
+// One workgroup: exclusive scan of the bin totals into bin starts.
+__global__ void __launch_bounds__(1024) k_sort_scan(DevParticles S)
+{
+  __shared__ int sh_part[1024];
+  const int nb = S.sort_nb;
+  const int per = (nb + blockDim.x - 1) / blockDim.x;
+  const int b0 = threadIdx.x * per;
+  int local = 0;
+  for (int k = 0; k < per; ++k)
+    if (b0 + k < nb)
+      local += S.sort_total[b0 + k];
+  sh_part[threadIdx.x] = local;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the per-thread partial sums
+  for (int o = 1; o < static_cast<int>(blockDim.x); o <<= 1) {
+    int v = threadIdx.x >= o ? sh_part[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh_part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sh_part[threadIdx.x] - local;
+  for (int k = 0; k < per; ++k) {
+    if (b0 + k < nb) {
+      const int c = S.sort_total[b0 + k];
+      S.sort_total[b0 + k] = run;
+      run += c;
+    }
+  }
+}
+
+// Each workgroup places its chunk: ranks within a bin come from LDS atomics
+// (the order inside a bin is free; no history depends on it).
+__global__ void __launch_bounds__(BLOCK) k_sort_scatter(DevParticles S)
+{
+  __shared__ int sh_off[SORT_MAX_BINS];
+  const int n = S.qcount[QC_XS];
+  const int nw = sort_workgroups(n, gridDim.x);
+  if (static_cast<int>(blockIdx.x) >= nw)
+    return;
+  const int nb = S.sort_nb;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x)
+    sh_off[k] = S.sort_total[k] + S.sort_count[k * SORT_MAX_BLOCKS + blockIdx.x];
+  __syncthreads();
+  const int chunk = (n + nw - 1) / nw;
+  const int lo = blockIdx.x * chunk;
+  const int hi = min(n, lo + chunk);
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const int pos = atomicAdd(&sh_off[S.q_key[i]], 1);
+    S.q_sorted[pos] = S.q_xs[i];
   }
 }
 
 //==============================================================================
 // calculate_xs event (src/particle.cpp:160-230, src/material.cpp:829-901)
+//
+// LPP lanes cooperate on one particle: lane `sub` of the group looks up
+// nuclides sub, sub + LPP, ... of the material, writes N_j * sigma_j to LDS,
+// and the group's first lane forms the macroscopic sums in material order
+// exactly as Material::calculate_neutron_xs does (product, then add).
 //==============================================================================
 
-__global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S)
+template<int NL, int LPP>
+__global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, const int* q)
+{
+  extern __shared__ double sh_xs[];
+  const int n = S.qcount[QC_XS];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.qcount[QC_ADV] = n;
+    S.qcount[QC_SURF] = 0;
+    S.qcount[QC_COLL] = 0;
+  }
+  constexpr int PPB = BLOCK / LPP; // particles per block pass
+  const int slot = threadIdx.x / LPP;
+  const int sub = threadIdx.x % LPP;
+  const int lane = lane_id();
+  const int leader = (lane / LPP) * LPP;
+  const int max_nn = P.max_mat_nuc;
+  double* prod = sh_xs + static_cast<int64_t>(slot) * max_nn * 2;
+  int* igs = reinterpret_cast<int*>(sh_xs + static_cast<int64_t>(PPB) * max_nn * 2) + slot * max_nn;
+  const bool learned = S.use_learned != 0;
+  const bool trace = S.trace_hash != 0;
+  unsigned long long n_lookups = 0, n_events = 0, n_probes = 0;
+
+  for (int base = blockIdx.x * PPB; base < n; base += gridDim.x * PPB) {
+    const int i = base + slot;
+    int p = 0;
+    int material = MATERIAL_VOID;
+    int need = 0;
+    double E = 0.0;
+    if (i < n && sub == 0) {
+      p = q[i];
+      n_events++;
+      const PHot* h = S.hot + p;
+      material = h->material;
+      // every particle reaching this event has a cell: source particles are
+      // located in k_init and crossings relocate in k_surface
+      if (h->wgt != 0.0) {
+        if (material != MATERIAL_VOID) {
+          need = material != h->material_last ? 1 : 0;
+          E = h->E;
+        } else {
+          S.hot[p].xs_t = 0.0;
+          S.hot[p].xs_a = 0.0;
+        }
+      }
+    }
+    if (LPP > 1) {
+      p = __shfl(p, leader);
+      material = __shfl(material, leader);
+      need = __shfl(need, leader);
+      E = __shfl(E, leader);
+    }
+    int nn = 0;
+    if (need) {
+      const double lx = detm::log(E / P.energy_min);
+      int i_log = static_cast<int>(lx / P.log_spacing);
+      i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+      const int j0 = P.mat_off[material];
+      nn = P.mat_off[material + 1] - j0;
+      double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+      int probes = 0;
+      for (int jj = sub; jj < nn; jj += LPP) {
+        const int j = j0 + jj;
+        Micro m = nuclide_xs(P, P.mat_nuc[j], E, i_log, lx, learned, probes);
+        micro[jj] = m.total;
+        const double dens = P.mat_dens[j] * 1.0;
+        prod[2 * jj] = dens * m.total;
+        prod[2 * jj + 1] = dens * m.absorption;
+        igs[jj] = m.i_grid;
+        n_lookups++;
+      }
+      n_probes += probes;
+    }
+    __syncthreads();
+    if (need && sub == 0) {
+      double xs_t = 0.0;
+      double xs_a = 0.0;
+      for (int jj = 0; jj < nn; ++jj) {
+        xs_t += prod[2 * jj];
+        xs_a += prod[2 * jj + 1];
+      }
+      S.hot[p].xs_t = xs_t;
+      S.hot[p].xs_a = xs_a;
+      if (trace) {
+        uint64_t h = S.hash[p];
+        hmix(h, 0x11);
+        hmix(h, static_cast<uint64_t>(material));
+        for (int jj = 0; jj < nn; ++jj)
+          hmix(h, static_cast<uint64_t>(static_cast<uint32_t>(igs[jj])));
+        S.hash[p] = h;
+      }
+    }
+    __syncthreads();
+  }
+  wave_count(&S.counters[CNT_LOOKUPS], n_lookups);
+  wave_count(&S.counters[CNT_XS_EVENTS], n_events);
+  wave_count(&S.counters[CNT_SEARCH_PROBES], n_probes);
+}
+
+// One lane per particle: the lane looks up the material's nuclides G at a
+// time (their searches interleaved) and forms the sums in material order as
+// it goes, so no LDS is needed and occupancy is set by registers alone.
+template<int NL, int G>
+__global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S, const int* q)
 {
   const int n = S.qcount[QC_XS];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -268,77 +576,64 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S)
     S.qcount[QC_SURF] = 0;
     S.qcount[QC_COLL] = 0;
   }
-  const int nl = P.n_levels;
-  const int64_t cap = S.cap;
-  unsigned long long n_lookups = 0, n_events = 0, n_lost = 0, n_probes = 0;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const int i = base + threadIdx.x;
-    if (i < n) {
-      const int p = S.q_xs[i];
-      n_events++;
-      int nc = S.n_coord[p];
-      int material = S.material[p];
-      int material_last = S.material_last[p];
-      bool alive = true;
-      if (S.cell[(nc - 1) * cap + p] == C_NONE) {
-        Geo g;
-        load_geo(S, nl, p, g);
-        bool found = exhaustive_find_cell(P, g);
-        if (!found || g.lost) {
-          S.wgt[p] = 0.0;
-          S.lost[p] = 1;
-          n_lost++;
-          alive = false;
-        }
-        store_geo(S, nl, p, g);
-        material = g.material;
-        material_last = g.material_last;
-      }
-      if (alive) {
-        if (material != MATERIAL_VOID) {
-          if (material != material_last) {
-            const double E = S.E[p];
-            const double lx = detm::log(E / P.energy_min);
-            int i_log = static_cast<int>(lx / P.log_spacing);
-            i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
-            const int j0 = P.mat_off[material];
-            const int j1 = P.mat_off[material + 1];
-            double xs_t = 0.0;
-            double xs_a = 0.0;
-            uint64_t h = S.hash[p];
-            const bool trace = S.trace_hash != 0;
-            if (trace) {
-              hmix(h, 0x11);
-              hmix(h, static_cast<uint64_t>(material));
-            }
-            int probes = 0;
-            const bool learned = S.use_learned != 0;
-            for (int j = j0; j < j1; ++j) {
-              Micro m = nuclide_xs(P, P.mat_nuc[j], E, i_log, lx, learned, probes);
-              S.micro_t[static_cast<int64_t>(j - j0) * cap + p] = m.total;
-              const double dens = P.mat_dens[j] * 1.0;
-              xs_t += dens * m.total;
-              xs_a += dens * m.absorption;
-              if (trace)
-                hmix(h, static_cast<uint64_t>(static_cast<uint32_t>(m.i_grid)));
-            }
-            n_lookups += static_cast<unsigned long long>(j1 - j0);
-            n_probes += static_cast<unsigned long long>(probes);
-            S.xs_t[p] = xs_t;
-            S.xs_a[p] = xs_a;
-            if (trace)
-              S.hash[p] = h;
-          }
-        } else {
-          S.xs_t[p] = 0.0;
-          S.xs_a[p] = 0.0;
+  const bool learned = S.use_learned != 0;
+  const bool trace = S.trace_hash != 0;
+  unsigned long long n_lookups = 0, n_events = 0, n_probes = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int p = q[i];
+    n_events++;
+    PHot* h = S.hot + p;
+    const int material = h->material;
+    if (h->wgt == 0.0)
+      continue;
+    if (material == MATERIAL_VOID) {
+      h->xs_t = 0.0;
+      h->xs_a = 0.0;
+      continue;
+    }
+    if (material == h->material_last)
+      continue;
+    const double E = h->E;
+    const double lx = detm::log(E / P.energy_min);
+    int i_log = static_cast<int>(lx / P.log_spacing);
+    i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+    const int j0 = P.mat_off[material];
+    const int nn = P.mat_off[material + 1] - j0;
+    double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+    uint64_t hh = 0;
+    if (trace) {
+      hh = S.hash[p];
+      hmix(hh, 0x11);
+      hmix(hh, static_cast<uint64_t>(material));
+    }
+    double xs_t = 0.0;
+    double xs_a = 0.0;
+    int probes = 0;
+    for (int jj = 0; jj < nn; jj += G) {
+      const int kk = nn - jj < G ? nn - jj : G;
+      Micro m[G];
+      nuclide_xs_group<G>(P, P.mat_nuc + j0 + jj, kk, E, i_log, lx, learned, probes, m);
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        if (k < kk) {
+          micro[jj + k] = m[k].total;
+          const double dens = P.mat_dens[j0 + jj + k] * 1.0;
+          xs_t += dens * m[k].total;
+          xs_a += dens * m[k].absorption;
+          if (trace)
+            hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
         }
       }
     }
+    n_lookups += nn;
+    n_probes += probes;
+    h->xs_t = xs_t;
+    h->xs_a = xs_a;
+    if (trace)
+      S.hash[p] = hh;
   }
   wave_count(&S.counters[CNT_LOOKUPS], n_lookups);
   wave_count(&S.counters[CNT_XS_EVENTS], n_events);
-  wave_count(&S.counters[CNT_LOST], n_lost);
   wave_count(&S.counters[CNT_SEARCH_PROBES], n_probes);
 }
 
@@ -346,27 +641,28 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S)
 // advance event (src/particle.cpp:232-286, src/event.cpp:107-128)
 //==============================================================================
 
-__global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S)
+template<int NL>
+__global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S, const int* q)
 {
   const int n = S.qcount[QC_ADV];
   if (blockIdx.x == 0 && threadIdx.x == 0)
     S.qcount[QC_XS] = 0;
-  const int nl = P.n_levels;
   unsigned long long n_events = 0, n_segs = 0, n_lost = 0;
   for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
     bool to_surf = false, to_coll = false;
     int p = 0;
     if (i < n) {
-      p = S.q_xs[i];
-      double wgt = S.wgt[p];
+      p = q[i];
+      const PHot h = S.hot[p];
+      double wgt = h.wgt;
       if (wgt != 0.0) {
         n_events++;
-        Geo g;
-        load_geo(S, nl, p, g);
-        const double E = S.E[p];
-        const double xs_t = S.xs_t[p];
-        uint64_t seed = S.seed[p];
+        Geo<NL> g;
+        load_geo<NL>(P, S, p, h, g);
+        const double E = h.E;
+        const double xs_t = h.xs_t;
+        uint64_t seed = h.seed;
         Boundary b = distance_to_boundary(P, g);
         double d_coll;
         if (xs_t == 0.0)
@@ -379,13 +675,27 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S)
           distance = d_coll;
         if (INFTY < distance)
           distance = INFTY;
-        V3 r0 = g.r[0];
-        for (int j = 0; j < g.n_coord; ++j)
-          g.r[j] = v_add(g.r[j], v_scale(g.u[j], distance));
+        const V3 r0 = g.r(0);
+        // move_distance (src/particle_data.cpp:86-91): r += distance * u on every level
+        const double mx = g.ux * distance, my = g.uy * distance, mz = g.uz * distance;
+        const V3 r1 = {g.rx[0] + mx, g.ry[0] + my, g.rz[0] + mz};
+        PHot* hp = S.hot + p;
+        hp->rx = r1.x;
+        hp->ry = r1.y;
+        hp->rz = r1.z;
+#pragma unroll
+        for (int l = 1; l < NL; ++l) {
+          if (l < g.n_coord) {
+            PLevel* L = level_rec<NL>(S, p, l);
+            L->rx = g.rx[l] + mx;
+            L->ry = g.ry[l] + my;
+            L->rz = g.rz[l] + mz;
+          }
+        }
         double dt = distance / speed;
-        S.time[p] = S.time[p] + dt;
+        hp->time = h.time + dt;
         if (P.tally.enabled)
-          n_segs += score_tracklength(P.tally, r0, g.r[0], g.u[0], E, wgt, distance);
+          n_segs += score_tracklength(P.tally, r0, r1, g.u(), E, wgt, distance);
         if (distance == INFTY)
           wgt = 0.0;
         if (g.lost) {
@@ -393,24 +703,17 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S)
           S.lost[p] = 1;
           n_lost++;
         }
-        const int64_t cap = S.cap;
-        for (int j = 0; j < g.n_coord; ++j) {
-          int64_t k = j * cap + p;
-          S.rx[k] = g.r[j].x;
-          S.ry[k] = g.r[j].y;
-          S.rz[k] = g.r[j].z;
-        }
-        S.seed[p] = seed;
-        S.b_surface[p] = b.surface;
-        S.b_info[p] = b.coord_level | ((b.trans[0] + 1) << 8) | ((b.trans[1] + 1) << 10) |
-                      ((b.trans[2] + 1) << 12);
-        S.wgt[p] = wgt;
+        hp->seed = seed;
+        hp->b_surface = b.surface;
+        hp->b_info = b.coord_level | ((b.trans[0] + 1) << 8) | ((b.trans[1] + 1) << 10) |
+                     ((b.trans[2] + 1) << 12);
+        hp->wgt = wgt;
         if (S.trace_hash) {
-          uint64_t h = S.hash[p];
-          hmix(h, 0x22);
-          hmix(h, dbits(distance));
-          hmix(h, d_coll > b.distance ? 2 : 1);
-          S.hash[p] = h;
+          uint64_t hh = S.hash[p];
+          hmix(hh, 0x22);
+          hmix(hh, dbits(distance));
+          hmix(hh, d_coll > b.distance ? 2 : 1);
+          S.hash[p] = hh;
         }
         if (wgt != 0.0) {
           to_surf = d_coll > b.distance;
@@ -430,36 +733,43 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S)
 // surface-crossing event (src/particle.cpp:288-329, 549-710)
 //==============================================================================
 
+template<int NL>
 __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_SURF];
-  const int nl = P.n_levels;
   unsigned long long n_events = 0, n_lost = 0;
   double leak = 0.0;
   for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
     bool push = false;
+    int key = 0;
     int p = 0;
     if (i < n) {
       p = S.q_surf[i];
       n_events++;
-      Geo g;
-      load_geo(S, nl, p, g);
-      double wgt = S.wgt[p];
+      const PHot h = S.hot[p];
+      Geo<NL> g;
+      load_geo<NL>(P, S, p, h, g);
+      double wgt = h.wgt;
       const int cell_last0 = g.cell[0];
-      const int info = S.b_info[p];
+      const int info = h.b_info;
       int trans[3] = {((info >> 8) & 3) - 1, ((info >> 10) & 3) - 1, ((info >> 12) & 3) - 1};
-      g.surface = S.b_surface[p];
+      g.surface = h.b_surface;
       g.n_coord = info & 0xff;
-      uint64_t h = 0;
+      if (g.n_coord < 1 || g.n_coord > NL) {
+        g.n_coord = 1;
+        g.lost = 1;
+      }
+      uint64_t hh = 0;
       const bool trace = S.trace_hash != 0;
       if (trace) {
-        h = S.hash[p];
-        hmix(h, 0x33);
-        hmix(h, static_cast<uint64_t>(static_cast<int64_t>(g.surface)));
-        hmix(h, static_cast<uint64_t>((trans[0] + 1) | ((trans[1] + 1) << 2) | ((trans[2] + 1) << 4)));
+        hh = S.hash[p];
+        hmix(hh, 0x33);
+        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(g.surface)));
+        hmix(hh, static_cast<uint64_t>((trans[0] + 1) | ((trans[1] + 1) << 2) | ((trans[2] + 1) << 4)));
       }
-      if (trans[0] != 0 || trans[1] != 0 || trans[2] != 0) {
+      if (g.lost) {
+      } else if (trans[0] != 0 || trans[1] != 0 || trans[2] != 0) {
         cross_lattice(P, g, trans);
       } else if (g.surface == SURFACE_NONE || abs(g.surface) > P.n_surfaces) {
         g.lost = 1;
@@ -472,12 +782,12 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
         } else if (surf.bc == BC_REFLECT) {
           // ReflectiveBC::handle_particle + cross_reflective_bc (src/boundary_condition.cpp:37-46,
           // src/particle.cpp:657-710)
-          V3 u = reflect(g.u[0], surf_normal(surf, g.r[0]));
+          V3 u = reflect(g.u(), surf_normal(surf, g.r(0)));
           u = v_div(u, v_norm(u));
           if (g.n_coord != 1) {
             g.lost = 1;
           } else {
-            g.u[0] = u;
+            g.set_u(u);
             g.cell[0] = cell_last0;
             g.surface = -g.surface;
             g.n_coord = 1;
@@ -491,7 +801,7 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
             if (!exhaustive_find_cell(P, g)) {
               g.surface = SURFACE_NONE;
               g.n_coord = 1;
-              g.r[0] = v_add(g.r[0], v_scale(g.u[0], TINY_BIT));
+              g.set_r(0, v_add(g.r(0), v_scale(g.u(), TINY_BIT)));
               if (!exhaustive_find_cell(P, g))
                 g.lost = 1;
             }
@@ -504,24 +814,24 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
         n_lost++;
       }
       if (trace) {
-        hmix(h, dbits(wgt));
-        hmix(h, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? g.cell[g.n_coord - 1] : -1)));
-        S.hash[p] = h;
+        hmix(hh, dbits(wgt));
+        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? lowest_cell(g) : -1)));
+        S.hash[p] = hh;
       }
       // event_revive_from_secondary (src/particle.cpp:417-473)
-      int n_event = S.n_event[p] + 1;
+      int n_event = h.n_event + 1;
       if (n_event == P.max_events)
         wgt = 0.0;
-      S.n_event[p] = n_event;
-      store_geo(S, nl, p, g);
-      S.wgt[p] = wgt;
+      store_geo<NL>(S, p, g);
+      S.hot[p].n_event = n_event;
+      S.hot[p].wgt = wgt;
       push = wgt != 0.0;
+      key = sort_key(S, g.material, h.E);
     }
-    wave_push(S.q_xs, &S.qcount[QC_XS], push, p);
+    wave_push_xs(S, push, p, key);
   }
   wave_count(&S.counters[CNT_SURF_EVENTS], n_events);
   wave_count(&S.counters[CNT_LOST], n_lost);
-  // leakage: wavefront sum, one atomic per wavefront
   for (int o = 32; o > 0; o >>= 1)
     leak += __shfl_xor(leak, o);
   if (lane_id() == 0 && leak != 0.0)
@@ -533,7 +843,7 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
 //==============================================================================
 
 // sample_cxs_target_velocity (src/physics.cpp:990-1041)
-__device__ inline V3 sample_cxs_target_velocity(double awr, double E, V3 u, double kT, uint64_t* seed)
+__device__ __forceinline__ V3 sample_cxs_target_velocity(double awr, double E, V3 u, double kT, uint64_t* seed)
 {
   double beta_vn = sqrt(awr * E / kT);
   double alpha = 1.0 / (1.0 + sqrt(PI) * beta_vn / 2.0);
@@ -561,33 +871,35 @@ __device__ inline V3 sample_cxs_target_velocity(double awr, double E, V3 u, doub
   return v_scale(rotate_angle(u, mu, seed), vt);
 }
 
+template<int NL>
 __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_COLL];
-  const int64_t cap = S.cap;
   unsigned long long n_events = 0, n_lost = 0, n_lookups = 0, n_probes = 0;
   for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
     bool push = false;
+    int key = 0;
     int p = 0;
     if (i < n) {
       p = S.q_coll[i];
       n_events++;
-      double E = S.E[p];
-      double wgt = S.wgt[p];
-      uint64_t seed = S.seed[p];
-      const int material = S.material[p];
-      const int nc = S.n_coord[p];
-      V3 u = {S.ux[p], S.uy[p], S.uz[p]};
+      const PHot h = S.hot[p];
+      double E = h.E;
+      double wgt = h.wgt;
+      uint64_t seed = h.seed;
+      const int material = h.material;
+      V3 u = {h.ux, h.uy, h.uz};
       // sample_nuclide (src/physics.cpp:494-518)
-      double cutoff = prn(&seed) * S.xs_t[p];
+      double cutoff = prn(&seed) * h.xs_t;
       const int j0 = P.mat_off[material];
       const int j1 = P.mat_off[material + 1];
+      const double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
       double prob = 0.0;
       int i_nuclide = -1;
       for (int j = j0; j < j1; ++j) {
         const double dens = P.mat_dens[j] * 1.0;
-        prob += dens * S.micro_t[static_cast<int64_t>(j - j0) * cap + p];
+        prob += dens * micro[j - j0];
         if (prob >= cutoff) {
           i_nuclide = P.mat_nuc[j];
           break;
@@ -641,37 +953,37 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
         }
       }
       if (S.trace_hash) {
-        uint64_t h = S.hash[p];
-        hmix(h, 0x44);
-        hmix(h, static_cast<uint64_t>(static_cast<int64_t>(i_nuclide)));
-        hmix(h, absorbed ? 101 : 2);
-        hmix(h, dbits(E));
-        hmix(h, dbits(u.x));
-        hmix(h, dbits(u.y));
-        hmix(h, dbits(u.z));
-        S.hash[p] = h;
+        uint64_t hh = S.hash[p];
+        hmix(hh, 0x44);
+        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(i_nuclide)));
+        hmix(hh, absorbed ? 101 : 2);
+        hmix(hh, dbits(E));
+        hmix(hh, dbits(u.x));
+        hmix(hh, dbits(u.y));
+        hmix(hh, dbits(u.z));
+        S.hash[p] = hh;
       }
       if (E < P.energy_cutoff)
         wgt = 0.0;
-      // Particle::event_collide epilogue (src/particle.cpp:388-406)
-      S.surface[p] = SURFACE_NONE;
-      S.material_last[p] = C_NONE;
-      for (int j = 0; j < nc; ++j) {
-        int64_t k = j * cap + p;
-        S.ux[k] = u.x;
-        S.uy[k] = u.y;
-        S.uz[k] = u.z;
-      }
-      int n_event = S.n_event[p] + 1;
+      // Particle::event_collide epilogue (src/particle.cpp:388-406); the
+      // direction of the lower levels is the shared one
+      int n_event = h.n_event + 1;
       if (n_event == P.max_events)
         wgt = 0.0;
-      S.n_event[p] = n_event;
-      S.E[p] = E;
-      S.wgt[p] = wgt;
-      S.seed[p] = seed;
+      PHot* hp = S.hot + p;
+      hp->surface = SURFACE_NONE;
+      hp->material_last = C_NONE;
+      hp->ux = u.x;
+      hp->uy = u.y;
+      hp->uz = u.z;
+      hp->n_event = n_event;
+      hp->E = E;
+      hp->wgt = wgt;
+      hp->seed = seed;
       push = wgt != 0.0;
+      key = sort_key(S, material, E);
     }
-    wave_push(S.q_xs, &S.qcount[QC_XS], push, p);
+    wave_push_xs(S, push, p, key);
   }
   wave_count(&S.counters[CNT_COLL_EVENTS], n_events);
   wave_count(&S.counters[CNT_COLLISIONS], n_events);
@@ -699,30 +1011,99 @@ __global__ void __launch_bounds__(BLOCK) k_accumulate(double* value, double* res
 // Host launch wrappers
 //==============================================================================
 
-void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st)
+namespace {
+
+template<int NL>
+void init_nl(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st)
 {
   int blocks = static_cast<int>((n + BLOCK - 1) / BLOCK);
-  if (blocks < 1)
-    blocks = 1;
-  hipLaunchKernelGGL(k_init, dim3(blocks), dim3(BLOCK), 0, st, P, S, n, id_base);
+  hipLaunchKernelGGL(k_init<NL>, dim3(blocks < 1 ? 1 : blocks), dim3(BLOCK), 0, st, P, S, n, id_base);
 }
 
-void launch_cycle(const DevProblem& P, const DevParticles& S, int blocks, hipStream_t st)
+template<int NL, int LPP>
+void xs_nl(const DevProblem& P, const DevParticles& S, const int* q, int blocks, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_xs, dim3(blocks), dim3(BLOCK), 0, st, P, S);
-  hipLaunchKernelGGL(k_advance, dim3(blocks), dim3(BLOCK), 0, st, P, S);
-  hipLaunchKernelGGL(k_surface, dim3(blocks), dim3(BLOCK), 0, st, P, S);
-  hipLaunchKernelGGL(k_collision, dim3(blocks), dim3(BLOCK), 0, st, P, S);
+  size_t lds = static_cast<size_t>(BLOCK / LPP) * (P.max_mat_nuc > 0 ? P.max_mat_nuc : 1) *
+               (2 * sizeof(double) + sizeof(int));
+  // the work per pass is BLOCK / LPP particles; scale the grid to keep the
+  // same particle coverage per launch
+  int b = blocks * (LPP < 8 ? LPP : 8);
+  hipLaunchKernelGGL((k_xs<NL, LPP>), dim3(b), dim3(BLOCK), lds, st, P, S, q);
 }
 
-void launch_kernel(int which, const DevProblem& P, const DevParticles& S, int blocks, hipStream_t st)
+template<int NL>
+void xs_lane(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int group, hipStream_t st)
 {
-  switch (which) {
-  case 0: hipLaunchKernelGGL(k_xs, dim3(blocks), dim3(BLOCK), 0, st, P, S); break;
-  case 1: hipLaunchKernelGGL(k_advance, dim3(blocks), dim3(BLOCK), 0, st, P, S); break;
-  case 2: hipLaunchKernelGGL(k_surface, dim3(blocks), dim3(BLOCK), 0, st, P, S); break;
-  default: hipLaunchKernelGGL(k_collision, dim3(blocks), dim3(BLOCK), 0, st, P, S); break;
+  switch (group) {
+  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  default: hipLaunchKernelGGL((k_xs_lane<NL, 4>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
   }
+}
+
+template<int NL>
+void xs_lpp(const DevProblem& P, const DevParticles& S, const int* q, const LaunchConfig& lc, hipStream_t st)
+{
+  const int blocks = lc.blocks;
+  switch (lc.xs_lanes) {
+  case 1: xs_lane<NL>(P, S, q, blocks, lc.xs_group, st); break;
+  case 2: xs_nl<NL, 2>(P, S, q, blocks, st); break;
+  case 4: xs_nl<NL, 4>(P, S, q, blocks, st); break;
+  case 8: xs_nl<NL, 8>(P, S, q, blocks, st); break;
+  case 16: xs_nl<NL, 16>(P, S, q, blocks, st); break;
+  case 32: xs_nl<NL, 32>(P, S, q, blocks, st); break;
+  default: xs_nl<NL, 64>(P, S, q, blocks, st); break;
+  }
+}
+
+template<int NL>
+void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
+{
+  const int* q = lc.sort ? S.q_sorted : S.q_xs;
+  switch (which) {
+  case KERNEL_SORT:
+    if (lc.sort) {
+      const int sb = lc.sort_blocks < 1 ? 1 : (lc.sort_blocks > SORT_MAX_BLOCKS ? SORT_MAX_BLOCKS : lc.sort_blocks);
+      hipLaunchKernelGGL(k_sort_count, dim3(sb), dim3(BLOCK), 0, st, S);
+      hipLaunchKernelGGL(k_sort_rows, dim3((S.sort_nb + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, sb);
+      hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, S);
+      hipLaunchKernelGGL(k_sort_scatter, dim3(sb), dim3(BLOCK), 0, st, S);
+    }
+    break;
+  case KERNEL_XS: xs_lpp<NL>(P, S, q, lc, st); break;
+  case KERNEL_ADVANCE: hipLaunchKernelGGL(k_advance<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  case KERNEL_SURFACE: hipLaunchKernelGGL(k_surface<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S); break;
+  default: hipLaunchKernelGGL(k_collision<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S); break;
+  }
+}
+
+} // namespace
+
+void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st)
+{
+  switch (P.n_levels) {
+  case 1: init_nl<1>(P, S, n, id_base, st); break;
+  case 2: init_nl<2>(P, S, n, id_base, st); break;
+  case 3: init_nl<3>(P, S, n, id_base, st); break;
+  default: init_nl<4>(P, S, n, id_base, st); break;
+  }
+}
+
+void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
+{
+  switch (P.n_levels) {
+  case 1: kernel_nl<1>(which, P, S, lc, st); break;
+  case 2: kernel_nl<2>(which, P, S, lc, st); break;
+  case 3: kernel_nl<3>(which, P, S, lc, st); break;
+  default: kernel_nl<4>(which, P, S, lc, st); break;
+  }
+}
+
+void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
+{
+  for (int k = KERNEL_SORT; k <= KERNEL_COLLISION; ++k)
+    launch_kernel(k, P, S, lc, st);
 }
 
 void launch_accumulate(double* value, double* results, int64_t n_bins, double norm, hipStream_t st)

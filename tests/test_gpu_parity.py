@@ -131,10 +131,28 @@ def test_transport_graph_and_direct_launch_identical():
     assert_tallies_close(a["results"], c["results"])
 
 
-def test_transport_learned_index_same_histories():
+@pytest.mark.parametrize("opts", [{"sort": 0, "xs_lanes": 1}, {"sort": 1, "xs_lanes": 2},
+                                  {"sort": 1, "xs_lanes": 64}, {"sort": 0, "xs_lanes": 8},
+                                  {"sort": 1, "xs_lanes": 1, "sort_blocks": 3, "blocks": 5},
+                                  {"sort": 1, "xs_lanes": 1, "sort_blocks": 256},
+                                  {"sort": 1, "xs_lanes": 1, "xs_group": 1},
+                                  {"sort": 0, "xs_lanes": 1, "xs_group": 2},
+                                  {"sort": 1, "xs_lanes": 1, "xs_group": 8}])
+def test_transport_launch_shapes_identical(opts):
+    """Queue sorting and lanes-per-particle change the order of work, never a history."""
+    d, model, nucs = prepare(1, n_particles=1500, n_batches=1, grid_lo=2000, grid_hi=6000, n_nuclides=12,
+                             lattice_n=5, mesh=(10, 10, 20), half_height=20.0)
+    o = oracle_run(d)
+    g = gpu_run(d, options=dict(trace_hash=1, **opts))
+    assert np.array_equal(g["hashes"], o.hashes(1500))
+    assert_tallies_close(g["results"], o.results())
+
+
+@pytest.mark.parametrize("group", [1, 4, 8])
+def test_transport_learned_index_same_histories(group):
     d, model, nucs = prepare(0, n_particles=2000, n_batches=1)
     a = gpu_run(d, options={"trace_hash": 1})
-    b = gpu_run(d, options={"trace_hash": 1}, learned=(256, 200))
+    b = gpu_run(d, options={"trace_hash": 1, "xs_group": group}, learned=(256, 200))
     assert np.array_equal(a["hashes"], b["hashes"])
     assert_tallies_close(a["results"], b["results"])
 
