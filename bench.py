@@ -91,11 +91,15 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
 def roofline(info):
     st = info["stats"]
     times = {"k_xs": st["t_xs_ms"], "k_advance": st["t_adv_ms"], "k_surface": st["t_surf_ms"],
-             "k_collision": st["t_coll_ms"], "k_sort": st["t_sort_ms"], "k_init": st["t_init_ms"]}
+             "k_collision": st["t_coll_ms"], "partition_xs": st["t_sort_ms"],
+             "partition_split": st["t_split_ms"], "k_init": st["t_init_ms"]}
     launches = {"k_xs": st["n_xs"], "k_advance": st["n_adv"], "k_surface": st["n_surf"],
-                "k_collision": st["n_coll"], "k_sort": st["n_sort"], "k_init": st["n_init"]}
+                "k_collision": st["n_coll"], "partition_xs": st["n_sort"], "partition_split": st["n_split"],
+                "k_init": st["n_init"]}
     by = kernel_bytes(st)
-    by["k_sort"] = 0
+    # queue bookkeeping and source sampling carry no algorithmic bytes
+    by["partition_xs"] = 0
+    by["partition_split"] = 0
     by["k_init"] = 0
     dom = max(times, key=times.get)
     achieved = by[dom] / (times[dom] * 1e-3) / 1e9

@@ -101,7 +101,8 @@ struct Sim {
   std::vector<double> gt = std::vector<double>(3 * N_GLOBAL, 0.0);
   double total_weight = 0;
   int* h_qcount = nullptr; // pinned
-  double* h_leak = nullptr; // pinned
+  std::vector<unsigned long long> h_counter_slots;
+  std::vector<double> h_leak_slots;
   std::vector<uint64_t> h_hashes;
   // options
   bool trace_hash = false;
@@ -298,18 +299,20 @@ void alloc_particles(Sim& s, int64_t cap)
   S.micro_t = dev_alloc<double>(L, static_cast<size_t>(S.max_nn) * cap);
   S.lost = dev_alloc<int>(L, cap);
   S.hash = dev_alloc<uint64_t>(L, cap);
-  S.leak = dev_alloc<double>(L, 1);
+  S.leak = dev_alloc<double>(L, CNT_SLOTS);
   S.q_xs = dev_alloc<int>(L, cap);
-  S.q_surf = dev_alloc<int>(L, cap);
-  S.q_coll = dev_alloc<int>(L, cap);
+  S.adv_key = dev_alloc<int>(L, cap);
+  S.q_sc = dev_alloc<int>(L, cap);
+  S.stage_p = dev_alloc<int>(L, cap);
+  S.stage_key = dev_alloc<int>(L, cap);
   S.qcount = dev_alloc<int>(L, QC_N);
-  S.counters = dev_alloc<unsigned long long>(L, CNT_N);
-  // counting sort by (material, energy bucket): the bucket is the sign,
-  // exponent and top mantissa bits of E, offset by those of energy_min
-  S.q_sorted = dev_alloc<int>(L, cap);
-  S.q_key = dev_alloc<int>(L, cap);
-  // four buckets per binade (sign, exponent and two mantissa bits), fewer
-  // when the materials times buckets exceed the LDS histogram
+  S.counters = dev_alloc<unsigned long long>(L, static_cast<size_t>(CNT_N) * CNT_SLOTS);
+  s.h_counter_slots.assign(static_cast<size_t>(CNT_N) * CNT_SLOTS, 0);
+  s.h_leak_slots.assign(CNT_SLOTS, 0.0);
+  // calculate-xs keys (material, energy bucket): the bucket is the sign,
+  // exponent and top mantissa bits of E, offset by those of energy_min;
+  // four buckets per binade (two mantissa bits), fewer when the materials
+  // times buckets exceed the LDS histogram
   const int n_mat = s.P.n_materials + 1;
   int shift = 50, eb = 256;
   while (n_mat * eb > SORT_MAX_BINS && eb > 1) {
@@ -322,8 +325,11 @@ void alloc_particles(Sim& s, int64_t cap)
   S.sort_eb = eb;
   S.sort_base = static_cast<int>(emin_bits >> shift) - 1;
   S.sort_nb = std::min(n_mat * eb, SORT_MAX_BINS);
-  S.sort_count = dev_alloc<int>(L, static_cast<size_t>(S.sort_nb) * SORT_MAX_BLOCKS);
-  S.sort_total = dev_alloc<int>(L, S.sort_nb);
+  // partition scratch, shared by the calculate-xs partition (sort_nb keys)
+  // and the advance split (two keys)
+  const int rows = std::max(S.sort_nb, 2);
+  S.sort_count = dev_alloc<int>(L, static_cast<size_t>(rows) * SORT_MAX_BLOCKS);
+  S.sort_total = dev_alloc<int>(L, rows);
   s.cap = cap;
   // launch shape: enough workgroups to fill 256 CUs several times over
   int64_t b = (cap + 255) / 256;
@@ -392,8 +398,6 @@ void run_cycles(Sim& s, int n_cycles)
   if (s.event_timing) {
     for (int c = 0; c < n_cycles; ++c) {
       for (int k = KERNEL_SORT; k <= KERNEL_COLLISION; ++k) {
-        if (k == KERNEL_SORT && !s.lc.sort)
-          continue;
         record_kernel(s, k, true);
         launch_kernel(k, s.P, s.S, s.lc, s.stream);
         record_kernel(s, k, false);
@@ -430,7 +434,7 @@ void transport_batch(Sim& s)
   s.S.use_learned = (s.use_learned && s.P.learned.enabled) ? 1 : 0;
   if (s.trace_hash)
     s.h_hashes.assign(m.n_particles, 0);
-  hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double), s.stream), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double) * CNT_SLOTS, s.stream), "hipMemsetAsync");
 
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
@@ -448,15 +452,17 @@ void transport_batch(Sim& s)
     launch_init(s.P, s.S, n, id_base, s.stream);
     if (s.event_timing)
       record_kernel(s, KERNEL_INIT, false);
-    s.h_qcount[QC_XS] = static_cast<int>(n);
-    s.h_qcount[QC_ADV] = 0;
+    // k_init staged the n source particles (key -1 for any it lost)
+    s.h_qcount[QC_XS] = 0;
+    s.h_qcount[QC_STAGE] = static_cast<int>(n);
     s.h_qcount[QC_SURF] = 0;
     s.h_qcount[QC_COLL] = 0;
     hip_check(hipMemcpyAsync(s.S.qcount, s.h_qcount, sizeof(int) * QC_N, hipMemcpyHostToDevice, s.stream),
       "hipMemcpyAsync");
-    // A batch ends when the calculate-xs queue stays empty after a cycle.
-    // Every history ends within max_particle_events events (one per cycle),
-    // so more cycles than that means a kernel failed to retire particles.
+    // A batch ends when no particle is left after the advance of a cycle
+    // (nothing to stage for the next one).  Every history ends within
+    // max_particle_events events (one per cycle), so more cycles than that
+    // means a kernel failed to retire particles.
     long long cycles = 0;
     while (true) {
       if (cycles > m.max_events + 16)
@@ -469,7 +475,7 @@ void transport_batch(Sim& s)
       s.n_checks++;
       if (s.event_timing)
         flush_kernel_timers(s);
-      if (s.h_qcount[QC_XS] == 0)
+      if (s.h_qcount[QC_STAGE] == 0)
         break;
     }
     if (s.trace_hash)
@@ -479,8 +485,13 @@ void transport_batch(Sim& s)
     offset += n;
   }
   hip_check(hipEventRecord(e1, s.stream), "hipEventRecord");
-  hip_check(hipMemcpyAsync(s.h_leak, s.S.leak, sizeof(double), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(s.h_leak_slots.data(), s.S.leak, sizeof(double) * CNT_SLOTS, hipMemcpyDeviceToHost,
+              s.stream),
+    "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  double leak = 0.0;
+  for (double v : s.h_leak_slots)
+    leak += v;
   auto w1 = std::chrono::steady_clock::now();
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
@@ -498,7 +509,7 @@ void transport_batch(Sim& s)
   // accumulate_tallies (src/tallies/tally.cpp:1059-1104)
   if (active) {
     n_realizations += 1;
-    double val = *s.h_leak / s.total_weight;
+    double val = leak / s.total_weight;
     s.gt[3 * GT_LEAKAGE + 1] += val;
     s.gt[3 * GT_LEAKAGE + 2] += val * val;
     if (!m.tallies.empty()) {
@@ -509,11 +520,17 @@ void transport_batch(Sim& s)
       s.results_dirty = true;
     }
   }
-  hip_check(hipMemcpyAsync(s.counters, s.S.counters, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost,
-              s.stream),
+  hip_check(hipMemcpyAsync(s.h_counter_slots.data(), s.S.counters,
+              sizeof(unsigned long long) * CNT_N * CNT_SLOTS, hipMemcpyDeviceToHost, s.stream),
     "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   hip_check(hipGetLastError(), "kernel launch");
+  for (int c = 0; c < CNT_N; ++c) {
+    unsigned long long sum = 0;
+    for (int k = 0; k < CNT_SLOTS; ++k)
+      sum += s.h_counter_slots[static_cast<size_t>(c) * CNT_SLOTS + k];
+    s.counters[c] = sum;
+  }
 }
 
 void sync_results(Sim& s)
@@ -590,8 +607,6 @@ int openmc_init(int argc, char* argv[], const void* intracomm)
     hip_check(hipSetDevice(0), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&g->h_qcount), sizeof(int) * QC_N, hipHostMallocDefault),
-      "hipHostMalloc");
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&g->h_leak), sizeof(double), hipHostMallocDefault),
       "hipHostMalloc");
     upload_problem(*g);
     if (g->m.event_based)
@@ -729,8 +744,6 @@ int openmc_finalize(void)
     free_list(g->dev_allocs);
     if (g->h_qcount)
       (void)hipHostFree(g->h_qcount);
-    if (g->h_leak)
-      (void)hipHostFree(g->h_leak);
     if (g->stream)
       (void)hipStreamDestroy(g->stream);
   }
@@ -917,8 +930,8 @@ int omc_set_option(const char* name, double value)
 
 static const char* k_stat_names =
   "transport_ms,active_batches,lookups,collisions,segments,lost,xs_events,adv_events,surf_events,"
-  "coll_events,search_probes,cycles,checks,capacity,t_sort_ms,t_xs_ms,t_adv_ms,t_surf_ms,t_coll_ms,"
-  "t_init_ms,n_sort,n_xs,n_adv,n_surf,n_coll,n_init,wall_s,blocks,xs_lanes,sort";
+  "coll_events,search_probes,cycles,checks,capacity,t_sort_ms,t_xs_ms,t_adv_ms,t_split_ms,t_surf_ms,"
+  "t_coll_ms,t_init_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,wall_s,blocks,xs_lanes,sort,xs_group";
 
 const char* omc_stat_names(void)
 {
@@ -944,14 +957,15 @@ int omc_get_stats(double* out, int n)
   v.push_back(static_cast<double>(g->n_cycles));
   v.push_back(static_cast<double>(g->n_checks));
   v.push_back(static_cast<double>(g->cap));
-  for (int k = 0; k < 6; ++k)
+  for (int k = 0; k < KERNEL_KINDS; ++k)
     v.push_back(g->kt.ms[k]);
-  for (int k = 0; k < 6; ++k)
+  for (int k = 0; k < KERNEL_KINDS; ++k)
     v.push_back(static_cast<double>(g->kt.launches[k]));
   v.push_back(g->t_batch_wall_s);
-  v.push_back(g->blocks);
+  v.push_back(g->lc.blocks);
   v.push_back(g->lc.xs_lanes);
   v.push_back(g->lc.sort);
+  v.push_back(g->lc.xs_group);
   for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
     out[k] = v[k];
   return static_cast<int>(v.size());

@@ -172,8 +172,17 @@ enum Counter : int {
   CNT_N
 };
 
-// Queue counters (device, int32)
-enum QueueCount : int { QC_XS = 0, QC_ADV, QC_SURF, QC_COLL, QC_N };
+// Queue counts (device, int32), each written by one partition pass:
+//   QC_XS    live particles in q_xs (the calculate-xs and advance queue)
+//   QC_STAGE entries staged for the next calculate-xs partition
+//   QC_SURF  surface crossings at the front of q_sc
+//   QC_COLL  collisions after them (QC_COLL == QC_SURF + 1 is relied on)
+enum QueueCount : int { QC_XS = 0, QC_STAGE, QC_SURF, QC_COLL, QC_N };
+
+// Counters and the leakage sum are spread over this many slots (a wavefront
+// adds into slot wave_id % CNT_SLOTS) so no address sees more than a few
+// atomics per launch; the host sums the slots.
+constexpr int CNT_SLOTS = 2048;
 
 // Hot state of one particle: exactly one 128-byte L2 cache line, so an event
 // kernel touching a particle in any queue order moves one line, not one line
@@ -217,22 +226,26 @@ struct DevParticles {
   int max_nn;
   int* lost;
   uint64_t* hash;
-  double* leak; // per-batch global leakage accumulator [1]
-  // queues
-  int *q_xs, *q_surf, *q_coll;
-  // counting sort of the calculate-xs queue by (material, energy bucket):
-  // producers append each particle's key beside it in q_key; sort_count is
-  // a [sort_nb][SORT_MAX_BLOCKS] matrix of per-workgroup bin counts
-  int* q_key;
-  int* q_sorted;
+  double* leak; // [CNT_SLOTS] per-batch leakage partial sums
+  // Queues.  No kernel appends to a shared counter: an event kernel writes
+  // its result for input i at position i of a staging array (the slot and a
+  // key, -1 for a particle that is done) and a partition pass compacts the
+  // staging array by key into the next queue.
+  int* q_xs;      // [cap] live particles for calculate-xs and advance, sorted by key
+  int* adv_key;   // [cap] advance outcome of q_xs[i]: 0 surface, 1 collision, -1 done
+  int* q_sc;      // [cap] surface crossings, then collisions
+  int* stage_p;   // [cap] slots after surface crossing / collision
+  int* stage_key; // [cap] their calculate-xs sort keys, -1 when done
+  // partition scratch: sort_count is a [bin][SORT_MAX_BLOCKS] matrix of
+  // per-workgroup bin counts, sort_total the bin totals then bin starts
   int* sort_count;
-  int* sort_total; // [sort_nb]: bin totals, then bin starts
-  int sort_nb;
+  int* sort_total;
+  int sort_nb;    // calculate-xs keys: (material + 1) * sort_eb + energy bucket
   int sort_shift; // energy bucket = (bits(E) >> sort_shift) - sort_base
   int sort_base;
   int sort_eb;    // energy buckets per material
   int* qcount;        // [QC_N]
-  unsigned long long* counters; // [CNT_N]
+  unsigned long long* counters; // [CNT_N][CNT_SLOTS]
   int trace_hash;
   int use_learned;
 };

@@ -7,20 +7,24 @@
 
 namespace omc {
 
+// The steps of one cycle, in launch order (KERNEL_SORT .. KERNEL_COLLISION),
+// then source sampling.  SORT and SPLIT are partition passes (four kernels).
 enum KernelKind : int {
-  KERNEL_SORT = 0,
+  KERNEL_SORT = 0,      // staged survivors -> calculate-xs queue
   KERNEL_XS = 1,
   KERNEL_ADVANCE = 2,
-  KERNEL_SURFACE = 3,
-  KERNEL_COLLISION = 4,
-  KERNEL_INIT = 5
+  KERNEL_SPLIT = 3,     // advance outcomes -> surface / collision ranges
+  KERNEL_SURFACE = 4,
+  KERNEL_COLLISION = 5,
+  KERNEL_INIT = 6,
+  KERNEL_KINDS = 7
 };
 
 struct LaunchConfig {
   int blocks;   // workgroups of the grid-stride event kernels
   int xs_lanes; // lanes per particle in the calculate-xs kernel (1..64, power of two)
-  int sort;     // counting-sort the calculate-xs queue each cycle
-  int sort_blocks; // workgroups of the histogram and scatter kernels
+  int sort;     // order the calculate-xs queue by key (else only compact it)
+  int sort_blocks; // maximum workgroups of the partition count and scatter kernels
   int xs_group;    // nuclide lookups interleaved per lane (1, 2, 4 or 8) when xs_lanes == 1
 };
 

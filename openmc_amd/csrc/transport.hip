@@ -44,25 +44,6 @@ __device__ __forceinline__ int lane_id()
   return __lane_id();
 }
 
-// Wavefront-compacted append of `val` to queue `q` for lanes with pred set.
-// Must be reached by every lane of the wavefront.
-__device__ __forceinline__ void wave_push(int* q, int* cnt, bool pred, int val)
-{
-  unsigned long long mask = __ballot(pred);
-  if (mask == 0ull)
-    return;
-  int lane = lane_id();
-  int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-  int base = 0;
-  if (lane == leader)
-    base = atomicAdd(cnt, __popcll(mask));
-  base = __shfl(base, leader);
-  if (pred) {
-    unsigned long long below = lane == 0 ? 0ull : (mask & ((1ull << lane) - 1ull));
-    q[base + __popcll(below)] = val;
-  }
-}
-
 // Key of a particle about to calculate cross sections, for the counting sort
 // of the calculate-xs queue: (material + 1, energy bucket), the bucket being
 // the sign, exponent and leading mantissa bits of E offset by those of the
@@ -78,34 +59,19 @@ __device__ __forceinline__ int sort_key(const DevParticles& S, int material, dou
   return key < 0 ? 0 : key;
 }
 
-// The same append to the calculate-xs queue, with the particle's sort key
-// stored at the same position of q_key.
-__device__ __forceinline__ void wave_push_xs(const DevParticles& S, bool pred, int val, int key)
+__device__ __forceinline__ int wave_slot()
 {
-  unsigned long long mask = __ballot(pred);
-  if (mask == 0ull)
-    return;
-  int lane = lane_id();
-  int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-  int base = 0;
-  if (lane == leader)
-    base = atomicAdd(&S.qcount[QC_XS], __popcll(mask));
-  base = __shfl(base, leader);
-  if (pred) {
-    unsigned long long below = lane == 0 ? 0ull : (mask & ((1ull << lane) - 1ull));
-    const int at = base + __popcll(below);
-    S.q_xs[at] = val;
-    S.q_key[at] = key;
-  }
+  return static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64u) & (CNT_SLOTS - 1);
 }
 
-// Wavefront sum then one atomic per wavefront.  Every lane must call it.
-__device__ __forceinline__ void wave_count(unsigned long long* ctr, unsigned long long v)
+// Wavefront sum of a counter, added into this wavefront's slot of the
+// counter's CNT_SLOTS slots.  Every lane must call it.
+__device__ __forceinline__ void wave_count(const DevParticles& S, int counter, unsigned long long v)
 {
   for (int o = 32; o > 0; o >>= 1)
     v += __shfl_xor(v, o);
   if (lane_id() == 0 && v != 0ull)
-    atomicAdd(ctr, v);
+    atomicAdd(&S.counters[counter * CNT_SLOTS + wave_slot()], v);
 }
 
 template<int NL>
@@ -342,20 +308,39 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
     S.lost[i] = lost ? 1 : 0;
     n_lost += lost ? 1 : 0;
     S.hash[i] = 0xcbf29ce484222325ULL ^ static_cast<uint64_t>(id);
-    S.q_xs[i] = static_cast<int>(i);
-    S.q_key[i] = sort_key(S, g.material, E);
+    S.stage_p[i] = static_cast<int>(i);
+    S.stage_key[i] = lost ? -1 : sort_key(S, g.material, E);
   }
-  wave_count(&S.counters[CNT_LOST], n_lost);
+  wave_count(S, CNT_LOST, n_lost);
 }
 
 //==============================================================================
-// Counting sort of the calculate-xs queue by (material, energy bucket)
+// Partition passes: counting sort of a staging array by key
 //==============================================================================
 
-// The sort of a queue of n particles uses nw(n) = clamp(ceil(n / 2048), 1,
-// max_blocks) workgroups (the launch has max_blocks; the others return at
-// once), so short queues late in a batch scan a small count matrix.
-// Workgroup b owns the contiguous chunk [b * chunk, min(n, (b + 1) * chunk)).
+// A partition reads n = *n_in (slot, key) pairs, drops those with key < 0,
+// and writes the others to `out` grouped by key in increasing key order (the
+// order inside a key is free: no history depends on processing order).  It
+// writes the number kept to *n_out and, if bin_tot is set, each key's count.
+// The calculate-xs partition keys by (material, energy bucket) -- the order
+// the comment at src/event.cpp:77-85 asks for -- so that neighbouring lanes
+// search neighbouring parts of the same energy grids (one key when sorting
+// is off: a plain compaction); the advance partition keys by outcome
+// (0 surface crossing, 1 collision).
+struct Part {
+  const int* val;
+  const int* key;
+  const int* n_in;
+  int* out;
+  int* n_out;
+  int* bin_tot;
+  int nb;
+};
+
+// A partition of n entries uses nw(n) = clamp(ceil(n / 2048), 1, max_blocks)
+// workgroups (the launch has max_blocks; the others return at once), so
+// short queues late in a batch scan a small count matrix.  Workgroup b owns
+// the contiguous chunk [b * chunk, min(n, (b + 1) * chunk)).
 constexpr int SORT_MIN_PER_BLOCK = 2048;
 
 __device__ __forceinline__ int sort_workgroups(int n, int max_blocks)
@@ -364,52 +349,81 @@ __device__ __forceinline__ int sort_workgroups(int n, int max_blocks)
   return w < 1 ? 1 : (w > max_blocks ? max_blocks : w);
 }
 
-// Per-workgroup bin counts of its chunk into sort_count[bin][b].
-__global__ void __launch_bounds__(BLOCK) k_sort_count(DevParticles S)
+__device__ __forceinline__ int part_key(const Part& A, int i)
+{
+  const int k = A.key[i];
+  return k < 0 ? -1 : (A.nb == 1 ? 0 : k);
+}
+
+// Per-workgroup key counts of its chunk into sort_count[key][b].
+__global__ void __launch_bounds__(BLOCK) k_part_count(DevParticles S, Part A)
 {
   __shared__ int sh_cnt[SORT_MAX_BINS];
-  const int n = S.qcount[QC_XS];
+  const int n = *A.n_in;
   const int nw = sort_workgroups(n, gridDim.x);
   if (static_cast<int>(blockIdx.x) >= nw)
     return;
-  const int nb = S.sort_nb;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x)
+  for (int k = threadIdx.x; k < A.nb; k += blockDim.x)
     sh_cnt[k] = 0;
   __syncthreads();
   const int chunk = (n + nw - 1) / nw;
   const int lo = blockIdx.x * chunk;
   const int hi = min(n, lo + chunk);
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    atomicAdd(&sh_cnt[S.q_key[i]], 1);
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const int k = part_key(A, i);
+    if (k >= 0)
+      atomicAdd(&sh_cnt[k], 1);
+  }
   __syncthreads();
-  for (int k = threadIdx.x; k < nb; k += blockDim.x)
+  for (int k = threadIdx.x; k < A.nb; k += blockDim.x)
     S.sort_count[k * SORT_MAX_BLOCKS + blockIdx.x] = sh_cnt[k];
 }
 
-// One thread per bin: exclusive scan of the bin's row over the workgroups
-// (sort_count[bin][b] becomes the offset of workgroup b inside the bin) and
-// the bin total.
-__global__ void __launch_bounds__(BLOCK) k_sort_rows(DevParticles S, int max_blocks)
+// One wavefront per key: exclusive scan of the key's row over the workgroups
+// (sort_count[key][b] becomes workgroup b's offset inside the key; each lane
+// holds four consecutive workgroups) and the key total into sort_total[key].
+__global__ void __launch_bounds__(BLOCK) k_part_rows(DevParticles S, Part A, int max_blocks)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S.sort_nb)
+  static_assert(SORT_MAX_BLOCKS == 4 * 64, "four workgroups per lane");
+  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  if (k >= A.nb)
     return;
-  const int nw = sort_workgroups(S.qcount[QC_XS], max_blocks);
+  const int lane = lane_id();
+  const int nw = sort_workgroups(*A.n_in, max_blocks);
   int* row = S.sort_count + static_cast<int64_t>(k) * SORT_MAX_BLOCKS;
-  int run = 0;
-  for (int b = 0; b < nw; ++b) {
-    const int v = row[b];
-    row[b] = run;
-    run += v;
+  int v[4];
+  int local = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int b = 4 * lane + j;
+    v[j] = b < nw ? row[b] : 0;
+    local += v[j];
   }
-  S.sort_total[k] = run;
+  int incl = local;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (lane >= o)
+      incl += y;
+  }
+  int run = incl - local;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int b = 4 * lane + j;
+    if (b < nw)
+      row[b] = run;
+    run += v[j];
+  }
+  const int total = __shfl(incl, 63);
+  if (lane == 0)
+    S.sort_total[k] = total;
 }
 
-// One workgroup: exclusive scan of the bin totals into bin starts.
-__global__ void __launch_bounds__(1024) k_sort_scan(DevParticles S)
+// One workgroup: exclusive scan of the key totals into key starts; writes
+// the number kept and the optional per-key counts.
+__global__ void __launch_bounds__(1024) k_part_scan(DevParticles S, Part A)
 {
   __shared__ int sh_part[1024];
-  const int nb = S.sort_nb;
+  const int nb = A.nb;
   const int per = (nb + blockDim.x - 1) / blockDim.x;
   const int b0 = threadIdx.x * per;
   int local = 0;
@@ -430,30 +444,35 @@ __global__ void __launch_bounds__(1024) k_sort_scan(DevParticles S)
     if (b0 + k < nb) {
       const int c = S.sort_total[b0 + k];
       S.sort_total[b0 + k] = run;
+      if (A.bin_tot)
+        A.bin_tot[b0 + k] = c;
       run += c;
     }
   }
+  if (threadIdx.x == blockDim.x - 1)
+    *A.n_out = sh_part[threadIdx.x];
 }
 
-// Each workgroup places its chunk: ranks within a bin come from LDS atomics
-// (the order inside a bin is free; no history depends on it).
-__global__ void __launch_bounds__(BLOCK) k_sort_scatter(DevParticles S)
+// Each workgroup places its chunk: ranks inside a key come from LDS atomics.
+__global__ void __launch_bounds__(BLOCK) k_part_scatter(DevParticles S, Part A)
 {
   __shared__ int sh_off[SORT_MAX_BINS];
-  const int n = S.qcount[QC_XS];
+  const int n = *A.n_in;
   const int nw = sort_workgroups(n, gridDim.x);
   if (static_cast<int>(blockIdx.x) >= nw)
     return;
-  const int nb = S.sort_nb;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x)
+  for (int k = threadIdx.x; k < A.nb; k += blockDim.x)
     sh_off[k] = S.sort_total[k] + S.sort_count[k * SORT_MAX_BLOCKS + blockIdx.x];
   __syncthreads();
   const int chunk = (n + nw - 1) / nw;
   const int lo = blockIdx.x * chunk;
   const int hi = min(n, lo + chunk);
   for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const int pos = atomicAdd(&sh_off[S.q_key[i]], 1);
-    S.q_sorted[pos] = S.q_xs[i];
+    const int k = part_key(A, i);
+    if (k >= 0) {
+      const int pos = atomicAdd(&sh_off[k], 1);
+      A.out[pos] = A.val[i];
+    }
   }
 }
 
@@ -471,11 +490,6 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
 {
   extern __shared__ double sh_xs[];
   const int n = S.qcount[QC_XS];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.qcount[QC_ADV] = n;
-    S.qcount[QC_SURF] = 0;
-    S.qcount[QC_COLL] = 0;
-  }
   constexpr int PPB = BLOCK / LPP; // particles per block pass
   const int slot = threadIdx.x / LPP;
   const int sub = threadIdx.x % LPP;
@@ -559,9 +573,9 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
     }
     __syncthreads();
   }
-  wave_count(&S.counters[CNT_LOOKUPS], n_lookups);
-  wave_count(&S.counters[CNT_XS_EVENTS], n_events);
-  wave_count(&S.counters[CNT_SEARCH_PROBES], n_probes);
+  wave_count(S, CNT_LOOKUPS, n_lookups);
+  wave_count(S, CNT_XS_EVENTS, n_events);
+  wave_count(S, CNT_SEARCH_PROBES, n_probes);
 }
 
 // One lane per particle: the lane looks up the material's nuclides G at a
@@ -571,11 +585,6 @@ template<int NL, int G>
 __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S, const int* q)
 {
   const int n = S.qcount[QC_XS];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.qcount[QC_ADV] = n;
-    S.qcount[QC_SURF] = 0;
-    S.qcount[QC_COLL] = 0;
-  }
   const bool learned = S.use_learned != 0;
   const bool trace = S.trace_hash != 0;
   unsigned long long n_lookups = 0, n_events = 0, n_probes = 0;
@@ -632,9 +641,9 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
     if (trace)
       S.hash[p] = hh;
   }
-  wave_count(&S.counters[CNT_LOOKUPS], n_lookups);
-  wave_count(&S.counters[CNT_XS_EVENTS], n_events);
-  wave_count(&S.counters[CNT_SEARCH_PROBES], n_probes);
+  wave_count(S, CNT_LOOKUPS, n_lookups);
+  wave_count(S, CNT_XS_EVENTS, n_events);
+  wave_count(S, CNT_SEARCH_PROBES, n_probes);
 }
 
 //==============================================================================
@@ -644,16 +653,12 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
 template<int NL>
 __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S, const int* q)
 {
-  const int n = S.qcount[QC_ADV];
-  if (blockIdx.x == 0 && threadIdx.x == 0)
-    S.qcount[QC_XS] = 0;
+  const int n = S.qcount[QC_XS];
   unsigned long long n_events = 0, n_segs = 0, n_lost = 0;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const int i = base + threadIdx.x;
-    bool to_surf = false, to_coll = false;
-    int p = 0;
-    if (i < n) {
-      p = q[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int outcome = -1;
+    {
+      const int p = q[i];
       const PHot h = S.hot[p];
       double wgt = h.wgt;
       if (wgt != 0.0) {
@@ -715,18 +720,15 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S,
           hmix(hh, d_coll > b.distance ? 2 : 1);
           S.hash[p] = hh;
         }
-        if (wgt != 0.0) {
-          to_surf = d_coll > b.distance;
-          to_coll = !to_surf;
-        }
+        if (wgt != 0.0)
+          outcome = d_coll > b.distance ? 0 : 1;
       }
     }
-    wave_push(S.q_surf, &S.qcount[QC_SURF], to_surf, p);
-    wave_push(S.q_coll, &S.qcount[QC_COLL], to_coll, p);
+    S.adv_key[i] = outcome;
   }
-  wave_count(&S.counters[CNT_ADV_EVENTS], n_events);
-  wave_count(&S.counters[CNT_SEGMENTS], n_segs);
-  wave_count(&S.counters[CNT_LOST], n_lost);
+  wave_count(S, CNT_ADV_EVENTS, n_events);
+  wave_count(S, CNT_SEGMENTS, n_segs);
+  wave_count(S, CNT_LOST, n_lost);
 }
 
 //==============================================================================
@@ -739,13 +741,11 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
   const int n = S.qcount[QC_SURF];
   unsigned long long n_events = 0, n_lost = 0;
   double leak = 0.0;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const int i = base + threadIdx.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     bool push = false;
     int key = 0;
-    int p = 0;
-    if (i < n) {
-      p = S.q_surf[i];
+    const int p = S.q_sc[i];
+    {
       n_events++;
       const PHot h = S.hot[p];
       Geo<NL> g;
@@ -828,14 +828,15 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
       push = wgt != 0.0;
       key = sort_key(S, g.material, h.E);
     }
-    wave_push_xs(S, push, p, key);
+    S.stage_p[i] = p;
+    S.stage_key[i] = push ? key : -1;
   }
-  wave_count(&S.counters[CNT_SURF_EVENTS], n_events);
-  wave_count(&S.counters[CNT_LOST], n_lost);
+  wave_count(S, CNT_SURF_EVENTS, n_events);
+  wave_count(S, CNT_LOST, n_lost);
   for (int o = 32; o > 0; o >>= 1)
     leak += __shfl_xor(leak, o);
   if (lane_id() == 0 && leak != 0.0)
-    unsafeAtomicAdd(S.leak, leak);
+    unsafeAtomicAdd(&S.leak[wave_slot()], leak);
 }
 
 //==============================================================================
@@ -875,14 +876,13 @@ template<int NL>
 __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_COLL];
+  const int n_surf = S.qcount[QC_SURF]; // collisions follow the crossings in q_sc
   unsigned long long n_events = 0, n_lost = 0, n_lookups = 0, n_probes = 0;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const int i = base + threadIdx.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     bool push = false;
     int key = 0;
-    int p = 0;
-    if (i < n) {
-      p = S.q_coll[i];
+    const int p = S.q_sc[n_surf + i];
+    {
       n_events++;
       const PHot h = S.hot[p];
       double E = h.E;
@@ -983,13 +983,14 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
       push = wgt != 0.0;
       key = sort_key(S, material, E);
     }
-    wave_push_xs(S, push, p, key);
+    S.stage_p[n_surf + i] = p;
+    S.stage_key[n_surf + i] = push ? key : -1;
   }
-  wave_count(&S.counters[CNT_COLL_EVENTS], n_events);
-  wave_count(&S.counters[CNT_COLLISIONS], n_events);
-  wave_count(&S.counters[CNT_LOST], n_lost);
-  wave_count(&S.counters[CNT_LOOKUPS], n_lookups);
-  wave_count(&S.counters[CNT_SEARCH_PROBES], n_probes);
+  wave_count(S, CNT_COLL_EVENTS, n_events);
+  wave_count(S, CNT_COLLISIONS, n_events);
+  wave_count(S, CNT_LOST, n_lost);
+  wave_count(S, CNT_LOOKUPS, n_lookups);
+  wave_count(S, CNT_SEARCH_PROBES, n_probes);
 }
 
 //==============================================================================
@@ -1057,22 +1058,36 @@ void xs_lpp(const DevProblem& P, const DevParticles& S, const int* q, const Laun
   }
 }
 
+void launch_partition(const DevParticles& S, const Part& A, const LaunchConfig& lc, hipStream_t st)
+{
+  const int sb = lc.sort_blocks < 1 ? 1 : (lc.sort_blocks > SORT_MAX_BLOCKS ? SORT_MAX_BLOCKS : lc.sort_blocks);
+  const int row_blocks = (A.nb * 64 + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_part_count, dim3(sb), dim3(BLOCK), 0, st, S, A);
+  hipLaunchKernelGGL(k_part_rows, dim3(row_blocks), dim3(BLOCK), 0, st, S, A, sb);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, S, A);
+  hipLaunchKernelGGL(k_part_scatter, dim3(sb), dim3(BLOCK), 0, st, S, A);
+}
+
 template<int NL>
 void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
 {
-  const int* q = lc.sort ? S.q_sorted : S.q_xs;
   switch (which) {
-  case KERNEL_SORT:
-    if (lc.sort) {
-      const int sb = lc.sort_blocks < 1 ? 1 : (lc.sort_blocks > SORT_MAX_BLOCKS ? SORT_MAX_BLOCKS : lc.sort_blocks);
-      hipLaunchKernelGGL(k_sort_count, dim3(sb), dim3(BLOCK), 0, st, S);
-      hipLaunchKernelGGL(k_sort_rows, dim3((S.sort_nb + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, sb);
-      hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, S);
-      hipLaunchKernelGGL(k_sort_scatter, dim3(sb), dim3(BLOCK), 0, st, S);
-    }
+  case KERNEL_SORT: {
+    // staged survivors -> q_xs, sorted by key (or only compacted)
+    Part A {S.stage_p, S.stage_key, S.qcount + QC_STAGE, S.q_xs, S.qcount + QC_XS, nullptr,
+      lc.sort ? S.sort_nb : 1};
+    launch_partition(S, A, lc, st);
     break;
-  case KERNEL_XS: xs_lpp<NL>(P, S, q, lc, st); break;
-  case KERNEL_ADVANCE: hipLaunchKernelGGL(k_advance<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  }
+  case KERNEL_XS: xs_lpp<NL>(P, S, S.q_xs, lc, st); break;
+  case KERNEL_ADVANCE: hipLaunchKernelGGL(k_advance<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S, S.q_xs); break;
+  case KERNEL_SPLIT: {
+    // advance outcomes -> q_sc = [surface crossings | collisions]; the two
+    // counts land in QC_SURF and QC_COLL, their sum in QC_STAGE
+    Part A {S.q_xs, S.adv_key, S.qcount + QC_XS, S.q_sc, S.qcount + QC_STAGE, S.qcount + QC_SURF, 2};
+    launch_partition(S, A, lc, st);
+    break;
+  }
   case KERNEL_SURFACE: hipLaunchKernelGGL(k_surface<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S); break;
   default: hipLaunchKernelGGL(k_collision<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S); break;
   }
