@@ -739,56 +739,104 @@ __device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, doub
 //    that point only needs reading when the lower bound is i_low;
 //  - the duplicate-point bump reads the next point only when it applies.
 // Entries k >= kk are not computed (their outputs are left unchanged).
-template<int K>
+// nuc_list and kk must be the same in every active lane of the wavefront:
+// the nuclide indices, grid offsets and sizes are then held in scalar
+// registers (readfirstlane of values every active lane has loaded alike).
+__device__ __forceinline__ int wave_uniform(int x)
+{
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+__device__ __forceinline__ int64_t wave_uniform(int64_t x)
+{
+  const uint64_t u = static_cast<uint64_t>(x);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(u & 0xffffffffu)));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ double wave_uniform(double x)
+{
+  return __longlong_as_double(wave_uniform(static_cast<int64_t>(__double_as_longlong(x))));
+}
+
+template<int K, bool LEARNED>
 __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int* nuc_list, int kk, double E,
-  int i_log, double logE, bool learned, int& probes, Micro* out)
+  int i_log, double logE, int& probes, Micro* out)
 {
   const double* Eg[K];
   int64_t off[K];
-  int n[K], fst[K], cnt[K], lo0[K], ig[K];
+  int n[K], fst[K], cnt[K], lo0[K], ig[K], nuc[K];
   bool search[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int i_nuc = nuc_list[k < kk ? k : 0];
-    const DevNuclide nd = P.nuc[i_nuc];
-    off[k] = nd.off;
-    n[k] = nd.n;
-    Eg[k] = P.E + nd.off;
+    nuc[k] = wave_uniform(nuc_list[k < kk ? k : 0]);
+    const DevNuclide nd = P.nuc[nuc[k]];
+    off[k] = wave_uniform(nd.off);
+    n[k] = wave_uniform(nd.n);
+    Eg[k] = P.E + off[k];
   }
-  // bracket by the log-grid bucket (or the learned window)
+  // bracket by the log-grid bucket (or the learned window); all K nuclides'
+  // loads are issued before any is used
+  double e_first[K], e_last[K];
+  int lo[K], hi[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int i_nuc = nuc_list[k < kk ? k : 0];
-    const double e_first = Eg[k][0];
-    const double e_last = Eg[k][n[k] - 1];
-    const int* gi = P.grid_index + static_cast<int64_t>(i_nuc) * (P.n_log_bins + 1);
-    int lo, hi;
-    if (!learned) {
-      lo = gi[i_log];
-      hi = gi[i_log + 1] + 1;
-    } else {
-      const DevLearned& Lm = P.learned;
-      double uu = (logE - Lm.u_lo[i_nuc]) * Lm.inv_w[i_nuc];
-      int s = uu < 0.0 ? 0 : static_cast<int>(uu);
-      s = s >= Lm.n_seg ? Lm.n_seg - 1 : s;
-      const int64_t sk = static_cast<int64_t>(i_nuc) * Lm.n_seg + s;
-      const float2 sg = reinterpret_cast<const float2*>(Lm.seg)[sk];
-      const int err = Lm.err[sk];
-      float fr = static_cast<float>(uu - static_cast<double>(s));
-      float pred = sg.x * fr + sg.y;
-      int j = static_cast<int>(pred);
-      lo = j - err - 1;
-      hi = j + err + 3;
-      lo = lo < 0 ? 0 : (lo > n[k] ? n[k] : lo);
-      hi = hi > n[k] ? n[k] : (hi < lo ? lo : hi);
+    e_first[k] = Eg[k][0];
+    e_last[k] = Eg[k][n[k] - 1];
+    if (!LEARNED) {
+      const int* gi = P.grid_index + static_cast<int64_t>(nuc[k]) * (P.n_log_bins + 1);
+      lo[k] = gi[i_log];
+      hi[k] = gi[i_log + 1];
     }
-    const bool below = E < e_first;
-    const bool above = !below && E > e_last;
+  }
+  if (LEARNED) {
+    const DevLearned& Lm = P.learned;
+    double u_lo[K], inv_w[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      u_lo[k] = wave_uniform(Lm.u_lo[nuc[k]]);
+      inv_w[k] = wave_uniform(Lm.inv_w[nuc[k]]);
+    }
+    float2 sg[K];
+    int err[K];
+    double uu[K];
+    int seg[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      uu[k] = (logE - u_lo[k]) * inv_w[k];
+      int sgi = uu[k] < 0.0 ? 0 : static_cast<int>(uu[k]);
+      seg[k] = sgi >= Lm.n_seg ? Lm.n_seg - 1 : sgi;
+      const int64_t sk = static_cast<int64_t>(nuc[k]) * Lm.n_seg + seg[k];
+      sg[k] = reinterpret_cast<const float2*>(Lm.seg)[sk];
+      err[k] = Lm.err[sk];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      float fr = static_cast<float>(uu[k] - static_cast<double>(seg[k]));
+      float pred = sg[k].x * fr + sg[k].y;
+      int j = static_cast<int>(pred);
+      int l = j - err[k] - 1;
+      int h = j + err[k] + 3;
+      l = l < 0 ? 0 : (l > n[k] ? n[k] : l);
+      h = h > n[k] ? n[k] : (h < l ? l : h);
+      lo[k] = l;
+      hi[k] = h;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      hi[k] += 1;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool below = E < e_first[k];
+    const bool above = !below && E > e_last[k];
     search[k] = k < kk && !below && !above;
     ig[k] = below ? 0 : n[k] - 2;
-    lo0[k] = lo;
-    fst[k] = lo;
-    cnt[k] = search[k] ? hi - lo : 0;
+    lo0[k] = lo[k];
+    fst[k] = lo[k];
+    cnt[k] = search[k] ? hi[k] - lo[k] : 0;
   }
   // lockstep lower_bound searches
   while (true) {
@@ -821,15 +869,14 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int*
     if (!search[k])
       continue;
     const int lb = fst[k];
-    if (!learned) {
+    if (!LEARNED) {
       ig[k] = lb - 1;
       if (lb == lo0[k] && Eg[k][lb] == E)
         ig[k] = lb;
     } else {
       ig[k] = lb - 1;
       if (lb < n[k] && Eg[k][lb] == E) {
-        const int i_nuc = nuc_list[k];
-        const int il = P.grid_index[static_cast<int64_t>(i_nuc) * (P.n_log_bins + 1) + i_log];
+        const int il = P.grid_index[static_cast<int64_t>(nuc[k]) * (P.n_log_bins + 1) + i_log];
         if (Eg[k][il] == E)
           ig[k] = il;
       }

@@ -578,68 +578,103 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
   wave_count(S, CNT_SEARCH_PROBES, n_probes);
 }
 
+// Macroscopic cross sections of one particle (material given), nuclides
+// looked up G at a time; sums in material order.  Returns the lookups done.
+// The material must be the same in every active lane of the wavefront.
+template<int G, bool LEARNED>
+__device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticles& S, int p, PHot* h,
+  int material, double E, bool trace, int& probes)
+{
+  const double lx = detm::log(E / P.energy_min);
+  int i_log = static_cast<int>(lx / P.log_spacing);
+  i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+  const int j0 = wave_uniform(P.mat_off[material]);
+  const int nn = wave_uniform(P.mat_off[material + 1]) - j0;
+  double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+  uint64_t hh = 0;
+  if (trace) {
+    hh = S.hash[p];
+    hmix(hh, 0x11);
+    hmix(hh, static_cast<uint64_t>(material));
+  }
+  double xs_t = 0.0;
+  double xs_a = 0.0;
+  for (int jj = 0; jj < nn; jj += G) {
+    const int kk = nn - jj < G ? nn - jj : G;
+    Micro m[G];
+    nuclide_xs_group<G, LEARNED>(P, P.mat_nuc + j0 + jj, kk, E, i_log, lx, probes, m);
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k < kk) {
+        micro[jj + k] = m[k].total;
+        const double dens = wave_uniform(P.mat_dens[j0 + jj + k]) * 1.0;
+        xs_t += dens * m[k].total;
+        xs_a += dens * m[k].absorption;
+        if (trace)
+          hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
+      }
+    }
+  }
+  h->xs_t = xs_t;
+  h->xs_a = xs_a;
+  if (trace)
+    S.hash[p] = hh;
+  return nn;
+}
+
 // One lane per particle: the lane looks up the material's nuclides G at a
 // time (their searches interleaved) and forms the sums in material order as
 // it goes, so no LDS is needed and occupancy is set by registers alone.
-template<int NL, int G>
+// A wavefront handles one material at a time (the queue is sorted by
+// material, so almost always in one pass): the material is then
+// wavefront-uniform and its nuclide list, grid offsets and sizes are read
+// with scalar loads into scalar registers.
+template<int NL, int G, bool LEARNED>
 __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S, const int* q)
 {
   const int n = S.qcount[QC_XS];
-  const bool learned = S.use_learned != 0;
   const bool trace = S.trace_hash != 0;
   unsigned long long n_lookups = 0, n_events = 0, n_probes = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int p = q[i];
-    n_events++;
-    PHot* h = S.hot + p;
-    const int material = h->material;
-    if (h->wgt == 0.0)
-      continue;
-    if (material == MATERIAL_VOID) {
-      h->xs_t = 0.0;
-      h->xs_a = 0.0;
-      continue;
-    }
-    if (material == h->material_last)
-      continue;
-    const double E = h->E;
-    const double lx = detm::log(E / P.energy_min);
-    int i_log = static_cast<int>(lx / P.log_spacing);
-    i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
-    const int j0 = P.mat_off[material];
-    const int nn = P.mat_off[material + 1] - j0;
-    double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
-    uint64_t hh = 0;
-    if (trace) {
-      hh = S.hash[p];
-      hmix(hh, 0x11);
-      hmix(hh, static_cast<uint64_t>(material));
-    }
-    double xs_t = 0.0;
-    double xs_a = 0.0;
-    int probes = 0;
-    for (int jj = 0; jj < nn; jj += G) {
-      const int kk = nn - jj < G ? nn - jj : G;
-      Micro m[G];
-      nuclide_xs_group<G>(P, P.mat_nuc + j0 + jj, kk, E, i_log, lx, learned, probes, m);
-#pragma unroll
-      for (int k = 0; k < G; ++k) {
-        if (k < kk) {
-          micro[jj + k] = m[k].total;
-          const double dens = P.mat_dens[j0 + jj + k] * 1.0;
-          xs_t += dens * m[k].total;
-          xs_a += dens * m[k].absorption;
-          if (trace)
-            hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
+  // every lane of a wavefront runs the same number of iterations, so the
+  // wavefront votes below see all of its lanes that hold a particle
+  const int stride = gridDim.x * blockDim.x;
+  const int first = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  for (int base = first; base < n; base += stride) {
+    const int i = base + lane_id();
+    bool need = false;
+    int material = MATERIAL_VOID;
+    int p = 0;
+    PHot* h = nullptr;
+    double E = 0.0;
+    if (i < n) {
+      p = q[i];
+      n_events++;
+      h = S.hot + p;
+      material = h->material;
+      if (h->wgt != 0.0) {
+        if (material == MATERIAL_VOID) {
+          h->xs_t = 0.0;
+          h->xs_a = 0.0;
+        } else if (material != h->material_last) {
+          need = true;
+          E = h->E;
         }
       }
     }
-    n_lookups += nn;
-    n_probes += probes;
-    h->xs_t = xs_t;
-    h->xs_a = xs_a;
-    if (trace)
-      S.hash[p] = hh;
+    // one pass per distinct material among the lanes that need cross
+    // sections (one pass for almost every wavefront of a sorted queue)
+    unsigned long long todo = __ballot(need);
+    while (todo != 0ull) {
+      const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      const int mat = __builtin_amdgcn_readfirstlane(__shfl(material, leader));
+      const bool mine = need && material == mat;
+      if (mine) {
+        int probes = 0;
+        n_lookups += particle_xs<G, LEARNED>(P, S, p, h, mat, E, trace, probes);
+        n_probes += probes;
+      }
+      todo &= ~__ballot(mine);
+    }
   }
   wave_count(S, CNT_LOOKUPS, n_lookups);
   wave_count(S, CNT_XS_EVENTS, n_events);
@@ -1032,15 +1067,24 @@ void xs_nl(const DevProblem& P, const DevParticles& S, const int* q, int blocks,
   hipLaunchKernelGGL((k_xs<NL, LPP>), dim3(b), dim3(BLOCK), lds, st, P, S, q);
 }
 
+template<int NL, bool LEARNED>
+void xs_lane_l(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int group, hipStream_t st)
+{
+  switch (group) {
+  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  default: hipLaunchKernelGGL((k_xs_lane<NL, 4, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  }
+}
+
 template<int NL>
 void xs_lane(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int group, hipStream_t st)
 {
-  switch (group) {
-  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  default: hipLaunchKernelGGL((k_xs_lane<NL, 4>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  }
+  if (S.use_learned)
+    xs_lane_l<NL, true>(P, S, q, blocks, group, st);
+  else
+    xs_lane_l<NL, false>(P, S, q, blocks, group, st);
 }
 
 template<int NL>
