@@ -156,7 +156,72 @@ def config2(n_particles=10_000_000, n_batches=20, n_nuclides=300, n_grid=200000,
     return model, nucs
 
 
-BUILDERS = {0: config0, 1: config1, 2: config2}
+def config4(n_particles=10_000_000, n_batches=16, n_grid=200000, n_layers=5, layer=20.0, per_layer=20,
+            half_width=50.0, density=0.01, ww_mesh=(100, 50, 50), ww_length=8.0, upper_ratio=5.0,
+            tally_mesh=(200, 200, 200), energy_groups=10, event_based=True):
+    """Deep-penetration slab with mesh weight windows.
+
+    Layers of `layer` cm along x (x = 0 and x = n_layers * layer vacuum,
+    the lateral faces reflective), each a distinct mix of `per_layer`
+    nuclides drawn (seeded) from config 2's 300, at `density` atom/b-cm.  A
+    14 MeV monodirectional (+x) plane source sits 10 um inside the x = 0 face
+    (a point exactly on the face has no cell for a direction along the face).
+    Weight windows on a `ww_mesh` mesh over the slab: lower bound
+    a0 * exp(-x / ww_length) at the bin centre, a0 ~ U[0.3, 0.6] from the seed,
+    upper = upper_ratio * lower, one energy group; splitting and roulette at
+    birth and collisions.  Tally: track-length flux on `tally_mesh` with
+    `energy_groups` log-spaced groups over [1e-5 eV, 20 MeV].
+    """
+    om.reset_auto_ids()
+    rng = np.random.default_rng(DATA_SEED_BASE + 44)
+    pick = sorted(int(k) for k in rng.choice(300, size=n_layers * per_layer, replace=False))
+    nucs = [data.synthetic_nuclide("H%03d" % k, (DATA_SEED_BASE + 2) * 1000 + k, n_grid=n_grid) for k in pick]
+    mats = []
+    for i in range(n_layers):
+        mt = om.Material(name="layer%d" % i)
+        for n in nucs[i * per_layer:(i + 1) * per_layer]:
+            mt.add_nuclide(n.name, float(rng.uniform(0.005, 0.05)))
+        mt.set_density("atom/b-cm", density)
+        mats.append(mt)
+    xs = [om.XPlane(i * layer, boundary_type="vacuum" if i in (0, n_layers) else "transmission")
+          for i in range(n_layers + 1)]
+    ym = om.YPlane(-half_width, boundary_type="reflective")
+    yp = om.YPlane(half_width, boundary_type="reflective")
+    zm = om.ZPlane(-half_width, boundary_type="reflective")
+    zp = om.ZPlane(half_width, boundary_type="reflective")
+    cells = [om.Cell(fill=mats[i], region=+xs[i] & -xs[i + 1] & +ym & -yp & +zm & -zp, name="layer%d" % i)
+             for i in range(n_layers)]
+    geom = om.Geometry(om.Universe(cells=cells))
+    depth = n_layers * layer
+    s = om.Settings()
+    s.particles = n_particles
+    s.batches = n_batches
+    s.event_based = event_based
+    s.source = om.IndependentSource(
+        space=stats.Box((1.0e-3, -half_width, -half_width), (1.0e-3, half_width, half_width)),
+        angle=stats.Monodirectional((1.0, 0.0, 0.0)), energy=stats.Discrete([14.0e6], [1.0]))
+    wm = om.RegularMesh()
+    wm.dimension = ww_mesh
+    wm.lower_left = (0.0, -half_width, -half_width)
+    wm.upper_right = (depth, half_width, half_width)
+    a0 = float(rng.uniform(0.3, 0.6))
+    xc = (np.arange(ww_mesh[0]) + 0.5) * depth / ww_mesh[0]
+    lower = np.broadcast_to((a0 * np.exp(-xc / ww_length))[:, None, None, None],
+                            tuple(ww_mesh) + (1,)).copy()
+    s.weight_windows = [om.WeightWindows(wm, lower, lower * upper_ratio)]
+    tm = om.RegularMesh()
+    tm.dimension = tally_mesh
+    tm.lower_left = (0.0, -half_width, -half_width)
+    tm.upper_right = (depth, half_width, half_width)
+    t = om.Tally(name="flux")
+    t.filters = [om.MeshFilter(tm)]
+    if energy_groups:
+        t.filters.append(om.EnergyFilter(list(np.geomspace(1.0e-5, 2.0e7, energy_groups + 1))))
+    model = om.Model(geom, om.Materials(mats), s, om.Tallies([t]))
+    return model, nucs
+
+
+BUILDERS = {0: config0, 1: config1, 2: config2, 4: config4}
 
 
 def cache_root():

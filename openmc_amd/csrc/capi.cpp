@@ -115,8 +115,11 @@ struct Sim {
   int sort_blocks_opt = 0; // 0 = one workgroup per CU
   int blocks_opt = 0;      // 0 = from the capacity
   int xs_group_opt = 4;
+  int bank_cap = 32; // secondary bank sites per particle (weight windows)
   LaunchConfig lc {1, 1, 0, 1, 4};
   int64_t max_in_flight_opt = 0;
+  std::vector<int64_t> nuc_off; // first grid point of each nuclide
+  void* matnuc_alloc = nullptr;
   // graph
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -172,6 +175,38 @@ void destroy_graph(Sim& s)
   s.graph_valid = false;
 }
 
+// Per-(material, nuclide) lookup records, with the learned index's scale
+// once it is built.
+void upload_matnuc(Sim& s)
+{
+  const HostModel& m = s.m;
+  std::vector<DevMatNuc> rec(m.mat_nuc.size());
+  const bool lrn = s.learned.h_ulo.size() == m.nuclides.size() && !m.nuclides.empty();
+  for (size_t j = 0; j < m.mat_nuc.size(); ++j) {
+    const int k = m.mat_nuc[j];
+    const auto& nuc = m.nuclides[k];
+    DevMatNuc& r = rec[j];
+    r.off = s.nuc_off[k];
+    r.n = static_cast<int>(nuc.energy.size());
+    r.nuc = k;
+    r.e_first = nuc.energy.empty() ? 0.0 : nuc.energy.front();
+    r.e_last = nuc.energy.empty() ? 0.0 : nuc.energy.back();
+    r.dens = m.mat_dens[j];
+    r.u_lo = lrn ? s.learned.h_ulo[k] : 0.0;
+    r.inv_w = lrn ? s.learned.h_invw[k] : 0.0;
+  }
+  if (s.matnuc_alloc) {
+    (void)hipFree(s.matnuc_alloc);
+    s.matnuc_alloc = nullptr;
+  }
+  if (!rec.empty()) {
+    hip_check(hipMalloc(&s.matnuc_alloc, sizeof(DevMatNuc) * rec.size()), "hipMalloc");
+    hip_check(hipMemcpy(s.matnuc_alloc, rec.data(), sizeof(DevMatNuc) * rec.size(), hipMemcpyHostToDevice),
+      "hipMemcpy");
+  }
+  s.P.matnuc = static_cast<const DevMatNuc*>(s.matnuc_alloc);
+}
+
 void upload_problem(Sim& s)
 {
   HostModel& m = s.m;
@@ -179,6 +214,7 @@ void upload_problem(Sim& s)
   std::memset(&P, 0, sizeof(P));
 
   std::vector<DevNuclide> nucs;
+  s.nuc_off.clear();
   std::vector<double> E, EL;
   std::vector<double2> XS;
   int64_t off = 0;
@@ -188,7 +224,10 @@ void upload_problem(Sim& s)
     d.n = static_cast<int>(n.energy.size());
     d.awr = n.awr;
     d.kT = n.kT;
+    d.e_first = n.energy.empty() ? 0.0 : n.energy.front();
+    d.e_last = n.energy.empty() ? 0.0 : n.energy.back();
     nucs.push_back(d);
+    s.nuc_off.push_back(off);
     off += d.n;
   }
   E.reserve(off);
@@ -219,10 +258,25 @@ void upload_problem(Sim& s)
   for (size_t k = 0; k + 1 < m.mat_off.size(); ++k)
     mx = std::max(mx, m.mat_off[k + 1] - m.mat_off[k]);
   P.max_mat_nuc = mx;
+  upload_matnuc(s);
 
   P.surf = dev_upload(s, m.surfaces);
   P.cell = dev_upload(s, m.cells);
   P.tokens = dev_upload(s, m.tokens);
+  {
+    // region tokens with their surfaces inlined (index k matches tokens[k])
+    std::vector<DevTok> ctok(m.tokens.size());
+    for (size_t k = 0; k < m.tokens.size(); ++k) {
+      const int t = m.tokens[k];
+      const size_t i = static_cast<size_t>(std::abs(t)) - 1;
+      if (t == 0 || i >= m.surfaces.size())
+        throw std::runtime_error("region token " + std::to_string(t) + " names no surface");
+      ctok[k].s = m.surfaces[i];
+      ctok[k].token = t;
+      ctok[k].pad = 0;
+    }
+    P.ctok = dev_upload(s, ctok);
+  }
   P.univ = dev_upload(s, m.universes);
   P.univ_cells = dev_upload(s, m.univ_cells);
   P.lat = dev_upload(s, m.lattices);
@@ -254,6 +308,30 @@ void upload_problem(Sim& s)
     T.value = dev_alloc<double>(s.dev_allocs, T.n_bins);
     s.d_results = dev_alloc<double>(s.dev_allocs, 3 * T.n_bins);
     s.h_results.assign(3 * T.n_bins, 0.0);
+  }
+
+  if (m.ww.on) {
+    const HostWW& w = m.ww;
+    DevWW& d = P.ww;
+    d.on = 1;
+    d.ckpt_collision = w.ckpt_collision ? 1 : 0;
+    d.ckpt_surface = w.ckpt_surface ? 1 : 0;
+    d.max_split = w.max_split;
+    d.n_dim = w.mesh.n_dim;
+    for (int k = 0; k < 3; ++k) {
+      d.shape[k] = w.mesh.shape[k];
+      d.ll[k] = w.mesh.ll[k];
+      d.width[k] = w.mesh.width[k];
+    }
+    d.n_e = static_cast<int>(w.ebounds.size()) - 1;
+    d.n_mesh = w.mesh.n_bins();
+    d.ebounds = dev_upload(s, w.ebounds);
+    d.lower = dev_upload(s, w.lower);
+    d.upper = dev_upload(s, w.upper);
+    d.survival_ratio = w.survival_ratio;
+    d.max_lb_ratio = w.max_lb_ratio;
+    d.weight_cutoff = w.weight_cutoff;
+    d.max_history_splits = w.max_history_splits;
   }
 
   P.n_particles = m.n_particles;
@@ -300,6 +378,16 @@ void alloc_particles(Sim& s, int64_t cap)
   S.lost = dev_alloc<int>(L, cap);
   S.hash = dev_alloc<uint64_t>(L, cap);
   S.leak = dev_alloc<double>(L, CNT_SLOTS);
+  S.total_weight = dev_alloc<double>(L, CNT_SLOTS);
+  S.status = dev_alloc<int>(L, 1);
+  S.bank_cap = s.bank_cap;
+  if (s.P.ww.on) {
+    S.ww = dev_alloc<PWW>(L, cap);
+    S.bank = dev_alloc<PSite>(L, static_cast<size_t>(cap) * s.bank_cap);
+  } else {
+    S.ww = nullptr;
+    S.bank = nullptr;
+  }
   S.q_xs = dev_alloc<int>(L, cap);
   S.adv_key = dev_alloc<int>(L, cap);
   S.q_sc = dev_alloc<int>(L, cap);
@@ -340,9 +428,11 @@ void alloc_particles(Sim& s, int64_t cap)
 int64_t bytes_per_particle(const Sim& s)
 {
   const int64_t deep = std::max(1, s.P.n_levels - 1);
-  // hot record, deeper levels, nuclide totals, lost flag, hash, four queues
+  // hot record, deeper levels, nuclide totals, lost flag, hash, five queue
+  // and staging arrays, and the weight-window record and bank when on
+  const int64_t ww = s.P.ww.on ? static_cast<int64_t>(sizeof(PWW)) + s.bank_cap * sizeof(PSite) : 0;
   return static_cast<int64_t>(sizeof(PHot)) + deep * static_cast<int64_t>(sizeof(PLevel)) +
-         std::max(1, s.P.max_mat_nuc) * 8 + 4 + 8 + 4 * 4;
+         std::max(1, s.P.max_mat_nuc) * 8 + 4 + 8 + 5 * 4 + ww;
 }
 
 void choose_capacity(Sim& s)
@@ -435,6 +525,8 @@ void transport_batch(Sim& s)
   if (s.trace_hash)
     s.h_hashes.assign(m.n_particles, 0);
   hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double) * CNT_SLOTS, s.stream), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(s.S.total_weight, 0, sizeof(double) * CNT_SLOTS, s.stream), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(s.S.status, 0, sizeof(int), s.stream), "hipMemsetAsync");
 
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
@@ -492,6 +584,18 @@ void transport_batch(Sim& s)
   double leak = 0.0;
   for (double v : s.h_leak_slots)
     leak += v;
+  // simulation::total_weight: the source weight after the birth weight
+  // windows (src/simulation.cpp:637-639)
+  hip_check(hipMemcpy(s.h_leak_slots.data(), s.S.total_weight, sizeof(double) * CNT_SLOTS, hipMemcpyDeviceToHost),
+    "hipMemcpy");
+  double total_weight = 0.0;
+  for (double v : s.h_leak_slots)
+    total_weight += v;
+  int status = 0;
+  hip_check(hipMemcpy(&status, s.S.status, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  if (status & 1)
+    throw std::runtime_error("a secondary bank overflowed its " + std::to_string(s.bank_cap) +
+                             " sites (weight-window splitting); raise the option bank_cap");
   auto w1 = std::chrono::steady_clock::now();
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
@@ -503,8 +607,7 @@ void transport_batch(Sim& s)
     s.active_batches++;
   }
 
-  // total_weight: every source particle starts with unit weight
-  s.total_weight = static_cast<double>(m.n_particles);
+  s.total_weight = total_weight;
 
   // accumulate_tallies (src/tallies/tally.cpp:1059-1104)
   if (active) {
@@ -741,6 +844,8 @@ int openmc_finalize(void)
     (void)hipStreamSynchronize(g->stream);
     free_particles(*g);
     free_learned(g->learned);
+    if (g->matnuc_alloc)
+      (void)hipFree(g->matnuc_alloc);
     free_list(g->dev_allocs);
     if (g->h_qcount)
       (void)hipHostFree(g->h_qcount);
@@ -898,6 +1003,18 @@ int omc_set_option(const char* name, double value)
   } else if (n == "sort") {
     g->sort = value != 0;
     update_launch_config(*g);
+  } else if (n == "bank_cap") {
+    int v = static_cast<int>(value);
+    if (v < 1 || v > 4096) {
+      set_errmsg("bank_cap must be in [1, 4096]");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    g->bank_cap = v;
+    if (g->sim_initialized)
+      return guarded([&]() -> int {
+        alloc_particles(*g, g->cap);
+        return 0;
+      });
   } else if (n == "xs_group") {
     int v = static_cast<int>(value);
     if (v != 1 && v != 2 && v != 4 && v != 8) {
@@ -1150,6 +1267,7 @@ int omc_learned_build(int n_seg, int epochs, int* max_err)
     return OPENMC_E_ALLOCATE;
   return guarded([&]() -> int {
     int err = build_learned_index(g->learned, g->m, g->P, n_seg, epochs, g->stream);
+    upload_matnuc(*g);
     if (max_err)
       *max_err = err;
     g->graph_valid = false;

@@ -43,6 +43,15 @@ struct DevSurface {
   double c[4];
 };
 
+// One token of a cell's region with its surface inlined, so a region is
+// walked with one independent load per token (no token -> surface chain).
+struct DevTok {
+  DevSurface s;
+  int token; // signed surface index + 1 (the region token)
+  int pad;
+};
+static_assert(sizeof(DevTok) == 48, "three 16-byte loads per token");
+
 struct DevCell {
   int universe;
   int fill_type;
@@ -74,7 +83,24 @@ struct DevNuclide {
   int pad;
   double awr;
   double kT;
+  double e_first; // E[0] and E[n-1] of the grid (the below/above-grid tests)
+  double e_last;
 };
+
+// One nuclide of one material (indexed like mat_nuc): everything a lookup
+// needs before its first grid access, in one 64-byte record, so a wavefront
+// working on one material reads its nuclides' metadata with one uniform load.
+struct alignas(64) DevMatNuc {
+  int64_t off; // first grid point of the nuclide
+  int n;       // grid points
+  int nuc;     // nuclide index
+  double e_first;
+  double e_last;
+  double dens;  // atom density (material order)
+  double u_lo;  // learned index: log(E_first / E_min) (0 until built)
+  double inv_w; // learned index: n_seg / (log(E_last / E_min) - u_lo)
+};
+static_assert(sizeof(DevMatNuc) == 64, "one 64-byte record per material nuclide");
 
 // Learned grid index: piecewise-linear model of i(log E) per nuclide.
 struct DevLearned {
@@ -114,7 +140,30 @@ struct DevTally {
   int64_t n_bins;
 };
 
+// Mesh weight windows for neutrons (src/weight_windows.cpp:56-146, 375-411)
+struct DevWW {
+  int on;
+  int ckpt_collision;
+  int ckpt_surface;
+  int max_split;
+  int n_dim;
+  int shape[3];
+  double ll[3];
+  double width[3];
+  int n_e; // energy groups (bounds n_e + 1)
+  int pad;
+  int64_t n_mesh;
+  const double* ebounds;
+  const double* lower; // [n_e][n_mesh]
+  const double* upper;
+  double survival_ratio;
+  double max_lb_ratio;
+  double weight_cutoff;
+  int64_t max_history_splits;
+};
+
 struct DevProblem {
+  DevWW ww;
   // cross sections
   const DevNuclide* nuc;
   const double* E;   // concatenated energy grids
@@ -130,6 +179,7 @@ struct DevProblem {
   const int* mat_off; // [n_mat + 1]
   const int* mat_nuc;
   const double* mat_dens;
+  const DevMatNuc* matnuc; // [mat_off[n_mat]]
   int n_materials;
   int max_mat_nuc;
 
@@ -137,6 +187,7 @@ struct DevProblem {
   const DevSurface* surf;
   const DevCell* cell;
   const int* tokens;
+  const DevTok* ctok; // tokens with their surfaces, same indexing as tokens
   const DevUniverse* univ;
   const int* univ_cells;
   const DevLattice* lat;
@@ -218,8 +269,35 @@ struct alignas(64) PLevel {
 };
 static_assert(sizeof(PLevel) == 64, "half a cache line per level");
 
+// Weight-window state of one particle's history
+// (include/openmc/particle_data.h:507-508, 563-564) and its bank count.
+struct alignas(32) PWW {
+  double wgt_born;
+  double wgt_ww_born;
+  double ww_factor;
+  int n_split;
+  int n_bank;
+};
+static_assert(sizeof(PWW) == 32, "32-byte weight-window record");
+
+// A secondary bank site (the SourceSite fields Particle::split sets,
+// src/particle.cpp:95-112).
+struct alignas(16) PSite {
+  double rx, ry, rz;
+  double ux, uy, uz;
+  double E, time, wgt;
+  int surface;
+  int pad;
+};
+static_assert(sizeof(PSite) == 80, "80-byte bank site");
+
 struct DevParticles {
   int64_t cap;
+  PWW* ww;       // [cap] (weight windows on)
+  PSite* bank;   // [cap][bank_cap] LIFO per particle (weight windows on)
+  int bank_cap;
+  int* status;   // [1] bit 0: a secondary bank overflowed
+  double* total_weight; // [CNT_SLOTS] per-batch source weight partial sums
   PHot* hot;      // [cap]
   PLevel* lev;    // [cap][MAX_LEVELS - 1]
   double* micro_t; // [cap][max_mat_nuc]: per-nuclide totals for nuclide sampling

@@ -279,12 +279,13 @@ __device__ __forceinline__ bool surf_sense(const DevSurface& s, V3 r, V3 u)
 __device__ __forceinline__ bool cell_contains(const DevProblem& P, const DevCell& c, V3 r, V3 u, int on_surface)
 {
   for (int k = c.tok_begin; k < c.tok_end; ++k) {
-    int token = P.tokens[k];
+    const DevTok t = P.ctok[k];
+    const int token = t.token;
     if (token == on_surface) {
     } else if (-token == on_surface) {
       return false;
     } else {
-      bool sense = surf_sense(P.surf[abs(token) - 1], r, u);
+      bool sense = surf_sense(t.s, r, u);
       if (sense != (token > 0))
         return false;
     }
@@ -299,9 +300,10 @@ __device__ __forceinline__ double cell_distance(const DevProblem& P, const DevCe
   double min_dist = INFTY;
   int isurf = 2147483647;
   for (int k = c.tok_begin; k < c.tok_end; ++k) {
-    int token = P.tokens[k];
+    const DevTok t = P.ctok[k];
+    const int token = t.token;
     bool coincident = abs(token) == abs(on_surface);
-    double d = surf_distance(P.surf[abs(token) - 1], r, u, coincident);
+    double d = surf_distance(t.s, r, u, coincident);
     if (d < min_dist) {
       if (min_dist - d >= FP_PRECISION * min_dist) {
         min_dist = d;
@@ -729,9 +731,12 @@ __device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, doub
 }
 
 // K lookups of one energy in K nuclides, interleaved: every phase issues the
-// K independent loads before it uses any of them, and the K bounded binary
-// searches run in lockstep, so one lane keeps up to K cache misses in flight
-// instead of one.  Each result equals nuclide_xs for the same nuclide:
+// K independent loads before it uses any of them, and the K bounded searches
+// run in lockstep, so one lane keeps up to K cache misses in flight instead of
+// one.  Each search halves its range only while more than XS_WINDOW points
+// remain and then reads the remaining points in one step, so a bucket of up
+// to XS_WINDOW points costs one dependent load instead of log2 of its size.
+// Each result equals nuclide_xs for the same nuclide:
 //  - the bucket search's special case (E equal to the first point of its
 //    bucket gives that point, include/openmc/search.h:17-18) is applied after
 //    the search: lower_bound over [i_low, i_high) returns i_low exactly when
@@ -739,9 +744,10 @@ __device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, doub
 //    that point only needs reading when the lower bound is i_low;
 //  - the duplicate-point bump reads the next point only when it applies.
 // Entries k >= kk are not computed (their outputs are left unchanged).
-// nuc_list and kk must be the same in every active lane of the wavefront:
-// the nuclide indices, grid offsets and sizes are then held in scalar
-// registers (readfirstlane of values every active lane has loaded alike).
+// mn (the material's nuclide records) and kk must be the same in every
+// active lane of the wavefront: the nuclide indices, grid offsets and sizes
+// are then held in scalar registers (readfirstlane of values every active
+// lane has loaded alike).
 __device__ __forceinline__ int wave_uniform(int x)
 {
   return __builtin_amdgcn_readfirstlane(x);
@@ -760,43 +766,46 @@ __device__ __forceinline__ double wave_uniform(double x)
   return __longlong_as_double(wave_uniform(static_cast<int64_t>(__double_as_longlong(x))));
 }
 
+// Points of the final lower_bound step, loaded together.
+constexpr int XS_WINDOW = 8;
+
 template<int K, bool LEARNED>
-__device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int* nuc_list, int kk, double E,
+__device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevMatNuc* mn, int kk, double E,
   int i_log, double logE, int& probes, Micro* out)
 {
   const double* Eg[K];
   int64_t off[K];
   int n[K], fst[K], cnt[K], lo0[K], ig[K], nuc[K];
+  double e_first[K], e_last[K];
   bool search[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    nuc[k] = wave_uniform(nuc_list[k < kk ? k : 0]);
-    const DevNuclide nd = P.nuc[nuc[k]];
-    off[k] = wave_uniform(nd.off);
-    n[k] = wave_uniform(nd.n);
+    const DevMatNuc* r = mn + (k < kk ? k : 0);
+    nuc[k] = wave_uniform(r->nuc);
+    off[k] = wave_uniform(r->off);
+    n[k] = wave_uniform(r->n);
+    e_first[k] = wave_uniform(r->e_first);
+    e_last[k] = wave_uniform(r->e_last);
     Eg[k] = P.E + off[k];
   }
   // bracket by the log-grid bucket (or the learned window); all K nuclides'
   // loads are issued before any is used
-  double e_first[K], e_last[K];
   int lo[K], hi[K];
+  if (!LEARNED) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    e_first[k] = Eg[k][0];
-    e_last[k] = Eg[k][n[k] - 1];
-    if (!LEARNED) {
+    for (int k = 0; k < K; ++k) {
       const int* gi = P.grid_index + static_cast<int64_t>(nuc[k]) * (P.n_log_bins + 1);
       lo[k] = gi[i_log];
-      hi[k] = gi[i_log + 1];
+      hi[k] = gi[i_log + 1] + 1;
     }
-  }
-  if (LEARNED) {
+  } else {
     const DevLearned& Lm = P.learned;
     double u_lo[K], inv_w[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      u_lo[k] = wave_uniform(Lm.u_lo[nuc[k]]);
-      inv_w[k] = wave_uniform(Lm.inv_w[nuc[k]]);
+      const DevMatNuc* r = mn + (k < kk ? k : 0);
+      u_lo[k] = wave_uniform(r->u_lo);
+      inv_w[k] = wave_uniform(r->inv_w);
     }
     float2 sg[K];
     int err[K];
@@ -823,10 +832,6 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int*
       lo[k] = l;
       hi[k] = h;
     }
-  } else {
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      hi[k] += 1;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -838,21 +843,24 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int*
     fst[k] = lo[k];
     cnt[k] = search[k] ? hi[k] - lo[k] : 0;
   }
-  // lockstep lower_bound searches
+  // lower_bound of E in [fst, fst + cnt): halve the ranges in lockstep
+  // until at most XS_WINDOW points are left, then load those points at once
+  // and count the ones below E (the grid is sorted, so they form a prefix and
+  // the lower bound is fst plus their count)
   while (true) {
     bool any = false;
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      any = any || cnt[k] > 0;
+      any = any || cnt[k] > XS_WINDOW;
     if (!any)
       break;
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      v[k] = cnt[k] > 0 ? Eg[k][fst[k] + (cnt[k] >> 1)] : 0.0;
+      v[k] = cnt[k] > XS_WINDOW ? Eg[k][fst[k] + (cnt[k] >> 1)] : 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (cnt[k] > 0) {
+      if (cnt[k] > XS_WINDOW) {
         ++probes;
         const int step = cnt[k] >> 1;
         if (v[k] < E) {
@@ -862,6 +870,26 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const int*
           cnt[k] = step;
         }
       }
+    }
+  }
+  {
+    double w[K][XS_WINDOW];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int t = 0; t < XS_WINDOW; ++t) {
+        const int i = fst[k] + t;
+        w[k][t] = Eg[k][i < n[k] - 1 ? i : n[k] - 1];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      int c = 0;
+#pragma unroll
+      for (int t = 0; t < XS_WINDOW; ++t)
+        c += (t < cnt[k] && w[k][t] < E) ? 1 : 0;
+      probes += cnt[k] > 0 ? 1 : 0;
+      fst[k] += c;
     }
   }
 #pragma unroll
@@ -944,7 +972,10 @@ __device__ __forceinline__ MeshDist mesh_distance(const DevTally& T, const int* 
   return d;
 }
 
-__device__ __forceinline__ void mesh_indices(const DevTally& T, V3 r, int* ijk, bool& in_mesh)
+// RegularMesh::get_indices / get_bin_from_indices (src/mesh.cpp:874-900,
+// 1328-1331) for any regular mesh description (tally or weight windows)
+template<class M>
+__device__ __forceinline__ void mesh_indices(const M& T, V3 r, int* ijk, bool& in_mesh)
 {
   in_mesh = true;
   for (int i = 0; i < T.n_dim; ++i) {
@@ -954,7 +985,8 @@ __device__ __forceinline__ void mesh_indices(const DevTally& T, V3 r, int* ijk, 
   }
 }
 
-__device__ __forceinline__ int64_t mesh_bin(const DevTally& T, const int* ijk)
+template<class M>
+__device__ __forceinline__ int64_t mesh_bin(const M& T, const int* ijk)
 {
   switch (T.n_dim) {
   case 1: return ijk[0] - 1;
@@ -1064,6 +1096,130 @@ __device__ __forceinline__ void hmix(uint64_t& h, uint64_t v)
 __device__ __forceinline__ uint64_t dbits(double x)
 {
   return static_cast<uint64_t>(__double_as_longlong(x));
+}
+
+//==============================================================================
+// Weight windows (src/weight_windows.cpp:56-146, 375-411;
+// src/physics_common.cpp:12-19; src/particle.cpp:95-112)
+//==============================================================================
+
+struct WeightWindow {
+  double lower;
+  double upper;
+  double max_lb_ratio;
+  double survival;
+  double cutoff;
+  int max_split;
+  __device__ __forceinline__ bool valid() const { return lower >= 0.0; }
+  __device__ __forceinline__ void scale(double f)
+  {
+    lower *= f;
+    upper *= f;
+    survival *= f;
+  }
+};
+
+// WeightWindows::get_weight_window for a neutron at r with energy E
+__device__ __forceinline__ WeightWindow get_weight_window(const DevWW& W, V3 r, double E)
+{
+  WeightWindow ww;
+  ww.lower = -1.0;
+  ww.upper = 1.0;
+  ww.max_lb_ratio = 1.0;
+  ww.survival = 0.5;
+  ww.cutoff = 1.0e-38;
+  ww.max_split = 10;
+  int ijk[3] = {0, 0, 0};
+  bool in_mesh;
+  mesh_indices(W, r, ijk, in_mesh);
+  if (!in_mesh)
+    return ww;
+  if (E < W.ebounds[0] || E > W.ebounds[W.n_e])
+    return ww;
+  // lower_bound_index over the energy bounds (include/openmc/search.h:13-20)
+  int e_bin = 0;
+  if (W.ebounds[0] != E) {
+    int probes = 0;
+    e_bin = lower_bound_idx(W.ebounds, 0, W.n_e + 1, E, probes) - 1;
+  }
+  const int64_t k = static_cast<int64_t>(e_bin) * W.n_mesh + mesh_bin(W, ijk);
+  ww.lower = W.lower[k];
+  ww.upper = W.upper[k];
+  ww.survival = ww.lower * W.survival_ratio;
+  ww.max_lb_ratio = W.max_lb_ratio;
+  ww.max_split = W.max_split;
+  ww.cutoff = W.weight_cutoff;
+  return ww;
+}
+
+// Where a split particle's secondaries go: the next free sites of its LIFO
+// bank (capacity bank_cap; overflow sets bit 0 of *status and the sites
+// beyond the capacity are dropped, which the host reports as an error).
+struct SplitSink {
+  PSite* bank;
+  int cap;
+  int* status;
+};
+
+// apply_weight_windows for a live neutron with weight wgt at (r, u, E, time,
+// surface); may split into the bank or play Russian roulette (tracking seed).
+__device__ __forceinline__ void apply_weight_windows(const DevWW& W, PWW& st, double& wgt, V3 r, V3 u, double E,
+  double time, int surface, uint64_t* seed, const SplitSink& sink)
+{
+  if (E <= 0 || wgt == 0.0)
+    return;
+  WeightWindow ww = get_weight_window(W, r, E);
+  if (st.wgt_ww_born == -1.0)
+    st.wgt_ww_born = ww.valid() ? (ww.lower + ww.upper) / 2 : 1.0;
+  if (!ww.valid())
+    return;
+  ww.scale(st.wgt_born / st.wgt_ww_born);
+  const double weight = wgt;
+  if (wgt < ww.cutoff) {
+    wgt = 0.0;
+    return;
+  }
+  if (st.ww_factor == 0.0 && ww.max_lb_ratio > 1.0 && wgt > ww.lower * ww.max_lb_ratio)
+    st.ww_factor = wgt / (ww.lower * ww.max_lb_ratio);
+  if (st.ww_factor > 1.0)
+    ww.scale(st.ww_factor);
+  if (weight > ww.upper) {
+    if (st.n_split >= W.max_history_splits)
+      return;
+    double n_split = ceil(weight / ww.upper);
+    const double max_split = ww.max_split;
+    n_split = fmin(n_split, max_split);
+    st.n_split = static_cast<int>(st.n_split + n_split);
+    const int i_split = static_cast<int>(round(n_split));
+    const double w_each = weight / n_split;
+    for (int l = 0; l < i_split - 1; l++) {
+      if (st.n_bank >= sink.cap) {
+        atomicOr(sink.status, 1);
+        continue;
+      }
+      PSite s;
+      s.rx = r.x;
+      s.ry = r.y;
+      s.rz = r.z;
+      s.ux = u.x;
+      s.uy = u.y;
+      s.uz = u.z;
+      s.E = E;
+      s.time = time;
+      s.wgt = w_each;
+      s.surface = surface;
+      s.pad = 0;
+      sink.bank[st.n_bank++] = s;
+    }
+    wgt = weight / n_split;
+  } else if (weight <= ww.lower) {
+    // russian_roulette
+    const double weight_survive = fmin(weight * ww.max_split, ww.survival);
+    if (weight_survive * prn(seed) < wgt)
+      wgt = weight_survive;
+    else
+      wgt = 0.0;
+  }
 }
 
 } // namespace omc

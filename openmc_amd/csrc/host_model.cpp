@@ -149,8 +149,112 @@ void init_alias(const std::vector<double>& p_in, std::vector<double>& prob, std:
   }
 }
 
+// <mesh type="regular"> (src/mesh.cpp: RegularMesh XML constructor)
+HostMesh parse_regular_mesh(const XmlNode& mn)
+{
+  if (mn.attr("type") && *mn.attr("type") != "regular")
+    throw std::runtime_error("only regular meshes are supported by the GPU path");
+  HostMesh me;
+  auto dim = to_ints(mn.value("dimension"));
+  me.n_dim = static_cast<int>(dim.size());
+  if (me.n_dim < 1 || me.n_dim > 3)
+    throw std::runtime_error("mesh must be one, two or three dimensional");
+  auto ll = to_doubles(mn.value("lower_left"));
+  std::vector<double> ur;
+  if (mn.has("upper_right")) {
+    ur = to_doubles(mn.value("upper_right"));
+  } else {
+    auto w = to_doubles(mn.value("width"));
+    for (int k = 0; k < me.n_dim; ++k)
+      ur.push_back(ll[k] + static_cast<double>(dim[k]) * w[k]);
+  }
+  if (static_cast<int>(ll.size()) != me.n_dim || static_cast<int>(ur.size()) != me.n_dim)
+    throw std::runtime_error("mesh corners do not match its dimension");
+  for (int k = 0; k < 3; ++k) {
+    me.shape[k] = k < me.n_dim ? static_cast<int>(dim[k]) : 1;
+    me.ll[k] = k < me.n_dim ? ll[k] : 0.0;
+    me.ur[k] = k < me.n_dim ? ur[k] : 0.0;
+    // RegularMesh::set_grid: width = (upper_right - lower_left) / shape
+    me.width[k] = k < me.n_dim ? (me.ur[k] - me.ll[k]) / static_cast<double>(me.shape[k]) : 0.0;
+  }
+  return me;
+}
+
+bool to_bool(const std::string& v)
+{
+  return v == "true" || v == "1" || v == "True";
+}
+
+// <weight_windows>, <weight_windows_on>, <weight_window_checkpoints>,
+// <max_history_splits> (src/settings.cpp:1203-1260, src/weight_windows.cpp:165-229)
+void read_weight_windows(HostModel& m, const XmlNode& s)
+{
+  HostWW& w = m.ww;
+  auto wws = s.children_named("weight_windows");
+  std::unordered_map<int, HostMesh> meshes;
+  for (const XmlNode* mn : s.children_named("mesh"))
+    meshes[std::stoi(mn->value("id"))] = parse_regular_mesh(*mn);
+  const XmlNode* sel = nullptr;
+  for (const XmlNode* n : wws) {
+    if (n->value("particle_type") != "neutron")
+      continue;
+    if (sel)
+      throw std::runtime_error("the GPU path supports one set of neutron weight windows");
+    sel = n;
+  }
+  w.on = sel != nullptr;
+  if (sel) {
+    auto mt = meshes.find(std::stoi(sel->value("mesh")));
+    if (mt == meshes.end())
+      throw std::runtime_error("weight windows refer to an unknown mesh");
+    w.mesh = mt->second;
+    if (sel->has("energy_bounds"))
+      w.ebounds = to_doubles(sel->value("energy_bounds"));
+    else
+      w.ebounds = {0.0, INFTY};
+    if (w.ebounds.size() < 2)
+      throw std::runtime_error("weight windows need at least two energy bounds");
+    if (sel->has("survival_ratio")) {
+      w.survival_ratio = std::stod(sel->value("survival_ratio"));
+      if (w.survival_ratio <= 1)
+        throw std::runtime_error("Survival to lower weight window ratio must bigger than 1");
+    }
+    if (sel->has("max_lower_bound_ratio")) {
+      w.max_lb_ratio = std::stod(sel->value("max_lower_bound_ratio"));
+      if (w.max_lb_ratio < 1.0)
+        throw std::runtime_error("Maximum lower bound ratio must be larger than 1");
+    }
+    if (sel->has("max_split")) {
+      w.max_split = static_cast<int>(std::stod(sel->value("max_split")));
+      if (w.max_split <= 1)
+        throw std::runtime_error("max split must be larger than 1");
+    }
+    if (sel->has("weight_cutoff")) {
+      w.weight_cutoff = std::stod(sel->value("weight_cutoff"));
+      if (w.weight_cutoff <= 0 || w.weight_cutoff > 1)
+        throw std::runtime_error("weight_cutoff must be in (0, 1]");
+    }
+    w.lower = to_doubles(sel->value("lower_ww_bounds"));
+    w.upper = to_doubles(sel->value("upper_ww_bounds"));
+    const size_t want = static_cast<size_t>(w.mesh.n_bins()) * (w.ebounds.size() - 1);
+    if (w.lower.size() != want || w.upper.size() != want)
+      throw std::runtime_error("weight window bounds do not match the mesh and energy bounds");
+  }
+  if (s.has("weight_windows_on"))
+    w.on = w.on && to_bool(s.value("weight_windows_on"));
+  if (const XmlNode* ck = s.child("weight_window_checkpoints")) {
+    if (ck->has("collision"))
+      w.ckpt_collision = to_bool(ck->value("collision"));
+    if (ck->has("surface"))
+      w.ckpt_surface = to_bool(ck->value("surface"));
+  }
+  if (s.has("max_history_splits"))
+    w.max_history_splits = std::stoll(s.value("max_history_splits"));
+}
+
 void read_settings(HostModel& m, const XmlNode& s)
 {
+  read_weight_windows(m, s);
   if (s.has("run_mode")) {
     m.run_mode = s.value("run_mode");
     if (m.run_mode != "fixed source")
@@ -623,38 +727,9 @@ void read_geometry(HostModel& m, const XmlNode& g)
 
 void read_tallies(HostModel& m, const XmlNode& t)
 {
-  struct Mesh {
-    int n_dim;
-    int shape[3];
-    double ll[3], ur[3], width[3];
-  };
-  std::unordered_map<int, Mesh> meshes;
-  for (const XmlNode* mn : t.children_named("mesh")) {
-    if (mn->attr("type") && *mn->attr("type") != "regular")
-      throw std::runtime_error("only regular meshes are supported by the GPU path");
-    Mesh me {};
-    auto dim = to_ints(mn->value("dimension"));
-    me.n_dim = static_cast<int>(dim.size());
-    if (me.n_dim < 1 || me.n_dim > 3)
-      throw std::runtime_error("mesh must be one, two or three dimensional");
-    auto ll = to_doubles(mn->value("lower_left"));
-    std::vector<double> ur;
-    if (mn->has("upper_right")) {
-      ur = to_doubles(mn->value("upper_right"));
-    } else {
-      auto w = to_doubles(mn->value("width"));
-      for (int k = 0; k < me.n_dim; ++k)
-        ur.push_back(ll[k] + static_cast<double>(dim[k]) * w[k]);
-    }
-    for (int k = 0; k < 3; ++k) {
-      me.shape[k] = k < me.n_dim ? static_cast<int>(dim[k]) : 1;
-      me.ll[k] = k < me.n_dim ? ll[k] : 0.0;
-      me.ur[k] = k < me.n_dim ? ur[k] : 0.0;
-      // RegularMesh::set_grid: width = (upper_right - lower_left) / shape
-      me.width[k] = k < me.n_dim ? (me.ur[k] - me.ll[k]) / static_cast<double>(me.shape[k]) : 0.0;
-    }
-    meshes[std::stoi(mn->value("id"))] = me;
-  }
+  std::unordered_map<int, HostMesh> meshes;
+  for (const XmlNode* mn : t.children_named("mesh"))
+    meshes[std::stoi(mn->value("id"))] = parse_regular_mesh(*mn);
   struct Filt {
     std::string type;
     std::vector<double> bins;
@@ -705,7 +780,7 @@ void read_tallies(HostModel& m, const XmlNode& t)
           auto mt = meshes.find(f.mesh);
           if (mt == meshes.end())
             throw std::runtime_error("mesh filter refers to unknown mesh");
-          const Mesh& me = mt->second;
+          const HostMesh& me = mt->second;
           ht.has_mesh = true;
           ht.mesh_id = f.mesh;
           ht.n_dim = me.n_dim;

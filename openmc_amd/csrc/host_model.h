@@ -36,7 +36,32 @@ struct HostTally {
   std::string estimator = "tracklength";
 };
 
+// A RegularMesh (src/mesh.cpp: RegularMesh::set_grid)
+struct HostMesh {
+  int n_dim = 3;
+  int shape[3] = {1, 1, 1};
+  double ll[3] = {0, 0, 0}, ur[3] = {0, 0, 0}, width[3] = {0, 0, 0};
+  int64_t n_bins() const { return static_cast<int64_t>(shape[0]) * shape[1] * shape[2]; }
+};
+
+// Mesh weight windows for neutrons (src/weight_windows.cpp:165-229 reads the
+// element; src/settings.cpp:1203-1260 the switches)
+struct HostWW {
+  bool on = false;
+  HostMesh mesh;
+  std::vector<double> ebounds;      // energy bounds (default [0, inf))
+  std::vector<double> lower, upper; // [n_energy][n_mesh_bins]
+  double survival_ratio = 3.0;
+  double max_lb_ratio = 1.0;
+  int max_split = 10;
+  double weight_cutoff = 1.0e-38;
+  bool ckpt_collision = true;
+  bool ckpt_surface = false;
+  int64_t max_history_splits = 10000000;
+};
+
 struct HostModel {
+  HostWW ww;
   // settings
   int64_t n_particles = 0;
   int n_batches = 0;

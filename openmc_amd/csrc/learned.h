@@ -20,6 +20,8 @@ struct Learned {
   double loss_trained = 0.0; // after the last gradient step
   std::vector<float> h_seg; // [n_nuc][n_seg][2]
   std::vector<int> h_err;   // [n_nuc][n_seg]
+  std::vector<double> h_ulo;  // [n_nuc] log(E_first / E_min)
+  std::vector<double> h_invw; // [n_nuc] n_seg / (log(E_last / E_min) - u_lo)
 };
 
 // Fit all nuclides' models (forward/backward/Adam kernels, `epochs` steps),

@@ -341,6 +341,8 @@ int build_learned_index(Learned& L, const HostModel& m, DevProblem& P, int n_seg
   for (int v : L.h_err)
     L.max_err = std::max(L.max_err, v);
   L.n_seg = n_seg;
+  L.h_ulo = u_lo;
+  L.h_invw = inv_w;
   P.learned.seg = reinterpret_cast<const float*>(params);
   P.learned.err = err;
   P.learned.u_lo = d_ulo;

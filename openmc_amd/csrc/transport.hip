@@ -210,6 +210,7 @@ template<int NL>
 __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, int64_t n, int64_t id_base)
 {
   unsigned long long n_lost = 0;
+  double total_w = 0.0; // source weight after the birth weight windows
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t id = id_base + i + 1;
@@ -290,11 +291,30 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
     h.uy = u.y;
     h.uz = u.z;
     h.E = E;
-    h.wgt = lost ? 0.0 : 1.0;
     h.xs_t = 0.0;
     h.xs_a = 0.0;
     h.time = 0.0;
     h.seed = future_seed(static_cast<uint64_t>(id) * P.stride, P.master_seed + 0);
+    // initialize_history: the birth weight window (src/simulation.cpp:599-639)
+    double wgt = 1.0;
+    uint64_t hh = 0xcbf29ce484222325ULL ^ static_cast<uint64_t>(id);
+    if (P.ww.on) {
+      PWW st;
+      st.wgt_born = wgt;
+      st.wgt_ww_born = -1.0;
+      st.ww_factor = 0.0;
+      st.n_split = 0;
+      st.n_bank = 0;
+      const SplitSink sink {S.bank + i * S.bank_cap, S.bank_cap, S.status};
+      apply_weight_windows(P.ww, st, wgt, r, u, E, 0.0, SURFACE_NONE, &h.seed, sink);
+      S.ww[i] = st;
+      if (S.trace_hash) {
+        hmix(hh, dbits(wgt));
+        hmix(hh, static_cast<uint64_t>(st.n_bank));
+      }
+    }
+    total_w += wgt;
+    h.wgt = lost ? 0.0 : wgt;
     h.cell0 = g.cell[0];
     h.material = g.material;
     h.material_last = g.material_last;
@@ -307,11 +327,15 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
     store_geo<NL>(S, i, g);
     S.lost[i] = lost ? 1 : 0;
     n_lost += lost ? 1 : 0;
-    S.hash[i] = 0xcbf29ce484222325ULL ^ static_cast<uint64_t>(id);
+    S.hash[i] = hh;
     S.stage_p[i] = static_cast<int>(i);
-    S.stage_key[i] = lost ? -1 : sort_key(S, g.material, E);
+    S.stage_key[i] = h.wgt == 0.0 ? -1 : sort_key(S, g.material, E);
   }
   wave_count(S, CNT_LOST, n_lost);
+  for (int o = 32; o > 0; o >>= 1)
+    total_w += __shfl_xor(total_w, o);
+  if (lane_id() == 0 && total_w != 0.0)
+    unsafeAtomicAdd(&S.total_weight[wave_slot()], total_w);
 }
 
 //==============================================================================
@@ -602,12 +626,12 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
   for (int jj = 0; jj < nn; jj += G) {
     const int kk = nn - jj < G ? nn - jj : G;
     Micro m[G];
-    nuclide_xs_group<G, LEARNED>(P, P.mat_nuc + j0 + jj, kk, E, i_log, lx, probes, m);
+    nuclide_xs_group<G, LEARNED>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
         micro[jj + k] = m[k].total;
-        const double dens = wave_uniform(P.mat_dens[j0 + jj + k]) * 1.0;
+        const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
         if (trace)
@@ -770,7 +794,45 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S,
 // surface-crossing event (src/particle.cpp:288-329, 549-710)
 //==============================================================================
 
+// Particle::event_revive_from_secondary (src/particle.cpp:417-473) for a dead
+// particle with banked secondaries: it continues as the last one it banked
+// (Particle::from_source, src/particle.cpp:114-158) with the same RNG streams
+// and n_event = 0.  The cell search of its next calculate_xs event is done
+// here (it uses only r, u and the surface and draws no random number).
+// Writes the particle's records; returns its new weight (0 if lost).
 template<int NL>
+__device__ __forceinline__ double revive_from_bank(const DevProblem& P, const DevParticles& S, int p, PWW& st,
+  int& material, double& E, unsigned long long& n_lost)
+{
+  const PSite s = S.bank[static_cast<int64_t>(p) * S.bank_cap + (--st.n_bank)];
+  Geo<NL> g;
+  geo_clear(g, V3 {s.rx, s.ry, s.rz}, V3 {s.ux, s.uy, s.uz});
+  g.surface = s.surface;
+  double wgt = s.wgt;
+  if (!exhaustive_find_cell(P, g) || g.lost) {
+    wgt = 0.0;
+    S.lost[p] = 1;
+    n_lost++;
+  }
+  store_geo<NL>(S, p, g);
+  PHot* hp = S.hot + p;
+  hp->E = s.E;
+  hp->wgt = wgt;
+  hp->time = s.time;
+  hp->n_event = 0;
+  if (S.trace_hash) {
+    uint64_t hh = S.hash[p];
+    hmix(hh, 0x66);
+    hmix(hh, dbits(s.wgt));
+    hmix(hh, dbits(s.E));
+    S.hash[p] = hh;
+  }
+  material = g.material;
+  E = s.E;
+  return wgt;
+}
+
+template<int NL, bool WW>
 __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_SURF];
@@ -848,9 +910,24 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
         S.lost[p] = 1;
         n_lost++;
       }
+      PWW st;
+      if (WW) {
+        st = S.ww[p];
+        // Particle::event_cross_surface: weight windows after a surface
+        // (not lattice) crossing when that checkpoint is on (src/particle.cpp:320-322)
+        const bool lattice = trans[0] != 0 || trans[1] != 0 || trans[2] != 0;
+        if (P.ww.ckpt_surface && !lattice) {
+          uint64_t seed = h.seed;
+          const SplitSink sink {S.bank + static_cast<int64_t>(p) * S.bank_cap, S.bank_cap, S.status};
+          apply_weight_windows(P.ww, st, wgt, g.r(0), g.u(), h.E, h.time, g.surface, &seed, sink);
+          S.hot[p].seed = seed;
+        }
+      }
       if (trace) {
         hmix(hh, dbits(wgt));
         hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? lowest_cell(g) : -1)));
+        if (WW)
+          hmix(hh, static_cast<uint64_t>(st.n_bank));
         S.hash[p] = hh;
       }
       // event_revive_from_secondary (src/particle.cpp:417-473)
@@ -860,8 +937,15 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
       store_geo<NL>(S, p, g);
       S.hot[p].n_event = n_event;
       S.hot[p].wgt = wgt;
+      int mat = g.material;
+      double E = h.E;
+      if (WW) {
+        if (wgt == 0.0 && st.n_bank > 0)
+          wgt = revive_from_bank<NL>(P, S, p, st, mat, E, n_lost);
+        S.ww[p] = st;
+      }
       push = wgt != 0.0;
-      key = sort_key(S, g.material, h.E);
+      key = sort_key(S, mat, E);
     }
     S.stage_p[i] = p;
     S.stage_key[i] = push ? key : -1;
@@ -907,7 +991,7 @@ __device__ __forceinline__ V3 sample_cxs_target_velocity(double awr, double E, V
   return v_scale(rotate_angle(u, mu, seed), vt);
 }
 
-template<int NL>
+template<int NL, bool WW>
 __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_COLL];
@@ -987,6 +1071,15 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
           u = v_div(v_n, vel);
         }
       }
+      PWW st;
+      if (WW) {
+        st = S.ww[p];
+        // collision(): weight windows after the reaction (src/physics.cpp:64-65)
+        if (P.ww.ckpt_collision) {
+          const SplitSink sink {S.bank + static_cast<int64_t>(p) * S.bank_cap, S.bank_cap, S.status};
+          apply_weight_windows(P.ww, st, wgt, V3 {h.rx, h.ry, h.rz}, u, E, h.time, SURFACE_NONE, &seed, sink);
+        }
+      }
       if (S.trace_hash) {
         uint64_t hh = S.hash[p];
         hmix(hh, 0x44);
@@ -996,6 +1089,10 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
         hmix(hh, dbits(u.x));
         hmix(hh, dbits(u.y));
         hmix(hh, dbits(u.z));
+        if (WW) {
+          hmix(hh, dbits(wgt));
+          hmix(hh, static_cast<uint64_t>(st.n_bank));
+        }
         S.hash[p] = hh;
       }
       if (E < P.energy_cutoff)
@@ -1015,8 +1112,14 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
       hp->E = E;
       hp->wgt = wgt;
       hp->seed = seed;
+      int mat = material;
+      if (WW) {
+        if (wgt == 0.0 && st.n_bank > 0)
+          wgt = revive_from_bank<NL>(P, S, p, st, mat, E, n_lost);
+        S.ww[p] = st;
+      }
       push = wgt != 0.0;
-      key = sort_key(S, material, E);
+      key = sort_key(S, mat, E);
     }
     S.stage_p[n_surf + i] = p;
     S.stage_key[n_surf + i] = push ? key : -1;
@@ -1132,8 +1235,18 @@ void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const Laun
     launch_partition(S, A, lc, st);
     break;
   }
-  case KERNEL_SURFACE: hipLaunchKernelGGL(k_surface<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S); break;
-  default: hipLaunchKernelGGL(k_collision<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S); break;
+  case KERNEL_SURFACE:
+    if (P.ww.on)
+      hipLaunchKernelGGL((k_surface<NL, true>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+    else
+      hipLaunchKernelGGL((k_surface<NL, false>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+    break;
+  default:
+    if (P.ww.on)
+      hipLaunchKernelGGL((k_collision<NL, true>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+    else
+      hipLaunchKernelGGL((k_collision<NL, false>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+    break;
   }
 }
 

@@ -366,6 +366,10 @@ class Settings:
         self.log_grid_bins = None
         self.max_particle_events = None
         self.cross_sections = None
+        self.weight_windows = []
+        self.weight_windows_on = None
+        self.weight_window_checkpoints = {}
+        self.max_history_splits = None
 
     def to_xml_element(self):
         el = ET.Element("settings")
@@ -387,6 +391,67 @@ class Settings:
             ET.SubElement(el, "log_grid_bins").text = str(int(self.log_grid_bins))
         if self.max_particle_events is not None:
             ET.SubElement(el, "max_particle_events").text = str(int(self.max_particle_events))
+        # weight windows and their meshes (openmc/settings.py:1797-1862)
+        written = set()
+        for ww in self.weight_windows:
+            el.append(ww.to_xml_element())
+            if ww.mesh.id not in written:
+                el.append(ww.mesh.to_xml_element())
+                written.add(ww.mesh.id)
+        if self.weight_windows_on is not None:
+            ET.SubElement(el, "weight_windows_on").text = str(bool(self.weight_windows_on)).lower()
+        if self.weight_window_checkpoints:
+            ck = ET.SubElement(el, "weight_window_checkpoints")
+            for key in ("collision", "surface"):
+                if key in self.weight_window_checkpoints:
+                    ET.SubElement(ck, key).text = str(bool(self.weight_window_checkpoints[key])).lower()
+        if self.max_history_splits is not None:
+            ET.SubElement(el, "max_history_splits").text = str(int(self.max_history_splits))
+        return el
+
+
+class WeightWindows:
+    """Mesh-based weight windows (openmc/weight_windows.py:316-357 writes the
+    same element).  lower/upper bounds have shape mesh.dimension + (n_e,) and
+    are written in Fortran order, i.e. flat index e * n_mesh + mesh_bin."""
+
+    def __init__(self, mesh, lower_ww_bounds, upper_ww_bounds=None, energy_bounds=None,
+                 upper_bound_ratio=None, particle_type="neutron", survival_ratio=3.0,
+                 max_lower_bound_ratio=None, max_split=10, weight_cutoff=1.0e-38, ww_id=None):
+        self.id = ww_id if ww_id is not None else _next_id("weight_windows")
+        self.mesh = mesh
+        self.energy_bounds = None if energy_bounds is None else np.asarray(energy_bounds, dtype=float)
+        n_e = 1 if self.energy_bounds is None else self.energy_bounds.size - 1
+        shape = tuple(int(d) for d in mesh.dimension) + (n_e,)
+        self.lower_ww_bounds = np.asarray(lower_ww_bounds, dtype=float).reshape(shape)
+        if upper_ww_bounds is None:
+            upper_ww_bounds = self.lower_ww_bounds * float(upper_bound_ratio)
+        self.upper_ww_bounds = np.asarray(upper_ww_bounds, dtype=float).reshape(shape)
+        self.particle_type = particle_type
+        self.survival_ratio = survival_ratio
+        self.max_lower_bound_ratio = max_lower_bound_ratio
+        self.max_split = max_split
+        self.weight_cutoff = weight_cutoff
+
+    def flat_bounds(self):
+        """(lower, upper) flattened as written: mesh bin fastest, energy slowest."""
+        return self.lower_ww_bounds.ravel("F"), self.upper_ww_bounds.ravel("F")
+
+    def to_xml_element(self):
+        el = ET.Element("weight_windows")
+        el.set("id", str(self.id))
+        ET.SubElement(el, "mesh").text = str(self.mesh.id)
+        ET.SubElement(el, "particle_type").text = self.particle_type
+        if self.energy_bounds is not None:
+            ET.SubElement(el, "energy_bounds").text = _fmt(self.energy_bounds)
+        lo, hi = self.flat_bounds()
+        ET.SubElement(el, "lower_ww_bounds").text = _fmt(lo)
+        ET.SubElement(el, "upper_ww_bounds").text = _fmt(hi)
+        ET.SubElement(el, "survival_ratio").text = repr(float(self.survival_ratio))
+        if self.max_lower_bound_ratio is not None:
+            ET.SubElement(el, "max_lower_bound_ratio").text = repr(float(self.max_lower_bound_ratio))
+        ET.SubElement(el, "max_split").text = str(int(self.max_split))
+        ET.SubElement(el, "weight_cutoff").text = repr(float(self.weight_cutoff))
         return el
 
 

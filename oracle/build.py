@@ -1,6 +1,6 @@
 """Build recipe of the oracle (test infrastructure): oracle/_build/liboracle.so.
 
-Compiled with g++ -O2 -ffp-contract=off (no -march=native, no fast-math),
+Compiled with g++ -O3 -ffp-contract=off (no -march=native, no fast-math),
 OpenMP for the CPU baseline timings.  A second build with glibc's log/sin/cos
 (-DORACLE_GLIBC_MATH) measures how often the deterministic libm and glibc
 lead to different histories.
@@ -17,7 +17,7 @@ LIB_GLIBC = os.path.join(OUT, "liboracle_glibc.so")
 
 def _cc(out, extra):
     os.makedirs(OUT, exist_ok=True)
-    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+    cmd = ["g++", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
            "-shared", "-I" + os.path.join(ROOT, "include"), os.path.join(HERE, "oracle.cpp"),
            "-o", out + ".tmp"] + extra
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -549,6 +549,20 @@ struct Problem {
   std::vector<double> energy_bins; // empty -> no energy filter
   int64_t n_tally_bins = 0;
 
+  // mesh weight windows for neutrons (src/weight_windows.cpp), one set
+  bool ww_on = false;
+  Mesh ww_mesh;
+  std::vector<double> ww_ebounds;
+  std::vector<double> ww_lower, ww_upper; // [n_energy][n_mesh_bins]
+  int64_t ww_n_mesh = 0;
+  double ww_survival_ratio = 3.0;
+  double ww_max_lb_ratio = 1.0;
+  int ww_max_split = 10;
+  double ww_weight_cutoff = 1.0e-38;
+  bool ww_ckpt_collision = true; // src/settings.cpp:86-87
+  bool ww_ckpt_surface = false;
+  int64_t max_history_splits = 10000000; // src/settings.cpp:121
+
   double energy_min = 0, energy_max = 0, log_spacing = 0;
 };
 
@@ -587,7 +601,19 @@ struct Counters {
   double tracklength = 0.0; // sum of wgt * distance over all advances
 };
 
+// A secondary bank site (SourceSite fields set by Particle::split,
+// src/particle.cpp:95-112)
+struct BankSite {
+  Pos r, u;
+  double E = 0, time = 0, wgt = 0;
+  int surface = SURFACE_NONE; // signed surface index + 1
+};
+
 struct Particle {
+  // weight-window state (include/openmc/particle_data.h:507-508, 563-564)
+  double wgt_born = 1.0, wgt_ww_born = -1.0, ww_factor = 0.0;
+  int n_split = 0;
+  std::vector<BankSite> bank; // secondary bank, LIFO
   std::vector<LocalCoord> coord;
   std::vector<int> cell_last;
   int n_coord = 1;
@@ -1104,6 +1130,115 @@ void score_tracklength_tally(Sim& S, Particle& p, double distance, Counters& cnt
 // Source -- src/source.cpp:177-417, 627-660
 //==============================================================================
 
+//==============================================================================
+// Weight windows -- src/weight_windows.cpp:56-146, 375-411;
+// src/physics_common.cpp:12-19; src/particle.cpp:95-112
+//==============================================================================
+
+struct WeightWindow {
+  double lower = -1.0; // -1: invalid
+  double upper = 1.0;
+  double max_lb_ratio = 1.0;
+  double survival = 0.5;
+  double cutoff = 1.0e-38;
+  int max_split = 10;
+  bool valid() const { return lower >= 0.0; }
+  void scale(double f)
+  {
+    lower *= f;
+    upper *= f;
+    survival *= f;
+  }
+};
+
+// lower_bound_index (include/openmc/search.h:13-20)
+inline int lower_bound_index(const std::vector<double>& v, double x)
+{
+  if (v[0] == x)
+    return 0;
+  return static_cast<int>(std::lower_bound(v.begin(), v.end(), x) - v.begin()) - 1;
+}
+
+WeightWindow get_weight_window(const Problem& P, Particle& p)
+{
+  WeightWindow ww;
+  int ijk[3] = {0, 0, 0};
+  bool in_mesh;
+  mesh_indices(P.ww_mesh, p.r(), ijk, in_mesh);
+  if (!in_mesh)
+    return ww;
+  const double E = p.E;
+  if (E < P.ww_ebounds.front() || E > P.ww_ebounds.back())
+    return ww;
+  const int e_bin = lower_bound_index(P.ww_ebounds, E);
+  const int64_t k = static_cast<int64_t>(e_bin) * P.ww_n_mesh + mesh_bin(P.ww_mesh, ijk);
+  ww.lower = P.ww_lower[k];
+  ww.upper = P.ww_upper[k];
+  ww.survival = ww.lower * P.ww_survival_ratio;
+  ww.max_lb_ratio = P.ww_max_lb_ratio;
+  ww.max_split = P.ww_max_split;
+  ww.cutoff = P.ww_weight_cutoff;
+  return ww;
+}
+
+void russian_roulette(Particle& p, double weight_survive)
+{
+  if (weight_survive * prn(p.current_seed()) < p.wgt)
+    p.wgt = weight_survive;
+  else
+    p.wgt = 0.0;
+}
+
+void split(Particle& p, double wgt)
+{
+  BankSite s;
+  s.r = p.r();
+  s.u = p.u();
+  s.E = p.E;
+  s.time = p.time;
+  s.wgt = wgt;
+  s.surface = p.surface;
+  p.bank.push_back(s);
+}
+
+void apply_weight_windows(const Problem& P, Particle& p)
+{
+  if (!P.ww_on)
+    return;
+  if (p.E <= 0 || !p.alive())
+    return;
+  WeightWindow ww = get_weight_window(P, p);
+  if (p.wgt_ww_born == -1.0)
+    p.wgt_ww_born = ww.valid() ? (ww.lower + ww.upper) / 2 : 1.0;
+  if (!ww.valid())
+    return;
+  ww.scale(p.wgt_born / p.wgt_ww_born);
+  const double weight = p.wgt;
+  if (p.wgt < ww.cutoff) {
+    p.wgt = 0.0;
+    return;
+  }
+  if (p.ww_factor == 0.0 && ww.max_lb_ratio > 1.0 && p.wgt > ww.lower * ww.max_lb_ratio)
+    p.ww_factor = p.wgt / (ww.lower * ww.max_lb_ratio);
+  if (p.ww_factor > 1.0)
+    ww.scale(p.ww_factor);
+  if (weight > ww.upper) {
+    if (p.n_split >= P.max_history_splits)
+      return;
+    double n_split = std::ceil(weight / ww.upper);
+    double max_split = ww.max_split;
+    n_split = std::min(n_split, max_split);
+    p.n_split = static_cast<int>(p.n_split + n_split);
+    const int i_split = static_cast<int>(std::round(n_split));
+    for (int l = 0; l < i_split - 1; l++)
+      split(p, weight / n_split);
+    p.wgt = weight / n_split;
+  } else if (weight <= ww.lower) {
+    const double weight_survive = std::min(weight * ww.max_split, ww.survival);
+    russian_roulette(p, weight_survive);
+  }
+}
+
 struct SourceSite {
   Pos r, u;
   double E = 0, time = 0, wgt = 1.0;
@@ -1211,14 +1346,25 @@ void initialize_history(Sim& S, Particle& p, int64_t index_source)
   p.n_event = 0;
   p.leakage = 0;
   p.lost = false;
+  p.n_split = 0;
+  p.ww_factor = 0.0;
+  p.wgt_born = p.wgt;
+  p.bank.clear();
   for (int i = 0; i < N_STREAMS; i++)
     p.seeds[i] = future_seed(static_cast<uint64_t>(id) * P.rng.stride, P.rng.master_seed + i);
   p.stream = STREAM_TRACKING;
+  // the particle's initial weight window (src/simulation.cpp:628-630)
+  p.wgt_ww_born = -1.0;
+  apply_weight_windows(P, p);
 #pragma omp atomic
   S.total_weight += p.wgt;
   // invalidate_neutron_xs
   p.micro.assign(P.nuclides.size(), MicroXS {0, 0, 0, 0, 0, 0.0});
   p.hash = 0xcbf29ce484222325ULL ^ static_cast<uint64_t>(id);
+  if (S.trace_hash && P.ww_on) {
+    mix(p.hash, as_bits(p.wgt));
+    mix(p.hash, static_cast<uint64_t>(p.bank.size()));
+  }
 }
 
 // Particle::event_calculate_xs (src/particle.cpp:160-230)
@@ -1324,12 +1470,39 @@ void cross_surface(const Problem& P, Particle& p, const Surface& surf)
   }
 }
 
-// Particle::event_revive_from_secondary (src/particle.cpp:417-473), no secondaries
-void event_revive(const Problem& P, Particle& p)
+// Particle::event_revive_from_secondary (src/particle.cpp:417-473): a dead
+// particle continues as the last secondary it banked (Particle::from_source,
+// src/particle.cpp:114-158); the RNG streams carry on.
+void event_revive(const Sim& S, Particle& p)
 {
+  const Problem& P = S.P;
   ++p.n_event;
   if (p.n_event == P.max_events)
     p.wgt = 0.0;
+  if (p.alive() || p.bank.empty())
+    return;
+  const BankSite s = p.bank.back();
+  p.bank.pop_back();
+  for (auto& c : p.coord)
+    c.reset();
+  p.n_coord = 1;
+  p.surface = s.surface;
+  p.material = C_NONE;
+  p.n_collision = 0;
+  p.wgt = s.wgt;
+  p.wgt_last = s.wgt;
+  p.coord[0].r = s.r;
+  p.coord[0].u = s.u;
+  p.r_last = s.r;
+  p.E = s.E;
+  p.E_last = s.E;
+  p.time = s.time;
+  p.n_event = 0;
+  if (S.trace_hash) {
+    mix(p.hash, 0x66);
+    mix(p.hash, as_bits(p.wgt));
+    mix(p.hash, as_bits(p.E));
+  }
 }
 
 // Particle::event_cross_surface (src/particle.cpp:288-329)
@@ -1353,10 +1526,14 @@ void event_cross_surface(Sim& S, Particle& p, Counters& cnt)
   } else {
     const Surface& surf = P.surfaces[std::abs(p.surface) - 1];
     cross_surface(P, p, surf);
+    if (P.ww_ckpt_surface)
+      apply_weight_windows(P, p);
   }
   if (S.trace_hash) {
     mix(p.hash, as_bits(p.wgt));
     mix(p.hash, static_cast<uint64_t>(static_cast<int64_t>(p.alive() ? p.lowest().cell : -1)));
+    if (P.ww_on)
+      mix(p.hash, static_cast<uint64_t>(p.bank.size()));
   }
 }
 
@@ -1468,6 +1645,9 @@ void event_collide(Sim& S, Particle& p, Counters& cnt)
     double kT = P.nuclides[i_nuclide].kT;
     elastic_scatter(P, i_nuclide, kT, p);
   }
+  // collision(): weight windows after the reaction (src/physics.cpp:64-65)
+  if (P.ww_ckpt_collision)
+    apply_weight_windows(P, p);
   if (S.trace_hash) {
     mix(p.hash, 0x44);
     mix(p.hash, static_cast<uint64_t>(i_nuclide));
@@ -1476,6 +1656,10 @@ void event_collide(Sim& S, Particle& p, Counters& cnt)
     mix(p.hash, as_bits(p.coord[0].u.x));
     mix(p.hash, as_bits(p.coord[0].u.y));
     mix(p.hash, as_bits(p.coord[0].u.z));
+    if (P.ww_on) {
+      mix(p.hash, as_bits(p.wgt));
+      mix(p.hash, static_cast<uint64_t>(p.bank.size()));
+    }
   }
   if (p.E < P.energy_cutoff)
     p.wgt = 0.0;
@@ -1517,7 +1701,7 @@ void transport_history_based(Sim& S, int64_t n)
             event_collide(S, p, c);
           }
         }
-        event_revive(S.P, p);
+        event_revive(S, p);
       }
       event_death(S, p);
       if (p.lost) {
@@ -1592,7 +1776,7 @@ void transport_event_based(Sim& S, int64_t n_total)
         for (int64_t idx : q_surf) {
           Particle& p = particles[idx];
           event_cross_surface(S, p, c);
-          event_revive(P, p);
+          event_revive(S, p);
           if (p.alive())
             q_xs.push_back(idx);
         }
@@ -1602,7 +1786,7 @@ void transport_event_based(Sim& S, int64_t n_total)
         for (int64_t idx : q_coll) {
           Particle& p = particles[idx];
           event_collide(S, p, c);
-          event_revive(P, p);
+          event_revive(S, p);
           if (p.alive())
             q_xs.push_back(idx);
         }
@@ -1981,6 +2165,45 @@ void load_problem(Problem& P, const std::string& path)
     for (int j = 0; j < m.n_dim; ++j)
       nb *= m.shape[j];
     P.n_tally_bins = nb * (P.energy_bins.empty() ? 1 : static_cast<int64_t>(P.energy_bins.size()) - 1);
+  }
+  t.expect("ww");
+  P.ww_on = t.i() != 0;
+  if (P.ww_on) {
+    Mesh& m = P.ww_mesh;
+    m.n_dim = static_cast<int>(t.i());
+    for (int j = 0; j < 3; ++j)
+      m.shape[j] = static_cast<int>(t.i());
+    for (int j = 0; j < 3; ++j)
+      m.ll[j] = t.d();
+    for (int j = 0; j < 3; ++j)
+      m.ur[j] = t.d();
+    for (int j = 0; j < m.n_dim; ++j)
+      m.width[j] = (m.ur[j] - m.ll[j]) / static_cast<double>(m.shape[j]);
+    P.ww_n_mesh = 1;
+    for (int j = 0; j < m.n_dim; ++j)
+      P.ww_n_mesh *= m.shape[j];
+    const int ne = static_cast<int>(t.i());
+    for (int j = 0; j < ne; ++j)
+      P.ww_ebounds.push_back(t.d());
+    const int64_t nbnd = t.i();
+    if (ne < 2 || nbnd != P.ww_n_mesh * (ne - 1))
+      throw std::runtime_error("weight window bounds do not match the mesh and energy bounds");
+    P.ww_lower.resize(nbnd);
+    P.ww_upper.resize(nbnd);
+    for (int64_t j = 0; j < nbnd; ++j)
+      P.ww_lower[j] = t.d();
+    for (int64_t j = 0; j < nbnd; ++j)
+      P.ww_upper[j] = t.d();
+    P.ww_survival_ratio = t.d();
+    P.ww_max_lb_ratio = t.d();
+    P.ww_max_split = static_cast<int>(t.i());
+    P.ww_weight_cutoff = t.d();
+    P.ww_ckpt_collision = t.i() != 0;
+    P.ww_ckpt_surface = t.i() != 0;
+    P.max_history_splits = t.i();
+  } else {
+    P.ww_ckpt_collision = false;
+    P.ww_ckpt_surface = false;
   }
   t.expect("end");
 }

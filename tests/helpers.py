@@ -17,9 +17,8 @@ from oracle.problem import write_oracle_problem  # noqa: E402
 def prepare(config, **kw):
     """Problem directory with model.xml (product) and oracle_problem.txt (oracle)."""
     d, model, nucs = configs.problem_dir(config, **kw)
-    path = os.path.join(d, "oracle_problem.txt")
-    if not os.path.exists(path):
-        write_oracle_problem(model, nucs, d)
+    # rewritten every time: it is small (nuclide data are referenced by path)
+    write_oracle_problem(model, nucs, d)
     return d, model, nucs
 
 

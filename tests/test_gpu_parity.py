@@ -210,3 +210,25 @@ def test_learned_training_reduces_loss_and_bounds_hold(lib):
             assert st["loss_trained"] < st["loss_init"]
         finally:
             lib.finalize()
+
+
+# ---------------------------------------------------------------- weight windows
+
+@pytest.mark.parametrize("bank_cap", [64, 16])
+def test_transport_weight_windows_histories(bank_cap):
+    """Splitting, roulette and LIFO revival from the per-particle bank give the
+    oracle's histories bit for bit (3-layer slab, 12-bin windows)."""
+    kw = dict(n_particles=600, n_batches=2, n_grid=2000, n_layers=3, layer=4.0, per_layer=3, half_width=5.0,
+              density=0.05, ww_mesh=(12, 1, 1), ww_length=2.0, tally_mesh=(12, 1, 1), energy_groups=2)
+    d, model, nucs = prepare(4, **kw)
+    o = oracle_run(d, event_based=True)
+    g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap})
+    h_o = o.hashes(600)
+    assert np.array_equal(g["hashes"], h_o), "%d of 600 histories differ" % np.sum(g["hashes"] != h_o)
+    assert_tallies_close(g["results"], o.results())
+    c = o.counters()
+    assert g["stats"]["collisions"] == c["collisions"]
+    assert g["stats"]["surf_events"] == c["surf_events"]
+    lk = o.leakage()
+    # global tallies [K_COLLISION, K_ABSORPTION, K_TRACKLENGTH, LEAKAGE] x [VALUE, SUM, SUM_SQ]
+    assert np.asarray(g["global"]).reshape(-1, 3)[3, 1] == pytest.approx(lk["leak_sum"], rel=1e-12)
