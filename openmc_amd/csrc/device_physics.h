@@ -766,8 +766,11 @@ __device__ __forceinline__ double wave_uniform(double x)
   return __longlong_as_double(wave_uniform(static_cast<int64_t>(__double_as_longlong(x))));
 }
 
-// Points of the final lower_bound step, loaded together.
-constexpr int XS_WINDOW = 8;
+// Points of the final lower_bound step, loaded together.  One point (a plain
+// binary search) measured fastest on MI355X: the lookup kernel is bound by
+// vector-memory instruction issue, and an 8-point final window (one
+// dependent step instead of ~3) cost more instructions than it saved.
+constexpr int XS_WINDOW = 1;
 
 template<int K, bool LEARNED>
 __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevMatNuc* mn, int kk, double E,

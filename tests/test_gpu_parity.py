@@ -214,13 +214,24 @@ def test_learned_training_reduces_loss_and_bounds_hold(lib):
 
 # ---------------------------------------------------------------- weight windows
 
-@pytest.mark.parametrize("bank_cap", [64, 16])
+WW_SLAB = dict(n_particles=600, n_batches=2, n_grid=2000, n_layers=3, layer=4.0, per_layer=3, half_width=5.0,
+               density=0.05, ww_mesh=(12, 1, 1), ww_length=2.0, tally_mesh=(12, 1, 1), energy_groups=2)
+
+
+def test_weight_windows_bank_overflow_is_an_error():
+    """A bank too small for the splitting stops the run with an error (the
+    kernels drop the extra sites inside their bounds and flag it)."""
+    d, model, nucs = prepare(4, **WW_SLAB)
+    from openmc_amd import lib
+    with pytest.raises(lib.OpenMCError, match="secondary bank overflowed"):
+        gpu_run(d, options={"bank_cap": 2})
+
+
+@pytest.mark.parametrize("bank_cap", [64])
 def test_transport_weight_windows_histories(bank_cap):
     """Splitting, roulette and LIFO revival from the per-particle bank give the
     oracle's histories bit for bit (3-layer slab, 12-bin windows)."""
-    kw = dict(n_particles=600, n_batches=2, n_grid=2000, n_layers=3, layer=4.0, per_layer=3, half_width=5.0,
-              density=0.05, ww_mesh=(12, 1, 1), ww_length=2.0, tally_mesh=(12, 1, 1), energy_groups=2)
-    d, model, nucs = prepare(4, **kw)
+    d, model, nucs = prepare(4, **WW_SLAB)
     o = oracle_run(d, event_based=True)
     g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap})
     h_o = o.hashes(600)
