@@ -174,8 +174,70 @@ def cpu_baseline(d, target_s=10.0):
             "modes": modes}
 
 
+# rocprof kernel name -> (name in this script's kernel table, unit of work)
+KERNEL_UNITS = {"k_xs_lane": ("k_xs", "xs_event"), "k_xs": ("k_xs", "xs_event"),
+                "k_advance": ("k_advance", "adv_event"), "k_surface": ("k_surface", "surf_event"),
+                "k_collision": ("k_collision", "coll_event")}
+
+
+def pmc_summary(dirs, events=None, copy_bytes=None):
+    """Per-kernel sums of the counters in rocprofv3 databases (``--pmc`` or
+    ``--kernel-trace`` runs), as text.  With the run's event counts (a JSON
+    dict from this script's output) and the copy kernel's size, also the
+    calibrated HBM bytes per event of each event kernel ("traffic" lines:
+    counter bytes scaled by copy_bytes / what the counter read for the copy)."""
+    import collections
+    import glob
+    import re
+    import sqlite3
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    dur = collections.defaultdict(float)
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "*_results.db")):
+            c = sqlite3.connect(f)
+            tables = {r[0] for r in c.execute("select name from sqlite_master")}
+            if "counters_collection" in tables:
+                for name, cn, val, did in c.execute(
+                        "select kernel_name, counter_name, value, dispatch_id from counters_collection"):
+                    k = re.sub(r"\(.*", "", name).replace("void ", "").replace("omc::", "")
+                    agg[k][cn] += val
+                    disp[k].add(did)
+            if "kernels" in tables:
+                for name, s, e, did in c.execute("select name, start, end, dispatch_id from kernels"):
+                    k = re.sub(r"\(.*", "", name).replace("void ", "").replace("omc::", "")
+                    dur[k] += (e - s) * 1e-6
+                    disp[k].add(did)
+    lines = ["%-34s %10s %14s %16s %16s" % ("kernel", "dispatches", "total_ms", "FETCH_KiB/disp", "WRITE_KiB/disp")]
+    for k in sorted(disp, key=lambda k: -max(dur.get(k, 0), agg[k].get("FETCH_SIZE", 0))):
+        n = max(len(disp[k]), 1)
+        lines.append("%-34s %10d %14.3f %16.1f %16.1f" % (k, n, dur.get(k, 0.0), agg[k].get("FETCH_SIZE", 0) / n,
+                                                        agg[k].get("WRITE_SIZE", 0) / n))
+    if events and copy_bytes:
+        copy = [k for k in agg if k.startswith("k_copy")]
+        if copy:
+            ck = copy[0]
+            fetch_scale = copy_bytes / (agg[ck]["FETCH_SIZE"] * 1024.0 / len(disp[ck]))
+            write_scale = copy_bytes / (agg[ck]["WRITE_SIZE"] * 1024.0 / len(disp[ck]))
+            lines.append("copy calibration: %d-byte copy; FETCH scale %.4f, WRITE scale %.4f"
+                         % (copy_bytes, fetch_scale, write_scale))
+            for k in agg:
+                base = re.sub(r"<.*", "", k)
+                if base not in KERNEL_UNITS:
+                    continue
+                name, unit = KERNEL_UNITS[base]
+                if not events.get(UNIT_COUNTER[unit]):
+                    continue
+                b = (agg[k]["FETCH_SIZE"] * 1024.0 * fetch_scale + agg[k]["WRITE_SIZE"] * 1024.0 * write_scale)
+                lines.append("traffic %s %s %.2f" % (name, unit, b / events[UNIT_COUNTER[unit]]))
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 result directories instead of running")
+    ap.add_argument("--events", help="with --pmc-summary: JSON line printed by the profiled run")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
@@ -188,6 +250,15 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run a plain device copy of this many bytes first (HBM counter calibration)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        events = None
+        if args.events:
+            with open(args.events) as f:
+                for line in f:
+                    if line.startswith("{"):
+                        events = json.loads(line).get("events")
+        print(pmc_summary(args.pmc_summary, events, args.copy_calibration or None))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
               "design is a per-rank particle split plus one tally reduction per batch)" % args.gpus,
@@ -238,6 +309,8 @@ def main():
         "events_per_particle": round((st["xs_events"] + st["adv_events"] + st["surf_events"] +
                                       st["coll_events"]) / (n * args.steps), 2),
         "kernels": per_kernel,
+        "events": {k: st[k] for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
+                                      "collisions", "segments", "search_probes", "cycles")},
         "roofline": roof,
     }
     if copy_info:

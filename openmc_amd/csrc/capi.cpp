@@ -116,6 +116,7 @@ struct Sim {
   int blocks_opt = 0;      // 0 = from the capacity
   int xs_group_opt = 4;
   int bank_cap = 32; // secondary bank sites per particle (weight windows)
+  int sort_bits = 5; // mantissa bits of E in the calculate-xs sort key (32 buckets per binade)
   LaunchConfig lc {1, 1, 0, 1, 4};
   int64_t max_in_flight_opt = 0;
   std::vector<int64_t> nuc_off; // first grid point of each nuclide
@@ -398,11 +399,12 @@ void alloc_particles(Sim& s, int64_t cap)
   s.h_counter_slots.assign(static_cast<size_t>(CNT_N) * CNT_SLOTS, 0);
   s.h_leak_slots.assign(CNT_SLOTS, 0.0);
   // calculate-xs keys (material, energy bucket): the bucket is the sign,
-  // exponent and top mantissa bits of E, offset by those of energy_min;
-  // four buckets per binade (two mantissa bits), fewer when the materials
-  // times buckets exceed the LDS histogram
+  // exponent and top sort_bits mantissa bits of E, offset by those of
+  // energy_min; 64 binades of 2^sort_bits buckets each (32 per binade by
+  // default: the lanes of a wavefront then read neighbouring grid points),
+  // fewer when the materials times buckets exceed the LDS histogram
   const int n_mat = s.P.n_materials + 1;
-  int shift = 50, eb = 256;
+  int shift = 52 - s.sort_bits, eb = 64 << s.sort_bits;
   while (n_mat * eb > SORT_MAX_BINS && eb > 1) {
     ++shift;
     eb >>= 1;
@@ -1003,13 +1005,17 @@ int omc_set_option(const char* name, double value)
   } else if (n == "sort") {
     g->sort = value != 0;
     update_launch_config(*g);
-  } else if (n == "bank_cap") {
+  } else if (n == "bank_cap" || n == "sort_bits") {
     int v = static_cast<int>(value);
-    if (v < 1 || v > 4096) {
+    if (n == "bank_cap" && (v < 1 || v > 4096)) {
       set_errmsg("bank_cap must be in [1, 4096]");
       return OPENMC_E_INVALID_ARGUMENT;
     }
-    g->bank_cap = v;
+    if (n == "sort_bits" && (v < 0 || v > 8)) {
+      set_errmsg("sort_bits must be in [0, 8]");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    (n == "bank_cap" ? g->bank_cap : g->sort_bits) = v;
     if (g->sim_initialized)
       return guarded([&]() -> int {
         alloc_particles(*g, g->cap);

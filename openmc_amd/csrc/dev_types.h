@@ -14,7 +14,7 @@
 namespace omc {
 
 constexpr int MAX_LEVELS = 4;
-constexpr int SORT_MAX_BINS = 2048;  // (material + 1, energy bucket) keys
+constexpr int SORT_MAX_BINS = 16384; // (material + 1, energy bucket) keys: a 64 KB LDS histogram
 constexpr int SORT_MAX_BLOCKS = 256; // workgroups of the sort kernels
 
 // Constants of the reference (include/openmc/constants.h:49-113)
@@ -102,8 +102,17 @@ struct alignas(64) DevMatNuc {
 };
 static_assert(sizeof(DevMatNuc) == 64, "one 64-byte record per material nuclide");
 
+// One segment of a learned index as the lookup reads it (one 16-byte load).
+struct alignas(16) LSeg {
+  float a;  // slope
+  float b;  // intercept
+  int err;  // error bound of the segment
+  int pad;
+};
+
 // Learned grid index: piecewise-linear model of i(log E) per nuclide.
 struct DevLearned {
+  const LSeg* rec;     // [n_nuc][n_seg] the lookup's copy of seg and err
   const float* seg;    // [n_nuc][n_seg][2] slope, intercept (i ~= slope*u + icpt)
   const int* err;      // [n_nuc][n_seg] max |i_true - i_pred| over the segment's grid points
   const double* u_lo;  // [n_nuc] log(E_first) of each nuclide grid
@@ -314,8 +323,9 @@ struct DevParticles {
   int* q_sc;      // [cap] surface crossings, then collisions
   int* stage_p;   // [cap] slots after surface crossing / collision
   int* stage_key; // [cap] their calculate-xs sort keys, -1 when done
-  // partition scratch: sort_count is a [bin][SORT_MAX_BLOCKS] matrix of
-  // per-workgroup bin counts, sort_total the bin totals then bin starts
+  // partition scratch: sort_count is a [workgroup][nb] matrix of per-
+  // workgroup key counts (SORT_MAX_BLOCKS x sort_nb ints), sort_total the
+  // key totals then key starts
   int* sort_count;
   int* sort_total;
   int sort_nb;    // calculate-xs keys: (material + 1) * sort_eb + energy bucket

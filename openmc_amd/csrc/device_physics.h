@@ -733,9 +733,12 @@ __device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, doub
 // K lookups of one energy in K nuclides, interleaved: every phase issues the
 // K independent loads before it uses any of them, and the K bounded searches
 // run in lockstep, so one lane keeps up to K cache misses in flight instead of
-// one.  Each search halves its range only while more than XS_WINDOW points
-// remain and then reads the remaining points in one step, so a bucket of up
-// to XS_WINDOW points costs one dependent load instead of log2 of its size.
+// one.  With the learned index the brackets of the predicted interval are read
+// first: when they enclose E strictly (E[j] < E < E[j+1]) that interval is
+// the answer and no search runs, which costs one model read and the three
+// bracket reads, the fewest loads any exact method needs.  (A final step
+// that reads several points at once instead of probing was measured slower:
+// the kernel is bound by vector-memory instruction issue.)
 // Each result equals nuclide_xs for the same nuclide:
 //  - the bucket search's special case (E equal to the first point of its
 //    bucket gives that point, include/openmc/search.h:17-18) is applied after
@@ -766,12 +769,6 @@ __device__ __forceinline__ double wave_uniform(double x)
   return __longlong_as_double(wave_uniform(static_cast<int64_t>(__double_as_longlong(x))));
 }
 
-// Points of the final lower_bound step, loaded together.  One point (a plain
-// binary search) measured fastest on MI355X: the lookup kernel is bound by
-// vector-memory instruction issue, and an 8-point final window (one
-// dependent step instead of ~3) cost more instructions than it saved.
-constexpr int XS_WINDOW = 1;
-
 template<int K, bool LEARNED>
 __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevMatNuc* mn, int kk, double E,
   int i_log, double logE, int& probes, Micro* out)
@@ -791,6 +788,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
     e_last[k] = wave_uniform(r->e_last);
     Eg[k] = P.E + off[k];
   }
+  // final bracket values; with the learned index they are first read
+  // speculatively at the predicted interval
+  double e0[K], e1[K];
+  double2 x0[K], x1[K];
+  bool hit[K];
   // bracket by the log-grid bucket (or the learned window); all K nuclides'
   // loads are issued before any is used
   int lo[K], hi[K];
@@ -800,70 +802,78 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
       const int* gi = P.grid_index + static_cast<int64_t>(nuc[k]) * (P.n_log_bins + 1);
       lo[k] = gi[i_log];
       hi[k] = gi[i_log + 1] + 1;
+      hit[k] = false;
     }
   } else {
     const DevLearned& Lm = P.learned;
-    double u_lo[K], inv_w[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const DevMatNuc* r = mn + (k < kk ? k : 0);
-      u_lo[k] = wave_uniform(r->u_lo);
-      inv_w[k] = wave_uniform(r->inv_w);
-    }
-    float2 sg[K];
-    int err[K];
+    LSeg sg[K];
     double uu[K];
     int seg[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      uu[k] = (logE - u_lo[k]) * inv_w[k];
+      const DevMatNuc* r = mn + (k < kk ? k : 0);
+      uu[k] = (logE - wave_uniform(r->u_lo)) * wave_uniform(r->inv_w);
       int sgi = uu[k] < 0.0 ? 0 : static_cast<int>(uu[k]);
       seg[k] = sgi >= Lm.n_seg ? Lm.n_seg - 1 : sgi;
-      const int64_t sk = static_cast<int64_t>(nuc[k]) * Lm.n_seg + seg[k];
-      sg[k] = reinterpret_cast<const float2*>(Lm.seg)[sk];
-      err[k] = Lm.err[sk];
+      sg[k] = Lm.rec[static_cast<int64_t>(nuc[k]) * Lm.n_seg + seg[k]];
     }
+    int js[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       float fr = static_cast<float>(uu[k] - static_cast<double>(seg[k]));
-      float pred = sg[k].x * fr + sg[k].y;
+      float pred = sg[k].a * fr + sg[k].b;
       int j = static_cast<int>(pred);
-      int l = j - err[k] - 1;
-      int h = j + err[k] + 3;
+      int l = j - sg[k].err - 1;
+      int h = j + sg[k].err + 3;
       l = l < 0 ? 0 : (l > n[k] ? n[k] : l);
       h = h > n[k] ? n[k] : (h < l ? l : h);
       lo[k] = l;
       hi[k] = h;
+      js[k] = j < 0 ? 0 : (j > n[k] - 2 ? n[k] - 2 : j);
+    }
+    // speculative brackets at the predicted interval js
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      e0[k] = Eg[k][js[k]];
+      e1[k] = Eg[k][js[k] + 1];
+      x0[k] = P.XS[off[k] + js[k]];
+      x1[k] = P.XS[off[k] + js[k] + 1];
+    }
+    // E[js] < E < E[js + 1] makes the lower bound js + 1 with E not a grid
+    // point (so neither the bucket-start case nor the duplicate bump applies):
+    // the interval is js
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      hit[k] = e0[k] < E && E < e1[k];
+      ig[k] = js[k];
     }
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool below = E < e_first[k];
     const bool above = !below && E > e_last[k];
-    search[k] = k < kk && !below && !above;
-    ig[k] = below ? 0 : n[k] - 2;
+    search[k] = k < kk && !below && !above && !hit[k];
+    if (!hit[k])
+      ig[k] = below ? 0 : n[k] - 2;
     lo0[k] = lo[k];
     fst[k] = lo[k];
     cnt[k] = search[k] ? hi[k] - lo[k] : 0;
   }
-  // lower_bound of E in [fst, fst + cnt): halve the ranges in lockstep
-  // until at most XS_WINDOW points are left, then load those points at once
-  // and count the ones below E (the grid is sorted, so they form a prefix and
-  // the lower bound is fst plus their count)
+  // lower_bound of E in [fst, fst + cnt), the K searches in lockstep
   while (true) {
     bool any = false;
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      any = any || cnt[k] > XS_WINDOW;
+      any = any || cnt[k] > 0;
     if (!any)
       break;
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      v[k] = cnt[k] > XS_WINDOW ? Eg[k][fst[k] + (cnt[k] >> 1)] : 0.0;
+      v[k] = cnt[k] > 0 ? Eg[k][fst[k] + (cnt[k] >> 1)] : 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (cnt[k] > XS_WINDOW) {
+      if (cnt[k] > 0) {
         ++probes;
         const int step = cnt[k] >> 1;
         if (v[k] < E) {
@@ -873,26 +883,6 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
           cnt[k] = step;
         }
       }
-    }
-  }
-  {
-    double w[K][XS_WINDOW];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-#pragma unroll
-      for (int t = 0; t < XS_WINDOW; ++t) {
-        const int i = fst[k] + t;
-        w[k][t] = Eg[k][i < n[k] - 1 ? i : n[k] - 1];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      int c = 0;
-#pragma unroll
-      for (int t = 0; t < XS_WINDOW; ++t)
-        c += (t < cnt[k] && w[k][t] < E) ? 1 : 0;
-      probes += cnt[k] > 0 ? 1 : 0;
-      fst[k] += c;
     }
   }
 #pragma unroll
@@ -913,10 +903,10 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
       }
     }
   }
-  double e0[K], e1[K];
-  double2 x0[K], x1[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    if (hit[k])
+      continue;
     int i = ig[k];
     i = i < 0 ? 0 : (i > n[k] - 2 ? n[k] - 2 : i);
     ig[k] = i;
@@ -930,7 +920,7 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
     if (k >= kk)
       continue;
     int i = ig[k];
-    if (i + 2 < n[k] && e0[k] == e1[k]) {
+    if (!hit[k] && i + 2 < n[k] && e0[k] == e1[k]) {
       ++i;
       e0[k] = e1[k];
       e1[k] = Eg[k][i + 1];

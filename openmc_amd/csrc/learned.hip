@@ -343,6 +343,19 @@ int build_learned_index(Learned& L, const HostModel& m, DevProblem& P, int n_seg
   L.n_seg = n_seg;
   L.h_ulo = u_lo;
   L.h_invw = inv_w;
+  {
+    // the lookup's packed copy: slope, intercept and error bound in one record
+    std::vector<LSeg> rec(L.h_err.size());
+    for (size_t k = 0; k < rec.size(); ++k) {
+      rec[k].a = L.h_seg[2 * k];
+      rec[k].b = L.h_seg[2 * k + 1];
+      rec[k].err = L.h_err[k];
+      rec[k].pad = 0;
+    }
+    LSeg* d_rec = lalloc<LSeg>(L.allocs, rec.size());
+    lcheck(hipMemcpy(d_rec, rec.data(), sizeof(LSeg) * rec.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    P.learned.rec = d_rec;
+  }
   P.learned.seg = reinterpret_cast<const float*>(params);
   P.learned.err = err;
   P.learned.u_lo = d_ulo;

@@ -2,31 +2,25 @@
 //
 // One batch runs as: k_init (source sampling, src/simulation.cpp:573-649),
 // then cycles of
-//     [k_sort_count -> k_sort_scan -> k_sort_scatter] -> k_xs -> k_advance
-//       -> k_surface -> k_collision
+//     partition(staged -> q_xs) -> k_xs -> k_advance
+//       -> partition(q_xs -> q_sc) -> k_surface -> k_collision
 // until no particle is alive.  The calculate-xs queue doubles as the advance
 // queue (every particle that computes cross sections advances next,
-// src/event.cpp:87-101); advance splits it into the surface-crossing and
-// collision queues; both of those refill the calculate-xs queue with the
-// survivors.  Appends are compacted per wavefront: a ballot of the lanes that
-// append, one atomicAdd per wavefront for the base slot, and each lane's rank
-// among the set lanes (popcount of the lower mask bits) as its offset.
+// src/event.cpp:87-101).  No kernel appends to a shared queue counter: each
+// writes its result for input i at position i of a staging array (the
+// particle's slot and a key, -1 when its history ended), and a partition pass
+// (per-workgroup key counts, a column scan, a key scan, an LDS-ranked
+// scatter) compacts the staging array by key into the next queue: by outcome
+// (surface crossing, collision) after advance, by (material, energy bucket)
+// for calculate-xs -- the order the comment at src/event.cpp:77-85 asks for,
+// so that neighbouring lanes read neighbouring parts of the same energy
+// grids.  Histories do not depend on the processing order.
 //
-// The optional counting sort orders the calculate-xs queue by (material,
-// energy bucket) -- the order the reference's event.cpp:77-85 comment asks
-// for -- so that neighbouring lanes search neighbouring parts of the same
-// energy grids.  Histories do not depend on the processing order.  Particle
-// state is one 128-byte record per particle (PHot) plus one 64-byte record
-// per deeper coordinate level, so visiting particles in queue order costs one
-// cache line per particle rather than one per field.
-//
-// Queue sizes live in device memory, so a cycle needs no host round trip and
-// a run of cycles can be captured in one hipGraph.  Each counter is reset by
-// a kernel that neither reads nor appends to it:
-//   k_xs        reads XS,   sets ADV = XS, resets SURF and COLL
-//   k_advance   reads ADV,  resets XS, appends SURF / COLL
-//   k_surface   reads SURF, appends XS
-//   k_collision reads COLL, appends XS
+// Particle state is one 128-byte record per particle (PHot) plus one 64-byte
+// record per deeper coordinate level, so visiting particles in queue order
+// costs one cache line per particle rather than one per field.  Queue sizes
+// live in device memory, written by the partition passes, so a cycle needs no
+// host round trip and a run of cycles is captured in one hipGraph.
 //
 // All kernels are templated on NL, the number of coordinate levels of the
 // model, so that per-level state stays in registers.
@@ -379,7 +373,8 @@ __device__ __forceinline__ int part_key(const Part& A, int i)
   return k < 0 ? -1 : (A.nb == 1 ? 0 : k);
 }
 
-// Per-workgroup key counts of its chunk into sort_count[key][b].
+// Per-workgroup key counts of its chunk into row b of the count matrix
+// sort_count[b][key] (row length nb: a workgroup writes one contiguous row).
 __global__ void __launch_bounds__(BLOCK) k_part_count(DevParticles S, Part A)
 {
   __shared__ int sh_cnt[SORT_MAX_BINS];
@@ -399,47 +394,41 @@ __global__ void __launch_bounds__(BLOCK) k_part_count(DevParticles S, Part A)
       atomicAdd(&sh_cnt[k], 1);
   }
   __syncthreads();
+  int* row = S.sort_count + static_cast<int64_t>(blockIdx.x) * A.nb;
   for (int k = threadIdx.x; k < A.nb; k += blockDim.x)
-    S.sort_count[k * SORT_MAX_BLOCKS + blockIdx.x] = sh_cnt[k];
+    row[k] = sh_cnt[k];
 }
 
-// One wavefront per key: exclusive scan of the key's row over the workgroups
-// (sort_count[key][b] becomes workgroup b's offset inside the key; each lane
-// holds four consecutive workgroups) and the key total into sort_total[key].
-__global__ void __launch_bounds__(BLOCK) k_part_rows(DevParticles S, Part A, int max_blocks)
+// One thread per key, 64 keys per workgroup: exclusive scan of the key's
+// column over the workgroup rows (sort_count[b][key] becomes workgroup b's
+// offset inside the key) and the key total into sort_total[key].  Each step
+// of the scan reads and writes 64 consecutive keys of one row.
+__global__ void __launch_bounds__(64) k_part_cols(DevParticles S, Part A, int max_blocks)
 {
-  static_assert(SORT_MAX_BLOCKS == 4 * 64, "four workgroups per lane");
-  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= A.nb)
     return;
-  const int lane = lane_id();
   const int nw = sort_workgroups(*A.n_in, max_blocks);
-  int* row = S.sort_count + static_cast<int64_t>(k) * SORT_MAX_BLOCKS;
-  int v[4];
-  int local = 0;
+  int* col = S.sort_count + k;
+  int run = 0;
+  int b = 0;
+  for (; b + 4 <= nw; b += 4) {
+    int v[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int b = 4 * lane + j;
-    v[j] = b < nw ? row[b] : 0;
-    local += v[j];
-  }
-  int incl = local;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o);
-    if (lane >= o)
-      incl += y;
-  }
-  int run = incl - local;
+    for (int j = 0; j < 4; ++j)
+      v[j] = col[static_cast<int64_t>(b + j) * A.nb];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int b = 4 * lane + j;
-    if (b < nw)
-      row[b] = run;
-    run += v[j];
+    for (int j = 0; j < 4; ++j) {
+      col[static_cast<int64_t>(b + j) * A.nb] = run;
+      run += v[j];
+    }
   }
-  const int total = __shfl(incl, 63);
-  if (lane == 0)
-    S.sort_total[k] = total;
+  for (; b < nw; ++b) {
+    const int v = col[static_cast<int64_t>(b) * A.nb];
+    col[static_cast<int64_t>(b) * A.nb] = run;
+    run += v;
+  }
+  S.sort_total[k] = run;
 }
 
 // One workgroup: exclusive scan of the key totals into key starts; writes
@@ -486,7 +475,7 @@ __global__ void __launch_bounds__(BLOCK) k_part_scatter(DevParticles S, Part A)
   if (static_cast<int>(blockIdx.x) >= nw)
     return;
   for (int k = threadIdx.x; k < A.nb; k += blockDim.x)
-    sh_off[k] = S.sort_total[k] + S.sort_count[k * SORT_MAX_BLOCKS + blockIdx.x];
+    sh_off[k] = S.sort_total[k] + S.sort_count[static_cast<int64_t>(blockIdx.x) * A.nb + k];
   __syncthreads();
   const int chunk = (n + nw - 1) / nw;
   const int lo = blockIdx.x * chunk;
@@ -1208,9 +1197,9 @@ void xs_lpp(const DevProblem& P, const DevParticles& S, const int* q, const Laun
 void launch_partition(const DevParticles& S, const Part& A, const LaunchConfig& lc, hipStream_t st)
 {
   const int sb = lc.sort_blocks < 1 ? 1 : (lc.sort_blocks > SORT_MAX_BLOCKS ? SORT_MAX_BLOCKS : lc.sort_blocks);
-  const int row_blocks = (A.nb * 64 + BLOCK - 1) / BLOCK;
+  const int col_blocks = (A.nb + 63) / 64;
   hipLaunchKernelGGL(k_part_count, dim3(sb), dim3(BLOCK), 0, st, S, A);
-  hipLaunchKernelGGL(k_part_rows, dim3(row_blocks), dim3(BLOCK), 0, st, S, A, sb);
+  hipLaunchKernelGGL(k_part_cols, dim3(col_blocks), dim3(64), 0, st, S, A, sb);
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, S, A);
   hipLaunchKernelGGL(k_part_scatter, dim3(sb), dim3(BLOCK), 0, st, S, A);
 }
