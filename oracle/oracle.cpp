@@ -2319,6 +2319,37 @@ void oracle_lookup(void* h, int i_nuc, double E, int* i_grid, double* out)
 }
 
 // Unit-level restatements used by the known-answer tests.
+// apply_weight_windows on one particle of the loaded problem:
+//   in  = {x, y, z, u, v, w, E, wgt, wgt_born, wgt_ww_born, ww_factor, n_split}
+//   out = {wgt, wgt_ww_born, ww_factor, n_split, n_bank}
+// bank_w receives the weights of the banked copies (at most bank_cap).
+int oracle_ww_apply(void* h, const double* in, uint64_t* seed, double* out, double* bank_w, int bank_cap)
+{
+  const Problem& P = static_cast<Sim*>(h)->P;
+  Particle p;
+  p.coord.resize(1);
+  p.coord[0].r = {in[0], in[1], in[2]};
+  p.coord[0].u = {in[3], in[4], in[5]};
+  p.E = in[6];
+  p.wgt = in[7];
+  p.wgt_born = in[8];
+  p.wgt_ww_born = in[9];
+  p.ww_factor = in[10];
+  p.n_split = static_cast<int>(in[11]);
+  p.stream = STREAM_TRACKING;
+  p.seeds[STREAM_TRACKING] = *seed;
+  apply_weight_windows(P, p);
+  *seed = p.seeds[STREAM_TRACKING];
+  out[0] = p.wgt;
+  out[1] = p.wgt_ww_born;
+  out[2] = p.ww_factor;
+  out[3] = p.n_split;
+  out[4] = static_cast<double>(p.bank.size());
+  for (size_t k = 0; k < p.bank.size() && static_cast<int>(k) < bank_cap; ++k)
+    bank_w[k] = p.bank[k].wgt;
+  return static_cast<int>(p.bank.size());
+}
+
 double oracle_prn(uint64_t* seed) { return prn(seed); }
 uint64_t oracle_future_seed(uint64_t n, uint64_t seed) { return future_seed(n, seed); }
 double oracle_maxwell(double T, uint64_t* seed) { return maxwell_spectrum(T, seed); }

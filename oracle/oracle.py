@@ -52,6 +52,9 @@ def load_library(glibc_math=False):
     lib.oracle_watt.restype = c_double
     lib.oracle_rotate_angle.argtypes = [POINTER(c_double), c_double, POINTER(c_double), POINTER(c_uint64),
                                         POINTER(c_double)]
+    lib.oracle_ww_apply.argtypes = [c_void_p, POINTER(c_double), POINTER(c_uint64), POINTER(c_double),
+                                    POINTER(c_double), c_int]
+    lib.oracle_ww_apply.restype = c_int
     for fn in ("oracle_log", "oracle_sin", "oracle_cos"):
         getattr(lib, fn).argtypes = [c_double]
         getattr(lib, fn).restype = c_double
@@ -124,6 +127,19 @@ class Oracle:
 
     def grid_index(self, i_nuc, n_log_bins=8000):
         return np.ctypeslib.as_array(self.lib.oracle_grid_index(self.h, i_nuc), (n_log_bins + 1,)).copy()
+
+    def ww_apply(self, r, E, wgt, u=(1.0, 0.0, 0.0), wgt_born=1.0, wgt_ww_born=-1.0, ww_factor=0.0, n_split=0,
+                 seed=1):
+        """apply_weight_windows on one particle: returns (state, bank weights, seed after),
+        state = dict(wgt, wgt_ww_born, ww_factor, n_split, n_bank)."""
+        vals = list(r) + list(u) + [E, wgt, wgt_born, wgt_ww_born, ww_factor, float(n_split)]
+        inp = (c_double * 12)(*vals)
+        out = (c_double * 5)()
+        bank = (c_double * 64)()
+        s = c_uint64(seed)
+        n = self.lib.oracle_ww_apply(self.h, inp, s, out, bank, 64)
+        state = dict(zip(["wgt", "wgt_ww_born", "ww_factor", "n_split", "n_bank"], list(out)))
+        return state, list(bank)[:min(n, 64)], s.value
 
     def lookup(self, i_nuc, energies):
         E = np.asarray(energies, dtype=np.float64)
