@@ -174,6 +174,125 @@ def cpu_baseline(d, target_s=10.0):
             "modes": modes}
 
 
+def run_problem_set(dirs, steps, warmup, options=None, tune_path=None):
+    """The problems of a set in one shared event loop: W warm-up and K timed
+    batches (every problem advances one batch per step)."""
+    from openmc_amd import autotune, lib
+    lib.init_problems(dirs, output=False)
+    info = {}
+    try:
+        for k, v in (options or {}).items():
+            lib.set_option(k, v)
+        if tune_path:
+            t = time.perf_counter()
+            best, rec = autotune.tune_cached(dirs, tune_path, log=log)
+            info["autotune"] = {"options": best, "seconds": round(time.perf_counter() - t, 2),
+                                "trials": rec["trials"], "from_cache": bool(rec.get("cached"))}
+        lib.set_option("event_timing", 1)
+        lib.simulation_init()
+        for _ in range(warmup):
+            lib.next_batch()
+        lib.reset_timers()
+        s0 = lib.stats()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            lib.next_batch()
+            log("  batch %d done (%.2f s)" % (k + 1, time.perf_counter() - t0))
+        t1 = time.perf_counter()
+        s1 = lib.stats()
+        info["options"] = {k: s1[k] for k in ("blocks", "block", "sort_every", "cycles_per_check", "sort_bits",
+                                              "xs_group")}
+        lib.simulation_finalize()
+        info["n_particles"] = [lib.problem_info(k)["n_particles"] for k in range(lib.n_problems())]
+    finally:
+        lib.finalize()
+    info.update({"elapsed_s": t1 - t0, "stats": diff(s0, s1)})
+    return info
+
+
+def cpu_baseline_set(problems, dirs, target_s=10.0):
+    """The oracle, history-based, on the first particles of batch 1 of the
+    problems of a set in turn, until about target_s of CPU work."""
+    from oracle.oracle import Oracle
+    from oracle.problem import write_oracle_problem
+    threads = min(os.cpu_count() or 1, 16)
+    done, secs, used = 0, 0.0, 0
+    for (model, nucs), d in zip(problems, dirs):
+        path = write_oracle_problem(model, nucs, d, data_dir=os.path.join("..", "data"))
+        n = min(model.settings.particles, 20000)
+        o = Oracle(path)
+        t = time.perf_counter()
+        o.run(event_based=False, threads=threads, n_particles=n, n_batches=1, trace_hash=False)
+        secs += time.perf_counter() - t
+        o.close()
+        done += n
+        used += 1
+        if secs >= target_s:
+            break
+    return {"value": round(done / secs, 2), "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": "the oracle (C++ restatement, -O3, OpenMP %d threads, history-based) on the first "
+                      "min(n, 20000) histories of batch 1 of problems 0-%d of the set, one after another; "
+                      "%d histories in %.1f s" % (threads, used - 1, done, secs)}
+
+
+def bench_problem_set(args):
+    """Batched uneven problems: 64 lattice problems from a seed in one event
+    loop, against the same problems run one after another."""
+    from openmc_amd import configs
+    n_batches = args.warmup + args.steps + 1
+    log("preparing the problem set")
+    d, dirs, problems, nucs = configs.problem_set_dir(n_batches=max(n_batches, 2))
+    opts = {k: float(v) for k, v in (o.split("=", 1) for o in args.option)}
+    tune_path = None if args.no_autotune else os.path.join(d, "autotune.json")
+    log("GPU problem set: %d problems, %d warm-up + %d timed batches" % (len(dirs), args.warmup, args.steps))
+    rs = run_problem_set(dirs, args.steps, args.warmup, options=opts, tune_path=tune_path)
+    st = rs["stats"]
+    total = sum(rs["n_particles"])
+    value = total * args.steps / rs["elapsed_s"]
+    dom, roof, per_kernel = roofline(dict(rs, stats=st))
+    chosen = dict(opts)
+    chosen.update(rs.get("autotune", {}).get("options", {}))
+    log("GPU problems one at a time (1 warm-up + 1 timed batch each)")
+    alone = []
+    for k, dd in enumerate(dirs):
+        r = run_gpu(dd, 1, 1, options=chosen)
+        alone.append({"particles": r["n_particles"], "ms_per_batch": round(1e3 * r["elapsed_s"], 3),
+                      "particles_per_s": round(r["n_particles"] / r["elapsed_s"], 1)})
+    seq_s = sum(a["ms_per_batch"] for a in alone) * 1e-3
+    out = {
+        "metric": "particles/s and XS lookups/s on 1 MI355X: learned-index vs binary search",
+        "value": round(value, 2),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * rs["elapsed_s"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic nuclides from fixed seeds; random source histories",
+        "config": {"workload": "batched-%d-lattice-problems-%.3gpps" % (len(dirs), total), "problems": len(dirs),
+                   "particles_per_batch": total, "nuclides": len(nucs),
+                   "mode": "event-based, one event loop shared by all problems"},
+        "xs_lookups_per_s": round(xs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
+        "sequential": {"particles_per_s": round(total / seq_s, 2), "seconds_per_batch": round(seq_s, 4)},
+        "batching_speedup": round(value / (total / seq_s), 3),
+        "per_problem": alone,
+        "launch_options": rs["options"],
+        "kernels": per_kernel,
+        "events": {k: st[k] for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
+                                      "collisions", "segments", "search_probes", "cycles")},
+        "roofline": roof,
+    }
+    if "autotune" in rs:
+        out["autotune"] = rs["autotune"]
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle, history-based, problem samples)")
+        out["cpu_baseline"] = cpu_baseline_set(problems, dirs)
+    print(json.dumps(out), flush=True)
+
+
 # rocprof kernel name -> (name in this script's kernel table, unit of work)
 KERNEL_UNITS = {"k_xs_lane": ("k_xs", "xs_event"), "k_xs": ("k_xs", "xs_event"),
                 "k_advance": ("k_advance", "adv_event"), "k_surface": ("k_surface", "surf_event"),
@@ -245,6 +364,8 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="with --config 3: keep the default launch options")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
                     help="transport option passed to omc_set_option (e.g. sort=0, xs_lanes=8)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -265,6 +386,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config == 3:
+        bench_problem_set(args)
+        return
     from openmc_amd import configs
     copy_info = None
     if args.copy_calibration > 0:

@@ -86,7 +86,12 @@ void openmc_set_stride(uint64_t new_stride);
 /* Options: "trace_hash" (per-particle history hash, 0/1), "learned_index"
  * (0/1), "max_in_flight" (particles), "event_timing" (per-kernel HIP event
  * timers, 0/1), "cycles_per_check" (event cycles between host checks),
- * "use_graph" (capture cycles in a hipGraph, 0/1). */
+ * "use_graph" (capture cycles in a hipGraph, 0/1).  Launch shape (none of
+ * them changes a history): "blocks" (workgroups of the event kernels, 0 =
+ * from the capacity), "block" (threads per workgroup: 64, 128 or 256),
+ * "sort" (0/1), "sort_bits" (energy buckets per binade = 2^sort_bits),
+ * "sort_every" (full material+energy sort every n cycles, material-only in
+ * between), "sort_blocks", "xs_lanes", "xs_group", "bank_cap". */
 int omc_set_option(const char* name, double value);
 /* Run statistics of the simulation so far (see omc_stat_names()). */
 int omc_get_stats(double* out, int n);
@@ -115,6 +120,23 @@ int omc_copy_kernel(int64_t n_bytes, int reps, double* ms);
 /* Elementwise primitives on the GPU for bitwise host/device checks: fn 0 log,
  * 1 sin, 2 cos, 3 sqrt, 4 x/y, 5 prn(seed = bits of x), 6 u64->f64 of bits of x. */
 int omc_math_batch(int fn, const double* x, const double* y, int64_t n, double* out);
+/* Problem sets: several independent fixed-source models (one directory
+ * each, read like openmc_init reads one) whose particles share one event loop
+ * (one population per batch, sorted and advanced together).  Each problem
+ * keeps its own particle ids, seeds, source, root universe, tally and
+ * leakage, so its histories and tallies are those of the problem run alone.
+ * The problems must share batches, seed, stride, the event limit and the
+ * energy range of their data; weight windows are not supported in a set.
+ * After omc_init_problems, openmc_run / openmc_next_batch run every problem;
+ * openmc_tally_results(index) reads any problem's tally, and
+ * omc_problem_info gives a problem's tally index. */
+int omc_init_problems(int n, const char* const* dirs);
+int omc_n_problems(void);
+int omc_problem_info(int problem, int64_t* n_particles, int64_t* first, int32_t* tally_index);
+int omc_problem_global_tallies(int problem, double** ptr);
+/* Host-only reading of a problem set (no GPU needed), for tests: counts and
+ * per-problem offsets (layout in capi.cpp). */
+int omc_check_problem_set(int n, const char* const* dirs, double* out, int n_out);
 /* Learned index: fit the per-nuclide piecewise-linear models on the GPU with
  * n_seg segments and `epochs` gradient steps; *max_err gets the largest
  * recorded error bound over all nuclides. */

@@ -14,7 +14,10 @@ Choices the benchmark text leaves open (recorded here and in the design notes):
     0.1 atom/b-cm so that 20 or 300 nuclides give a realistic total density;
   * config 1's source is a Watt spectrum sampled uniformly in the fuel
     cylinders: a box over the lattice with a cell-domain constraint;
-  * config 2's source is a Watt spectrum sampled uniformly in the cube.
+  * config 2's source is a Watt spectrum sampled uniformly in the cube;
+  * config 3's problems split their 10-60 nuclides into the three lattice
+    materials in thirds (in the order drawn), each material scaled to
+    0.1 atom/b-cm like config 1, 10 batches each.
 """
 import hashlib
 import json
@@ -219,6 +222,106 @@ def config4(n_particles=10_000_000, n_batches=16, n_grid=200000, n_layers=5, lay
         t.filters.append(om.EnergyFilter(list(np.geomspace(1.0e-5, 2.0e7, energy_groups + 1))))
     model = om.Model(geom, om.Materials(mats), s, om.Tallies([t]))
     return model, nucs
+
+
+def config3_problem(nucs, rng, lattice_n, n_nuc, n_particles, mesh_m, n_batches=10, half_height=50.0,
+                    event_based=True):
+    """One problem of the batched set: a lattice_n x lattice_n pin lattice like
+    config 1, its three materials sharing n_nuc nuclides drawn from the
+    config-1 set, a Watt source in the fuel and an m^3 flux mesh."""
+    om.reset_auto_ids()
+    pick = sorted(rng.choice(len(nucs), size=n_nuc, replace=False))
+    chosen = [nucs[k] for k in pick]
+    cut = [0, n_nuc // 3, 2 * (n_nuc // 3), n_nuc]
+    mats = []
+    for k, name in enumerate(["fuel", "clad", "moderator"]):
+        mt = om.Material(name=name)
+        for n in chosen[cut[k]:cut[k + 1]]:
+            mt.add_nuclide(n.name, float(rng.uniform(0.005, 0.05)))
+        mt.set_density("atom/b-cm", 0.1)
+        mats.append(mt)
+    pitch = 1.26
+    half = 0.5 * pitch * lattice_n
+    cyl1 = om.ZCylinder(r=0.4)
+    cyl2 = om.ZCylinder(r=0.47)
+    fuel = om.Cell(fill=mats[0], region=-cyl1, name="fuel")
+    clad = om.Cell(fill=mats[1], region=+cyl1 & -cyl2, name="clad")
+    mod = om.Cell(fill=mats[2], region=+cyl2, name="moderator")
+    pin = om.Universe(cells=[fuel, clad, mod])
+    lat = om.RectLattice()
+    lat.lower_left = (-half, -half)
+    lat.pitch = (pitch, pitch)
+    lat.universes = [[pin] * lattice_n for _ in range(lattice_n)]
+    xm = om.XPlane(-half, boundary_type="reflective")
+    xp = om.XPlane(half, boundary_type="reflective")
+    ym = om.YPlane(-half, boundary_type="reflective")
+    yp = om.YPlane(half, boundary_type="reflective")
+    zm = om.ZPlane(-half_height, boundary_type="vacuum")
+    zp = om.ZPlane(half_height, boundary_type="vacuum")
+    root = om.Cell(fill=lat, region=+xm & -xp & +ym & -yp & +zm & -zp)
+    geom = om.Geometry(om.Universe(cells=[root]))
+    s = om.Settings()
+    s.particles = int(n_particles)
+    s.batches = n_batches
+    s.event_based = event_based
+    s.source = om.IndependentSource(
+        space=stats.Box((-half, -half, -half_height), (half, half, half_height)),
+        angle=stats.Isotropic(), energy=stats.Watt(0.988e6, 2.249e-6),
+        constraints={"domains": [fuel]})
+    m = om.RegularMesh()
+    m.dimension = (mesh_m,) * 3
+    m.lower_left = (-half, -half, -half_height)
+    m.upper_right = (half, half, half_height)
+    t = om.Tally(name="flux")
+    t.filters = [om.MeshFilter(m)]
+    return om.Model(geom, om.Materials(mats), s, om.Tallies([t])), chosen
+
+
+def config3(n_problems=64, n_batches=10, particles=(1e4, 1e6), lattice=(3, 17), nuclides=(10, 60),
+            mesh=(10, 64), grid_lo=20000, grid_hi=100000, seed=DATA_SEED_BASE + 3):
+    """Batched uneven problems: n_problems lattice problems drawn from a seed
+    (lattice n x n with n uniform in `lattice`, 10-60 of the config-1
+    nuclides, particles per batch log-uniform in `particles`, an m^3 mesh with
+    m uniform in `mesh`).  Returns (list of (model, nuclides used), all nuclides)."""
+    nucs = _nuclide_set("L", 60, DATA_SEED_BASE + 1, grid_lo, grid_hi)
+    rng = np.random.default_rng(seed)
+    problems = []
+    for _ in range(n_problems):
+        n = int(rng.integers(lattice[0], lattice[1] + 1))
+        k = int(rng.integers(nuclides[0], nuclides[1] + 1))
+        npart = int(round(float(np.exp(rng.uniform(np.log(particles[0]), np.log(particles[1]))))))
+        m = int(rng.integers(mesh[0], mesh[1] + 1))
+        problems.append(config3_problem(nucs, rng, n, k, npart, m, n_batches=n_batches))
+    return problems, nucs
+
+
+def write_problem_set(problems, nuclides, directory):
+    """Write one shared nuclide library and one model directory per problem
+    (directory/p000, p001, ...); returns the problem directories."""
+    os.makedirs(directory, exist_ok=True)
+    xs = data.write_library(nuclides, os.path.join(directory, "data"))
+    dirs = []
+    for k, (model, _) in enumerate(problems):
+        d = os.path.join(directory, "p%03d" % k)
+        os.makedirs(d, exist_ok=True)
+        model.materials.cross_sections = os.path.relpath(xs, d)
+        model.export_to_model_xml(os.path.join(d, "model.xml"))
+        dirs.append(d)
+    return dirs
+
+
+def problem_set_dir(**kwargs):
+    """Build (or reuse from the cache) a config-3 problem set.
+    Returns (set directory, problem directories, problems, nuclides)."""
+    d = os.path.join(cache_root(), problem_key(3, **kwargs))
+    problems, nucs = config3(**kwargs)
+    stamp = os.path.join(d, "complete")
+    if not os.path.exists(stamp):
+        dirs = write_problem_set(problems, nucs, d)
+        with open(stamp, "w") as f:
+            f.write(json.dumps({"config": 3, "kwargs": kwargs, "problems": len(dirs)}, default=str))
+    dirs = [os.path.join(d, "p%03d" % k) for k in range(len(problems))]
+    return d, dirs, problems, nucs
 
 
 BUILDERS = {0: config0, 1: config1, 2: config2, 4: config4}

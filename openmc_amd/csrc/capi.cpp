@@ -95,10 +95,13 @@ struct Sim {
   int64_t cap = 0;
   int blocks = 0;
   double* d_results = nullptr;
-  std::vector<double> h_results;
+  std::vector<double> h_results; // [all tallies' bins][3]
+  int64_t total_bins = 0;
+  double* d_value = nullptr; // VALUE of every tally, each at HostTally::off
   bool results_dirty = false;
   int n_real = 0;
   std::vector<double> gt = std::vector<double>(3 * N_GLOBAL, 0.0);
+  std::vector<std::vector<double>> gt_part; // global tallies of each problem of a problem set
   double total_weight = 0;
   int* h_qcount = nullptr; // pinned
   std::vector<unsigned long long> h_counter_slots;
@@ -117,7 +120,9 @@ struct Sim {
   int xs_group_opt = 4;
   int bank_cap = 32; // secondary bank sites per particle (weight windows)
   int sort_bits = 5; // mantissa bits of E in the calculate-xs sort key (32 buckets per binade)
-  LaunchConfig lc {1, 1, 0, 1, 4};
+  int block_opt = 256;  // threads per workgroup of the event kernels
+  int sort_every = 1;   // full (material, energy) sort every this many cycles
+  LaunchConfig lc {1, 1, 0, 1, 4, 256, 1};
   int64_t max_in_flight_opt = 0;
   std::vector<int64_t> nuc_off; // first grid point of each nuclide
   void* matnuc_alloc = nullptr;
@@ -282,20 +287,40 @@ void upload_problem(Sim& s)
   P.univ_cells = dev_upload(s, m.univ_cells);
   P.lat = dev_upload(s, m.lattices);
   P.lat_univ = dev_upload(s, m.lat_univ);
-  P.root_universe = m.root_universe;
   P.n_levels = m.n_levels;
   P.n_surfaces = static_cast<int>(m.surfaces.size());
 
-  P.src = m.src;
-  P.src.disc_x = dev_upload(s, m.disc_x);
-  P.src.disc_prob = dev_upload(s, m.disc_prob);
-  P.src.disc_alias = dev_upload(s, m.disc_alias);
-  P.src.domain_ids = dev_upload(s, m.domain_ids);
-  P.src.n_domain = static_cast<int>(m.domain_ids.size());
+  // A single model is a problem set of one problem.
+  if (m.parts.empty()) {
+    HostPart part;
+    part.n_particles = m.n_particles;
+    part.first = 0;
+    part.root_universe = m.root_universe;
+    part.tally = m.tallies.empty() ? -1 : 0;
+    part.n_nuclides = static_cast<int>(m.nuclides.size());
+    part.src = m.src;
+    part.disc_x = m.disc_x;
+    part.disc_p = m.disc_p;
+    part.disc_prob = m.disc_prob;
+    part.disc_alias = m.disc_alias;
+    part.domain_ids = m.domain_ids;
+    m.parts.push_back(std::move(part));
+  }
 
-  if (!m.tallies.empty()) {
-    const HostTally& t = m.tallies[0];
-    DevTally& T = P.tally;
+  // Tallies: one VALUE array for all of them, each at its own offset
+  int64_t total_bins = 0;
+  for (HostTally& t : m.tallies) {
+    t.off = total_bins;
+    total_bins += t.n_filter_bins;
+  }
+  double* value = nullptr;
+  if (total_bins > 0) {
+    value = dev_alloc<double>(s.dev_allocs, total_bins);
+    s.d_results = dev_alloc<double>(s.dev_allocs, 3 * total_bins);
+    s.h_results.assign(3 * total_bins, 0.0);
+  }
+  auto dev_tally = [&](const HostTally& t) {
+    DevTally T {};
     T.enabled = 0; // switched on for active batches
     T.n_dim = t.n_dim;
     for (int k = 0; k < 3; ++k) {
@@ -306,10 +331,37 @@ void upload_problem(Sim& s)
     T.n_energy = t.ebins.empty() ? 0 : static_cast<int>(t.ebins.size()) - 1;
     T.ebins = dev_upload(s, t.ebins);
     T.n_bins = t.n_filter_bins;
-    T.value = dev_alloc<double>(s.dev_allocs, T.n_bins);
-    s.d_results = dev_alloc<double>(s.dev_allocs, 3 * T.n_bins);
-    s.h_results.assign(3 * T.n_bins, 0.0);
+    T.value = value + t.off;
+    return T;
+  };
+
+  std::vector<DevPart> parts;
+  for (const HostPart& hp : m.parts) {
+    DevPart d {};
+    d.src = hp.src;
+    d.src.disc_x = dev_upload(s, hp.disc_x);
+    d.src.disc_prob = dev_upload(s, hp.disc_prob);
+    d.src.disc_alias = dev_upload(s, hp.disc_alias);
+    d.src.domain_ids = dev_upload(s, hp.domain_ids);
+    d.src.n_domain = static_cast<int>(hp.domain_ids.size());
+    if (hp.tally >= 0)
+      d.tally = dev_tally(m.tallies[hp.tally]);
+    d.first = hp.first;
+    d.n_particles = hp.n_particles;
+    d.root_universe = hp.root_universe;
+    d.mat_begin = hp.mat_begin;
+    d.surf_begin = hp.surf_begin;
+    d.cell_begin = hp.cell_begin;
+    parts.push_back(d);
   }
+  P.src = parts[0].src;
+  P.tally = parts[0].tally;
+  P.root_universe = parts[0].root_universe;
+  P.part = dev_upload(s, parts);
+  P.n_parts = static_cast<int>(parts.size());
+  P.mat_nuc_local = P.n_parts > 1 ? dev_upload(s, m.mat_nuc_local) : nullptr;
+  s.total_bins = total_bins;
+  s.d_value = value;
 
   if (m.ww.on) {
     const HostWW& w = m.ww;
@@ -355,6 +407,15 @@ void update_launch_config(Sim& s)
   s.lc.sort = s.sort ? 1 : 0;
   s.lc.sort_blocks = std::min(s.lc.blocks, s.sort_blocks_opt > 0 ? s.sort_blocks_opt : 256);
   s.lc.xs_group = s.xs_group_opt;
+  s.lc.block = s.block_opt;
+  s.lc.sort_every = s.sort_every;
+}
+
+// Leakage partial sums: CNT_SLOTS for a single model, LEAK_PART_SLOTS per
+// problem in a problem set.
+size_t leak_slots(const Sim& s)
+{
+  return s.P.n_parts > 1 ? static_cast<size_t>(s.P.n_parts) * LEAK_PART_SLOTS : CNT_SLOTS;
 }
 
 void free_particles(Sim& s)
@@ -378,7 +439,7 @@ void alloc_particles(Sim& s, int64_t cap)
   S.micro_t = dev_alloc<double>(L, static_cast<size_t>(S.max_nn) * cap);
   S.lost = dev_alloc<int>(L, cap);
   S.hash = dev_alloc<uint64_t>(L, cap);
-  S.leak = dev_alloc<double>(L, CNT_SLOTS);
+  S.leak = dev_alloc<double>(L, leak_slots(s));
   S.total_weight = dev_alloc<double>(L, CNT_SLOTS);
   S.status = dev_alloc<int>(L, 1);
   S.bank_cap = s.bank_cap;
@@ -397,7 +458,7 @@ void alloc_particles(Sim& s, int64_t cap)
   S.qcount = dev_alloc<int>(L, QC_N);
   S.counters = dev_alloc<unsigned long long>(L, static_cast<size_t>(CNT_N) * CNT_SLOTS);
   s.h_counter_slots.assign(static_cast<size_t>(CNT_N) * CNT_SLOTS, 0);
-  s.h_leak_slots.assign(CNT_SLOTS, 0.0);
+  s.h_leak_slots.assign(std::max<size_t>(CNT_SLOTS, leak_slots(s)), 0.0);
   // calculate-xs keys (material, energy bucket): the bucket is the sign,
   // exponent and top sort_bits mantissa bits of E, offset by those of
   // energy_min; 64 binades of 2^sort_bits buckets each (32 per binade by
@@ -491,7 +552,7 @@ void run_cycles(Sim& s, int n_cycles)
     for (int c = 0; c < n_cycles; ++c) {
       for (int k = KERNEL_SORT; k <= KERNEL_COLLISION; ++k) {
         record_kernel(s, k, true);
-        launch_kernel(k, s.P, s.S, s.lc, s.stream);
+        launch_kernel(k, s.P, s.S, s.lc, s.stream, c);
         record_kernel(s, k, false);
       }
     }
@@ -500,7 +561,7 @@ void run_cycles(Sim& s, int n_cycles)
       destroy_graph(s);
       hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       for (int c = 0; c < s.cycles_per_check; ++c)
-        launch_cycle(s.P, s.S, s.lc, s.stream);
+        launch_cycle(s.P, s.S, s.lc, s.stream, c);
       hip_check(hipStreamEndCapture(s.stream, &s.graph), "hipStreamEndCapture");
       hip_check(hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
       s.graph_valid = true;
@@ -508,7 +569,7 @@ void run_cycles(Sim& s, int n_cycles)
     hip_check(hipGraphLaunch(s.graph_exec, s.stream), "hipGraphLaunch");
   } else {
     for (int c = 0; c < n_cycles; ++c)
-      launch_cycle(s.P, s.S, s.lc, s.stream);
+      launch_cycle(s.P, s.S, s.lc, s.stream, c);
   }
   s.n_cycles += n_cycles;
 }
@@ -526,7 +587,7 @@ void transport_batch(Sim& s)
   s.S.use_learned = (s.use_learned && s.P.learned.enabled) ? 1 : 0;
   if (s.trace_hash)
     s.h_hashes.assign(m.n_particles, 0);
-  hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double) * CNT_SLOTS, s.stream), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double) * leak_slots(s), s.stream), "hipMemsetAsync");
   hip_check(hipMemsetAsync(s.S.total_weight, 0, sizeof(double) * CNT_SLOTS, s.stream), "hipMemsetAsync");
   hip_check(hipMemsetAsync(s.S.status, 0, sizeof(int), s.stream), "hipMemsetAsync");
 
@@ -540,10 +601,9 @@ void transport_batch(Sim& s)
   int64_t offset = 0;
   while (remaining > 0) {
     int64_t n = std::min(remaining, s.cap);
-    int64_t id_base = (static_cast<int64_t>(current_batch) - 1) * m.n_particles + offset;
     if (s.event_timing)
       record_kernel(s, KERNEL_INIT, true);
-    launch_init(s.P, s.S, n, id_base, s.stream);
+    launch_init(s.P, s.S, n, offset, current_batch, s.stream);
     if (s.event_timing)
       record_kernel(s, KERNEL_INIT, false);
     // k_init staged the n source particles (key -1 for any it lost)
@@ -579,20 +639,28 @@ void transport_batch(Sim& s)
     offset += n;
   }
   hip_check(hipEventRecord(e1, s.stream), "hipEventRecord");
-  hip_check(hipMemcpyAsync(s.h_leak_slots.data(), s.S.leak, sizeof(double) * CNT_SLOTS, hipMemcpyDeviceToHost,
+  hip_check(hipMemcpyAsync(s.h_leak_slots.data(), s.S.leak, sizeof(double) * leak_slots(s), hipMemcpyDeviceToHost,
               s.stream),
     "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  const bool set = s.P.n_parts > 1;
+  std::vector<double> leak_part(m.parts.size(), 0.0);
   double leak = 0.0;
-  for (double v : s.h_leak_slots)
-    leak += v;
+  if (set) {
+    for (size_t k = 0; k < m.parts.size(); ++k)
+      for (int j = 0; j < LEAK_PART_SLOTS; ++j)
+        leak_part[k] += s.h_leak_slots[k * LEAK_PART_SLOTS + j];
+  } else {
+    for (size_t j = 0; j < CNT_SLOTS; ++j)
+      leak += s.h_leak_slots[j];
+  }
   // simulation::total_weight: the source weight after the birth weight
   // windows (src/simulation.cpp:637-639)
   hip_check(hipMemcpy(s.h_leak_slots.data(), s.S.total_weight, sizeof(double) * CNT_SLOTS, hipMemcpyDeviceToHost),
     "hipMemcpy");
   double total_weight = 0.0;
-  for (double v : s.h_leak_slots)
-    total_weight += v;
+  for (size_t j = 0; j < CNT_SLOTS; ++j)
+    total_weight += s.h_leak_slots[j];
   int status = 0;
   hip_check(hipMemcpy(&status, s.S.status, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
   if (status & 1)
@@ -611,16 +679,32 @@ void transport_batch(Sim& s)
 
   s.total_weight = total_weight;
 
-  // accumulate_tallies (src/tallies/tally.cpp:1059-1104)
+  // accumulate_tallies (src/tallies/tally.cpp:1059-1104), per problem in a
+  // problem set (each has its own source weight: no weight windows in a set,
+  // so it is the problem's particle count)
   if (active) {
     n_realizations += 1;
-    double val = leak / s.total_weight;
-    s.gt[3 * GT_LEAKAGE + 1] += val;
-    s.gt[3 * GT_LEAKAGE + 2] += val * val;
+    if (set) {
+      if (s.gt_part.size() != m.parts.size())
+        s.gt_part.assign(m.parts.size(), std::vector<double>(3 * N_GLOBAL, 0.0));
+      for (size_t k = 0; k < m.parts.size(); ++k) {
+        double val = leak_part[k] / static_cast<double>(m.parts[k].n_particles);
+        s.gt_part[k][3 * GT_LEAKAGE + 1] += val;
+        s.gt_part[k][3 * GT_LEAKAGE + 2] += val * val;
+      }
+      s.gt = s.gt_part[0];
+    } else {
+      double val = leak / s.total_weight;
+      s.gt[3 * GT_LEAKAGE + 1] += val;
+      s.gt[3 * GT_LEAKAGE + 2] += val * val;
+    }
     if (!m.tallies.empty()) {
       double total_source = 1.0;
-      double norm = total_source / (static_cast<double>(m.n_particles) * 1);
-      launch_accumulate(s.P.tally.value, s.d_results, s.P.tally.n_bins, norm, s.stream);
+      for (const HostTally& t : m.tallies) {
+        const double n_part = static_cast<double>(set ? m.parts[t.part].n_particles : m.n_particles);
+        double norm = total_source / (n_part * 1);
+        launch_accumulate(s.d_value + t.off, s.d_results + 3 * t.off, t.n_filter_bins, norm, s.stream);
+      }
       s.n_real++;
       s.results_dirty = true;
     }
@@ -645,6 +729,28 @@ void sync_results(Sim& s)
   hip_check(hipMemcpy(s.h_results.data(), s.d_results, sizeof(double) * s.h_results.size(), hipMemcpyDeviceToHost),
     "hipMemcpy");
   s.results_dirty = false;
+}
+
+// Device set-up shared by openmc_init and omc_init_problems.
+void finish_init(Sim& s)
+{
+  initialize_data(s.m);
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+    throw std::runtime_error("no GPU visible: the MI355X transport path needs a HIP device");
+  hip_check(hipSetDevice(0), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&s.h_qcount), sizeof(int) * QC_N, hipHostMallocDefault),
+    "hipHostMalloc");
+  upload_problem(s);
+  if (s.m.event_based)
+    event_based = true;
+  n_particles = s.m.n_particles;
+  n_inactive = s.m.n_inactive;
+  n_coord_levels = s.m.n_levels;
+  current_batch = 0;
+  n_realizations = 0;
+  s.initialized = true;
 }
 
 template<class F>
@@ -705,26 +811,33 @@ int openmc_init(int argc, char* argv[], const void* intracomm)
     delete g;
     g = new Sim();
     read_model(g->m, dir);
-    initialize_data(g->m);
-    int dev_count = 0;
-    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
-      throw std::runtime_error("no GPU visible: the MI355X transport path needs a HIP device");
-    hip_check(hipSetDevice(0), "hipSetDevice");
-    hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&g->h_qcount), sizeof(int) * QC_N, hipHostMallocDefault),
-      "hipHostMalloc");
-    upload_problem(*g);
-    if (g->m.event_based)
-      event_based = true;
-    n_particles = g->m.n_particles;
-    n_inactive = g->m.n_inactive;
-    n_coord_levels = g->m.n_levels;
-    current_batch = 0;
-    n_realizations = 0;
-    g->initialized = true;
+    finish_init(*g);
     return 0;
   });
 }
+
+// A problem set: several independent models (one directory each) whose
+// particles share one event loop.  No reference counterpart (the reference
+// runs one model per process); the callers are batch drivers that run many
+// small fixed-source problems.
+int omc_init_problems(int n, const char* const* dirs)
+{
+  return guarded([&]() -> int {
+    if (g && g->initialized)
+      openmc_finalize();
+    if (n < 1 || !dirs)
+      throw std::runtime_error("a problem set needs at least one directory");
+    std::vector<std::string> d;
+    for (int k = 0; k < n; ++k)
+      d.push_back(dirs[k] ? dirs[k] : ".");
+    delete g;
+    g = new Sim();
+    read_problem_set(g->m, d);
+    finish_init(*g);
+    return 0;
+  });
+}
+
 
 int openmc_simulation_init(void)
 {
@@ -735,8 +848,14 @@ int openmc_simulation_init(void)
   return guarded([&]() -> int {
     if (g->sim_initialized)
       return 0;
-    g->m.n_particles = n_particles;
-    g->P.n_particles = n_particles;
+    if (g->P.n_parts > 1) {
+      if (n_particles != g->m.n_particles)
+        throw std::runtime_error("the particle counts of a problem set come from its problems' settings");
+    } else {
+      g->m.n_particles = n_particles;
+      g->m.parts[0].n_particles = n_particles;
+      g->P.n_particles = n_particles;
+    }
     choose_capacity(*g);
     current_batch = 0;
     g->sim_initialized = true;
@@ -803,6 +922,8 @@ int openmc_reset(void)
       std::fill(g->h_results.begin(), g->h_results.end(), 0.0);
     }
     std::fill(g->gt.begin(), g->gt.end(), 0.0);
+    for (auto& v : g->gt_part)
+      std::fill(v.begin(), v.end(), 0.0);
     g->n_real = 0;
     n_realizations = 0;
     g->results_dirty = false;
@@ -927,7 +1048,7 @@ int openmc_tally_results(int32_t index, double** results, size_t* shape)
   }
   return guarded([&]() -> int {
     sync_results(*g);
-    *results = g->h_results.data();
+    *results = g->h_results.data() + 3 * g->m.tallies[index].off;
     shape[0] = static_cast<size_t>(g->m.tallies[index].n_filter_bins);
     shape[1] = 1;
     shape[2] = 3;
@@ -946,6 +1067,43 @@ int openmc_global_tallies(double** ptr)
 size_t tallies_size(void)
 {
   return g ? g->m.tallies.size() : 0;
+}
+
+int omc_n_problems(void)
+{
+  return g && g->initialized ? static_cast<int>(g->m.parts.size()) : 0;
+}
+
+int omc_problem_info(int problem, int64_t* n_particles_out, int64_t* first, int32_t* tally_index)
+{
+  if (!g || !g->initialized || problem < 0 || problem >= static_cast<int>(g->m.parts.size())) {
+    set_errmsg("Index in problems array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  const HostPart& p = g->m.parts[problem];
+  if (n_particles_out)
+    *n_particles_out = p.n_particles;
+  if (first)
+    *first = p.first;
+  if (tally_index)
+    *tally_index = p.tally;
+  return 0;
+}
+
+int omc_problem_global_tallies(int problem, double** ptr)
+{
+  if (!g || !g->initialized || problem < 0 || problem >= static_cast<int>(g->m.parts.size())) {
+    set_errmsg("Index in problems array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  if (g->P.n_parts <= 1) {
+    *ptr = g->gt.data();
+    return 0;
+  }
+  if (g->gt_part.size() != g->m.parts.size())
+    g->gt_part.assign(g->m.parts.size(), std::vector<double>(3 * N_GLOBAL, 0.0));
+  *ptr = g->gt_part[problem].data();
+  return 0;
 }
 
 int64_t openmc_get_seed(void)
@@ -1029,6 +1187,23 @@ int omc_set_option(const char* name, double value)
     }
     g->xs_group_opt = v;
     update_launch_config(*g);
+  } else if (n == "block") {
+    int v = static_cast<int>(value);
+    if (v != 64 && v != 128 && v != 256) {
+      set_errmsg("block must be 64, 128 or 256 (threads per workgroup of the event kernels; their launch bound "
+                 "is 256 so that they keep up to 256 VGPRs)");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    g->block_opt = v;
+    update_launch_config(*g);
+  } else if (n == "sort_every") {
+    int v = static_cast<int>(value);
+    if (v < 1 || v > 1024) {
+      set_errmsg("sort_every must be in [1, 1024]");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    g->sort_every = v;
+    update_launch_config(*g);
   } else if (n == "sort_blocks" || n == "blocks") {
     int v = static_cast<int>(value);
     if (v < 0 || v > 65536) {
@@ -1054,7 +1229,8 @@ int omc_set_option(const char* name, double value)
 static const char* k_stat_names =
   "transport_ms,active_batches,lookups,collisions,segments,lost,xs_events,adv_events,surf_events,"
   "coll_events,search_probes,cycles,checks,capacity,t_sort_ms,t_xs_ms,t_adv_ms,t_split_ms,t_surf_ms,"
-  "t_coll_ms,t_init_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,wall_s,blocks,xs_lanes,sort,xs_group";
+  "t_coll_ms,t_init_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,wall_s,blocks,xs_lanes,sort,xs_group,"
+  "block,sort_every,cycles_per_check,sort_bits,sort_blocks";
 
 const char* omc_stat_names(void)
 {
@@ -1089,6 +1265,11 @@ int omc_get_stats(double* out, int n)
   v.push_back(g->lc.xs_lanes);
   v.push_back(g->lc.sort);
   v.push_back(g->lc.xs_group);
+  v.push_back(g->lc.block);
+  v.push_back(g->lc.sort_every);
+  v.push_back(g->cycles_per_check);
+  v.push_back(g->sort_bits);
+  v.push_back(g->lc.sort_blocks);
   for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
     out[k] = v[k];
   return static_cast<int>(v.size());
@@ -1216,6 +1397,46 @@ int omc_check_model(const char* dir, double* out, int n_out, int* grid_index_out
     if (grid_index_out)
       for (int64_t k = 0; k < n_gi && k < static_cast<int64_t>(m.grid_index.size()); ++k)
         grid_index_out[k] = m.grid_index[k];
+    return static_cast<int>(v.size());
+  });
+}
+
+// Host-side reading of a problem set, without a GPU (for tests):
+// out = {nuclides, materials, cells, surfaces, universes, lattices, levels,
+//        total particles, tallies, energy_min bits as double, energy_max}
+// followed by 8 values per problem {first, n_particles, root universe,
+// mat_begin, surf_begin, cell_begin, tally index, the tally's first bin}.
+int omc_check_problem_set(int n, const char* const* dirs, double* out, int n_out)
+{
+  return guarded([&]() -> int {
+    std::vector<std::string> d;
+    for (int k = 0; k < n; ++k)
+      d.push_back(dirs[k] ? dirs[k] : ".");
+    HostModel m;
+    read_problem_set(m, d);
+    initialize_data(m);
+    int64_t off = 0;
+    for (HostTally& t : m.tallies) {
+      t.off = off;
+      off += t.n_filter_bins;
+    }
+    std::vector<double> v = {static_cast<double>(m.nuclides.size()), static_cast<double>(m.mat_ids.size()),
+      static_cast<double>(m.cells.size()), static_cast<double>(m.surfaces.size()),
+      static_cast<double>(m.universes.size()), static_cast<double>(m.lattices.size()),
+      static_cast<double>(m.n_levels), static_cast<double>(m.n_particles), static_cast<double>(m.tallies.size()),
+      m.energy_min, m.energy_max};
+    for (const HostPart& p : m.parts) {
+      v.push_back(static_cast<double>(p.first));
+      v.push_back(static_cast<double>(p.n_particles));
+      v.push_back(p.root_universe);
+      v.push_back(p.mat_begin);
+      v.push_back(p.surf_begin);
+      v.push_back(p.cell_begin);
+      v.push_back(p.tally);
+      v.push_back(p.tally >= 0 ? static_cast<double>(m.tallies[p.tally].off) : -1.0);
+    }
+    for (int k = 0; k < n_out && k < static_cast<int>(v.size()); ++k)
+      out[k] = v[k];
     return static_cast<int>(v.size());
   });
 }

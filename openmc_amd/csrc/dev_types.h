@@ -60,7 +60,7 @@ struct DevCell {
   int tok_begin;
   int tok_end;
   int id;
-  int pad;
+  int part; // problem of a problem set (0 for a single model)
 };
 
 struct DevUniverse {
@@ -171,6 +171,20 @@ struct DevWW {
   int64_t max_history_splits;
 };
 
+// One problem of a problem set: several independent models whose particles
+// share one event loop.  A cell index names its problem (DevCell::part);
+// what is not reached through a cell is here.
+struct DevPart {
+  DevSource src;
+  DevTally tally;      // value points at the problem's slice (n_bins 0 without a tally)
+  int64_t first;       // first population index of the problem in a batch
+  int64_t n_particles; // per batch
+  int root_universe;
+  int mat_begin;  // first material, surface and cell of the problem (history
+  int surf_begin; // hashes use problem-local indices so they equal the hashes
+  int cell_begin; // of the problem run alone)
+};
+
 struct DevProblem {
   DevWW ww;
   // cross sections
@@ -209,6 +223,13 @@ struct DevProblem {
   DevSource src;
   DevTally tally;
 
+  // problem set: n_parts > 1 switches the kernels to per-problem sources,
+  // root universes, tallies and leakage (a single model has n_parts == 1)
+  const DevPart* part;
+  const int* mat_nuc_local; // problem-local nuclide index of each mat_nuc entry
+  int n_parts;
+  int pad_parts;
+
   // settings
   int64_t n_particles; // per batch (id arithmetic)
   int64_t max_events;
@@ -243,6 +264,8 @@ enum QueueCount : int { QC_XS = 0, QC_STAGE, QC_SURF, QC_COLL, QC_N };
 // adds into slot wave_id % CNT_SLOTS) so no address sees more than a few
 // atomics per launch; the host sums the slots.
 constexpr int CNT_SLOTS = 2048;
+// In a problem set each problem's leakage has this many slots of its own.
+constexpr int LEAK_PART_SLOTS = 16;
 
 // Hot state of one particle: exactly one 128-byte L2 cache line, so an event
 // kernel touching a particle in any queue order moves one line, not one line
@@ -313,7 +336,7 @@ struct DevParticles {
   int max_nn;
   int* lost;
   uint64_t* hash;
-  double* leak; // [CNT_SLOTS] per-batch leakage partial sums
+  double* leak; // [CNT_SLOTS] per-batch leakage partial sums ([n_parts][LEAK_PART_SLOTS] in a problem set)
   // Queues.  No kernel appends to a shared counter: an event kernel writes
   // its result for input i at position i of a staging array (the slot and a
   // key, -1 for a particle that is done) and a partition pass compacts the

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <climits>
 #include <fstream>
 #include <functional>
 #include <stdexcept>
@@ -33,6 +34,16 @@ bool file_exists(const std::string& p)
 {
   std::ifstream f(p);
   return static_cast<bool>(f);
+}
+
+// The canonical form of an existing path (so the problems of a problem set
+// that reach one data file by different relative paths share it)
+std::string canonical_path(const std::string& p)
+{
+  char buf[PATH_MAX];
+  if (::realpath(p.c_str(), buf))
+    return std::string(buf);
+  return p;
 }
 
 // Nuclide file reader; derived columns as Nuclide::create_derived
@@ -357,7 +368,7 @@ void read_settings(HostModel& m, const XmlNode& s)
   }
 }
 
-void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_dir)
+void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_dir, const HostModel* pool)
 {
   std::string xs_path;
   if (mats.has("cross_sections"))
@@ -431,7 +442,7 @@ void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_
     std::string type = ln->has("type") ? ln->value("type") : "neutron";
     if (type != "neutron")
       continue;
-    std::string p = join_path(lib_dir, ln->value("path"));
+    std::string p = canonical_path(join_path(lib_dir, ln->value("path")));
     for (const auto& name : split_ws(ln->value("materials")))
       paths[name] = p;
   }
@@ -440,8 +451,20 @@ void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_
     auto it = paths.find(kv.first);
     if (it == paths.end())
       throw std::runtime_error("nuclide " + kv.first + " is not in the cross section library");
-    read_nuclide_file(m.nuclides[kv.second], it->second);
-    m.nuclides[kv.second].name = kv.first;
+    HostNuclide& nuc = m.nuclides[kv.second];
+    if (pool) {
+      auto pt = pool->nuclide_map.find(kv.first);
+      if (pt != pool->nuclide_map.end() && pool->nuclides[pt->second].path == it->second) {
+        const HostNuclide& src = pool->nuclides[pt->second];
+        nuc.awr = src.awr;
+        nuc.kT = src.kT;
+        nuc.pooled = pt->second;
+      }
+    }
+    if (nuc.pooled < 0)
+      read_nuclide_file(nuc, it->second);
+    nuc.name = kv.first;
+    nuc.path = it->second;
   }
 
   // Material::normalize_density (src/material.cpp:417-467)
@@ -815,7 +838,7 @@ void read_tallies(HostModel& m, const XmlNode& t)
 
 } // namespace
 
-void read_model(HostModel& m, const std::string& dir)
+void read_model(HostModel& m, const std::string& dir, const HostModel* pool)
 {
   std::string model_path = join_path(dir, "model.xml");
   std::unique_ptr<XmlNode> model_doc, set_doc, mat_doc, geo_doc, tal_doc;
@@ -841,7 +864,7 @@ void read_model(HostModel& m, const std::string& dir)
   if (!settings || !materials || !geometry)
     throw std::runtime_error("model needs <settings>, <materials> and <geometry>");
   read_settings(m, *settings);
-  read_materials(m, *materials, dir);
+  read_materials(m, *materials, dir, pool);
   read_geometry(m, *geometry);
   if (tallies)
     read_tallies(m, *tallies);
@@ -880,6 +903,170 @@ void initialize_data(HostModel& m)
   for (double e : m.disc_x)
     if (e > m.energy_max)
       throw std::runtime_error("Source energy above range of energies of at least one cross section table");
+  for (const HostPart& part : m.parts)
+    for (double e : part.disc_x)
+      if (e > m.energy_max)
+        throw std::runtime_error("Source energy above range of energies of at least one cross section table");
+}
+
+void read_problem_set(HostModel& out, const std::vector<std::string>& dirs)
+{
+  if (dirs.empty())
+    throw std::runtime_error("a problem set needs at least one problem");
+  out = HostModel();
+  out.n_particles = 0;
+  out.n_levels = 1;
+  HostModel a; // the settings of the first problem
+  for (size_t k = 0; k < dirs.size(); ++k) {
+    HostModel m;
+    read_model(m, dirs[k], &out);
+    if (k == 0) {
+      a.n_batches = out.n_batches = m.n_batches;
+      a.n_inactive = out.n_inactive = m.n_inactive;
+      a.seed = out.seed = m.seed;
+      a.stride = out.stride = m.stride;
+      a.n_log_bins = out.n_log_bins = m.n_log_bins;
+      a.max_events = out.max_events = m.max_events;
+      a.free_gas_threshold = out.free_gas_threshold = m.free_gas_threshold;
+      a.energy_cutoff = out.energy_cutoff = m.energy_cutoff;
+      a.run_mode = out.run_mode = m.run_mode;
+      out.event_based = m.event_based;
+    }
+    const std::string where = "problem " + std::to_string(k) + " (" + dirs[k] + ")";
+    if (m.n_batches != a.n_batches || m.n_inactive != a.n_inactive || m.seed != a.seed || m.stride != a.stride ||
+        m.n_log_bins != a.n_log_bins || m.max_events != a.max_events ||
+        m.free_gas_threshold != a.free_gas_threshold || m.energy_cutoff != a.energy_cutoff ||
+        m.run_mode != a.run_mode)
+      throw std::runtime_error(where + ": the problems of a set must share batches, inactive batches, seed, "
+                                       "stride, log-grid bins, the event limit, free-gas threshold and energy cutoff");
+    if (m.ww.on)
+      throw std::runtime_error(where + ": weight windows are not supported in a problem set");
+    if (m.tallies.size() > 1)
+      throw std::runtime_error(where + ": one tally per problem");
+    if (m.n_particles <= 0)
+      throw std::runtime_error(where + ": no particles");
+
+    HostPart part;
+    part.dir = dirs[k];
+    part.n_particles = m.n_particles;
+    part.first = out.n_particles;
+    out.n_particles += m.n_particles;
+    part.n_nuclides = static_cast<int>(m.nuclides.size());
+
+    // nuclides: one copy per data file across the set
+    std::vector<int> nuc_map(m.nuclides.size());
+    part.energy_min = 0.0;
+    part.energy_max = INFTY;
+    for (size_t j = 0; j < m.nuclides.size(); ++j) {
+      HostNuclide& n = m.nuclides[j];
+      int idx = n.pooled;
+      if (idx < 0) {
+        auto it = out.nuclide_map.find(n.name);
+        if (it != out.nuclide_map.end()) {
+          if (out.nuclides[it->second].path != n.path)
+            throw std::runtime_error(where + ": nuclide " + n.name + " is read from " + n.path +
+                                     " but another problem read it from " + out.nuclides[it->second].path);
+          idx = it->second;
+        } else {
+          idx = static_cast<int>(out.nuclides.size());
+          out.nuclide_map[n.name] = idx;
+          out.nuclides.push_back(std::move(n));
+        }
+      }
+      nuc_map[j] = idx;
+      const HostNuclide& d = out.nuclides[idx];
+      part.energy_min = std::max(part.energy_min, d.energy.front());
+      part.energy_max = std::min(part.energy_max, d.energy.back());
+    }
+
+    // materials
+    part.mat_begin = static_cast<int>(out.mat_ids.size());
+    if (out.mat_off.empty())
+      out.mat_off.push_back(0);
+    for (size_t i = 0; i < m.mat_ids.size(); ++i) {
+      out.mat_ids.push_back(m.mat_ids[i]);
+      for (int e = m.mat_off[i]; e < m.mat_off[i + 1]; ++e) {
+        out.mat_nuc.push_back(nuc_map[m.mat_nuc[e]]);
+        out.mat_nuc_local.push_back(m.mat_nuc[e]);
+        out.mat_dens.push_back(m.mat_dens[e]);
+      }
+      out.mat_off.push_back(static_cast<int>(out.mat_nuc.size()));
+    }
+
+    // geometry: every index shifted by what the earlier problems hold
+    const int surf0 = static_cast<int>(out.surfaces.size());
+    const int cell0 = static_cast<int>(out.cells.size());
+    const int univ0 = static_cast<int>(out.universes.size());
+    const int lat0 = static_cast<int>(out.lattices.size());
+    const int tok0 = static_cast<int>(out.tokens.size());
+    const int ucell0 = static_cast<int>(out.univ_cells.size());
+    const int latu0 = static_cast<int>(out.lat_univ.size());
+    part.surf_begin = surf0;
+    part.cell_begin = cell0;
+    out.surf_ids.insert(out.surf_ids.end(), m.surf_ids.begin(), m.surf_ids.end());
+    out.surfaces.insert(out.surfaces.end(), m.surfaces.begin(), m.surfaces.end());
+    for (int t : m.tokens)
+      out.tokens.push_back(t > 0 ? t + surf0 : t - surf0);
+    for (DevCell c : m.cells) {
+      c.universe += univ0;
+      if (c.fill_type == FILL_MATERIAL) {
+        if (c.material >= 0)
+          c.material += part.mat_begin;
+      } else if (c.fill_type == FILL_UNIVERSE) {
+        c.fill += univ0;
+      } else {
+        c.fill += lat0;
+      }
+      c.tok_begin += tok0;
+      c.tok_end += tok0;
+      c.part = static_cast<int>(k);
+      out.cells.push_back(c);
+    }
+    out.univ_ids.insert(out.univ_ids.end(), m.univ_ids.begin(), m.univ_ids.end());
+    for (DevUniverse u : m.universes) {
+      u.cell_begin += ucell0;
+      u.cell_end += ucell0;
+      out.universes.push_back(u);
+    }
+    for (int c : m.univ_cells)
+      out.univ_cells.push_back(c + cell0);
+    out.lat_ids.insert(out.lat_ids.end(), m.lat_ids.begin(), m.lat_ids.end());
+    for (DevLattice L : m.lattices) {
+      L.univ_offset += latu0;
+      if (L.outer != C_NONE)
+        L.outer += univ0;
+      out.lattices.push_back(L);
+    }
+    for (int u : m.lat_univ)
+      out.lat_univ.push_back(u == C_NONE ? C_NONE : u + univ0);
+    part.root_universe = m.root_universe + univ0;
+    out.n_levels = std::max(out.n_levels, m.n_levels);
+
+    // source
+    part.src = m.src;
+    part.disc_x = m.disc_x;
+    part.disc_p = m.disc_p;
+    part.disc_prob = m.disc_prob;
+    part.disc_alias = m.disc_alias;
+    part.domain_ids = m.domain_ids;
+
+    // tally
+    if (!m.tallies.empty()) {
+      part.tally = static_cast<int>(out.tallies.size());
+      HostTally t = m.tallies[0];
+      t.part = static_cast<int>(k);
+      out.tallies.push_back(t);
+    }
+    out.parts.push_back(std::move(part));
+  }
+  // one energy range for the whole set: the log-grid buckets, the sort keys
+  // and the source's upper energy limit all use it
+  for (const HostPart& p : out.parts)
+    if (p.energy_min != out.parts[0].energy_min || p.energy_max != out.parts[0].energy_max)
+      throw std::runtime_error("problem " + p.dir + ": the problems of a set must have the same energy range "
+                               "of their nuclide data (E_min, E_max)");
+  out.root_universe = out.parts[0].root_universe;
+  out.src = out.parts[0].src;
 }
 
 } // namespace omc

@@ -16,9 +16,14 @@ namespace omc {
 
 struct HostNuclide {
   std::string name;
+  std::string path; // the data file it was read from
   double awr = 0;
   double kT = 0;
   std::vector<double> energy, total, absorption, elastic;
+  // Set when a problem of a problem set names a nuclide the set has already
+  // loaded from the same file: the index of that nuclide in the pool, whose
+  // data this entry does not hold.
+  int pooled = -1;
 };
 
 struct HostTally {
@@ -34,6 +39,8 @@ struct HostTally {
   std::vector<int> filter_nbins;
   int64_t n_filter_bins = 1;
   std::string estimator = "tracklength";
+  int64_t off = 0; // first bin in the concatenated VALUE array of all tallies
+  int part = 0;    // problem of a problem set that owns the tally
 };
 
 // A RegularMesh (src/mesh.cpp: RegularMesh::set_grid)
@@ -58,6 +65,24 @@ struct HostWW {
   bool ckpt_collision = true;
   bool ckpt_surface = false;
   int64_t max_history_splits = 10000000;
+};
+
+// One problem of a problem set: several independent models whose particles
+// share one event loop.  Everything that differs between the problems and is
+// not reached through a cell (materials, surfaces, universes and lattices are
+// concatenated, so a cell index names its problem) lives here.
+struct HostPart {
+  std::string dir;
+  int64_t n_particles = 0;
+  int64_t first = 0; // first population index of the problem in a batch
+  int root_universe = 0;
+  int tally = -1; // index into HostModel::tallies
+  int mat_begin = 0, surf_begin = 0, cell_begin = 0;
+  int n_nuclides = 0;
+  double energy_min = 0, energy_max = 0; // of the problem's own nuclides
+  DevSource src {};
+  std::vector<double> disc_x, disc_p, disc_prob;
+  std::vector<int> disc_alias, domain_ids;
 };
 
 struct HostModel {
@@ -106,11 +131,25 @@ struct HostModel {
 
   double energy_min = 0, energy_max = 0, log_spacing = 0;
   std::vector<int> grid_index; // [n_nuc * (n_log_bins + 1)]
+
+  // problem set (empty for a single model)
+  std::vector<HostPart> parts;
+  std::vector<int> mat_nuc_local; // problem-local nuclide index of each mat_nuc entry
 };
 
 // Reads <cwd>/model.xml (or the separate settings/materials/geometry/tallies
 // files) exactly as openmc_init's read_model_xml does (src/initialize.cpp:360-470).
-void read_model(HostModel& m, const std::string& dir);
+// With a pool, nuclides the pool already holds from the same file are not
+// read again (their entries get `pooled` set and no data).
+void read_model(HostModel& m, const std::string& dir, const HostModel* pool = nullptr);
+
+// Reads several independent models (one directory each) into one problem
+// set: materials, surfaces, cells, universes, lattices and tallies are
+// concatenated with their indices shifted, and each nuclide data file is
+// held once.  The problems must agree on what is global to a run (batches,
+// seed and stride, the event limit, free-gas threshold, energy cutoff, the
+// energy range of their data); weight windows are not supported in a set.
+void read_problem_set(HostModel& out, const std::vector<std::string>& dirs);
 
 // initialize_data + Nuclide::init_grid (src/simulation.cpp:682-761, src/nuclide.cpp:502-533)
 void initialize_data(HostModel& m);

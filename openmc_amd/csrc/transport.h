@@ -26,11 +26,19 @@ struct LaunchConfig {
   int sort;     // order the calculate-xs queue by key (else only compact it)
   int sort_blocks; // maximum workgroups of the partition count and scatter kernels
   int xs_group;    // nuclide lookups interleaved per lane (1, 2, 4 or 8) when xs_lanes == 1
+  int block;       // threads per workgroup of the event kernels (64, 128 or 256)
+  int sort_every;  // order by (material, energy) every this many cycles, by material in between
 };
 
-void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st);
-void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st);
-void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st);
+// Source particles for population indices g_base + [0, n) of batch `batch`
+// (1-based) into slots [0, n).
+void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t g_base, int64_t batch,
+  hipStream_t st);
+void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st, int cycle = 0);
+// `cycle` is the index of the event cycle the kernel belongs to (it selects
+// the calculate-xs partition's key under sort_every).
+void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st,
+  int cycle = 0);
 void launch_accumulate(double* value, double* results, int64_t n_bins, double norm, hipStream_t st);
 
 // Standalone batched lookups (xs_lookup.hip): E[n] -> i_grid[n], xs[n][2]

@@ -74,6 +74,20 @@ __device__ __forceinline__ PLevel* level_rec(const DevParticles& S, int64_t p, i
   return S.lev + p * (NL > 1 ? NL - 1 : 1) + (l - 1);
 }
 
+// First material, surface and cell index of a particle's problem: history
+// hashes record problem-local indices, so a problem's hashes in a problem set
+// equal its hashes when run alone (all 0 for a single model).
+struct PartBase {
+  int mat, surf, cell;
+};
+__device__ __forceinline__ PartBase part_base(const DevProblem& P, int cell0)
+{
+  if (P.n_parts <= 1 || cell0 < 0)
+    return PartBase {0, 0, 0};
+  const DevPart& q = P.part[P.cell[cell0].part];
+  return PartBase {q.mat_begin, q.surf_begin, q.cell_begin};
+}
+
 // Geometry state from a particle's records (h already loaded by the caller).
 template<int NL>
 __device__ __forceinline__ void load_geo(const DevProblem& P, const DevParticles& S, int64_t p, const PHot& h,
@@ -86,7 +100,8 @@ __device__ __forceinline__ void load_geo(const DevProblem& P, const DevParticles
   g.uy = h.uy;
   g.uz = h.uz;
   g.cell[0] = h.cell0;
-  g.univ[0] = h.cell0 == C_NONE ? C_NONE : P.root_universe; // level 0 is always the root universe
+  // level 0 is always the root universe (of the particle's problem in a problem set)
+  g.univ[0] = h.cell0 == C_NONE ? C_NONE : (P.n_parts > 1 ? P.cell[h.cell0].universe : P.root_universe);
   g.lat[0] = C_NONE;
   g.lix[0] = g.liy[0] = g.liz[0] = 0;
 #pragma unroll
@@ -164,22 +179,24 @@ __device__ __forceinline__ void geo_clear(Geo<NL>& g, V3 r, V3 u)
 //==============================================================================
 
 template<int NL>
-__device__ __forceinline__ bool satisfies_spatial_constraints(const DevProblem& P, V3 r)
+__device__ __forceinline__ bool satisfies_spatial_constraints(const DevProblem& P, const DevSource& src, int root,
+  V3 r)
 {
   Geo<NL> g;
   geo_clear(g, r, V3 {0.0, 0.0, 1.0});
+  g.univ[0] = root;
   if (!exhaustive_find_cell(P, g))
     return false;
   bool accepted = true;
-  if (P.src.n_domain > 0) {
+  if (src.n_domain > 0) {
 #pragma unroll
     for (int i = 0; i < NL; i++) {
       if (i >= g.n_coord)
         break;
       const int id = P.cell[g.cell[i]].id;
       accepted = false;
-      for (int k = 0; k < P.src.n_domain; ++k)
-        accepted = accepted || (P.src.domain_ids[k] == id);
+      for (int k = 0; k < src.n_domain; ++k)
+        accepted = accepted || (src.domain_ids[k] == id);
       if (accepted)
         break;
     }
@@ -200,39 +217,69 @@ __device__ __forceinline__ double watt_spectrum(double a, double b, uint64_t* se
 
 constexpr int MAX_SOURCE_TRIES = 1 << 20;
 
-template<int NL>
-__global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, int64_t n, int64_t id_base)
+// Problem of population index gi in a problem set: the last problem whose
+// first index is <= gi (problems are laid out in population order).
+__device__ __forceinline__ int part_of_index(const DevProblem& P, int64_t gi)
+{
+  int lo = 0, hi = P.n_parts - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (P.part[mid].first <= gi)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Source particles for population indices g_base + [0, n) of batch `batch`.
+// A particle's id is its index within its own problem plus the batches
+// before (src/simulation.cpp:581-583, 613-616), so a problem of a problem set
+// samples exactly the histories it samples when run alone.
+template<int NL, bool MULTI>
+__global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, int64_t n, int64_t g_base,
+  int64_t batch)
 {
   unsigned long long n_lost = 0;
   double total_w = 0.0; // source weight after the birth weight windows
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t id = id_base + i + 1;
+    const int64_t gi = g_base + i;
+    int part = 0;
+    int64_t id;
+    if (MULTI) {
+      part = part_of_index(P, gi);
+      id = (batch - 1) * P.part[part].n_particles + (gi - P.part[part].first) + 1;
+    } else {
+      id = (batch - 1) * P.n_particles + gi + 1;
+    }
+    const DevSource& src = MULTI ? P.part[part].src : P.src;
+    const int root = MULTI ? P.part[part].root_universe : P.root_universe;
     uint64_t seed = future_seed(static_cast<uint64_t>(id) * P.stride, P.master_seed + 1);
     V3 r = {0, 0, 0};
     bool lost = false;
     int tries = 0;
     bool accepted = false;
     while (!accepted) {
-      if (P.src.space_type == 0) {
-        r = {P.src.p0[0], P.src.p0[1], P.src.p0[2]};
+      if (src.space_type == 0) {
+        r = {src.p0[0], src.p0[1], src.p0[2]};
       } else {
         double x1 = prn(&seed);
         double x2 = prn(&seed);
         double x3 = prn(&seed);
-        V3 ll = {P.src.p0[0], P.src.p0[1], P.src.p0[2]};
-        V3 ur = {P.src.p1[0], P.src.p1[1], P.src.p1[2]};
+        V3 ll = {src.p0[0], src.p0[1], src.p0[2]};
+        V3 ur = {src.p1[0], src.p1[1], src.p1[2]};
         V3 ext = v_sub(ur, ll);
         r = v_add(ll, V3 {x1 * ext.x, x2 * ext.y, x3 * ext.z});
       }
-      accepted = satisfies_spatial_constraints<NL>(P, r);
+      accepted = satisfies_spatial_constraints<NL>(P, src, root, r);
       if (++tries >= MAX_SOURCE_TRIES && !accepted) {
         lost = true;
         break;
       }
     }
     V3 u;
-    if (P.src.angle_type == 0) {
+    if (src.angle_type == 0) {
       // isotropic_direction (src/distribution_multi.cpp:106-112)
       double phi = uniform_distribution(0., 2.0 * PI, &seed);
       double mu = uniform_distribution(-1., 1., &seed);
@@ -240,24 +287,24 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
       detm::sincos(phi, &s, &c);
       u = {mu, sqrt(1.0 - mu * mu) * c, sqrt(1.0 - mu * mu) * s};
     } else {
-      u = {P.src.u_ref[0], P.src.u_ref[1], P.src.u_ref[2]};
+      u = {src.u_ref[0], src.u_ref[1], src.u_ref[2]};
     }
     double E = 0.0;
     tries = 0;
     while (true) {
-      if (P.src.energy_type == 0) {
-        int nd = P.src.n_discrete;
+      if (src.energy_type == 0) {
+        int nd = src.n_discrete;
         int k = 0;
         if (nd > 1) {
           size_t uu = static_cast<size_t>(prn(&seed) * nd);
-          if (prn(&seed) < P.src.disc_prob[uu])
+          if (prn(&seed) < src.disc_prob[uu])
             k = static_cast<int>(uu);
           else
-            k = P.src.disc_alias[uu];
+            k = src.disc_alias[uu];
         }
-        E = P.src.disc_x[k];
+        E = src.disc_x[k];
       } else {
-        E = watt_spectrum(P.src.watt_a, P.src.watt_b, &seed);
+        E = watt_spectrum(src.watt_a, src.watt_b, &seed);
       }
       if (E < P.energy_max && (E > 0.0 && E < INFTY))
         break;
@@ -273,6 +320,7 @@ __global__ void __launch_bounds__(BLOCK) k_init(DevProblem P, DevParticles S, in
     // leaves the same coordinate levels, material and material_last.
     Geo<NL> g;
     geo_clear(g, r, u);
+    g.univ[0] = root;
     if (!lost) {
       if (!exhaustive_find_cell(P, g) || g.lost)
         lost = true;
@@ -353,6 +401,7 @@ struct Part {
   int* n_out;
   int* bin_tot;
   int nb;
+  int shift; // keys are used >> shift (the material part of a calculate-xs key)
 };
 
 // A partition of n entries uses nw(n) = clamp(ceil(n / 2048), 1, max_blocks)
@@ -370,7 +419,7 @@ __device__ __forceinline__ int sort_workgroups(int n, int max_blocks)
 __device__ __forceinline__ int part_key(const Part& A, int i)
 {
   const int k = A.key[i];
-  return k < 0 ? -1 : (A.nb == 1 ? 0 : k);
+  return k < 0 ? -1 : (A.nb == 1 ? 0 : k >> A.shift);
 }
 
 // Per-workgroup key counts of its chunk into row b of the count matrix
@@ -578,7 +627,7 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
       if (trace) {
         uint64_t h = S.hash[p];
         hmix(h, 0x11);
-        hmix(h, static_cast<uint64_t>(material));
+        hmix(h, static_cast<uint64_t>(material - part_base(P, S.hot[p].cell0).mat));
         for (int jj = 0; jj < nn; ++jj)
           hmix(h, static_cast<uint64_t>(static_cast<uint32_t>(igs[jj])));
         S.hash[p] = h;
@@ -608,7 +657,7 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
   if (trace) {
     hh = S.hash[p];
     hmix(hh, 0x11);
-    hmix(hh, static_cast<uint64_t>(material));
+    hmix(hh, static_cast<uint64_t>(material - part_base(P, h->cell0).mat));
   }
   double xs_t = 0.0;
   double xs_a = 0.0;
@@ -698,7 +747,7 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
 // advance event (src/particle.cpp:232-286, src/event.cpp:107-128)
 //==============================================================================
 
-template<int NL>
+template<int NL, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S, const int* q)
 {
   const int n = S.qcount[QC_XS];
@@ -747,8 +796,15 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S,
         }
         double dt = distance / speed;
         hp->time = h.time + dt;
-        if (P.tally.enabled)
-          n_segs += score_tracklength(P.tally, r0, r1, g.u(), E, wgt, distance);
+        if (P.tally.enabled) {
+          if (MULTI) {
+            const DevTally T = P.part[h.cell0 >= 0 ? P.cell[h.cell0].part : 0].tally;
+            if (T.n_bins > 0)
+              n_segs += score_tracklength(T, r0, r1, g.u(), E, wgt, distance);
+          } else {
+            n_segs += score_tracklength(P.tally, r0, r1, g.u(), E, wgt, distance);
+          }
+        }
         if (distance == INFTY)
           wgt = 0.0;
         if (g.lost) {
@@ -848,10 +904,13 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
       }
       uint64_t hh = 0;
       const bool trace = S.trace_hash != 0;
+      PartBase base {0, 0, 0};
       if (trace) {
+        base = part_base(P, h.cell0);
+        const int s_local = g.surface > 0 ? g.surface - base.surf : (g.surface < 0 ? g.surface + base.surf : 0);
         hh = S.hash[p];
         hmix(hh, 0x33);
-        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(g.surface)));
+        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(s_local)));
         hmix(hh, static_cast<uint64_t>((trans[0] + 1) | ((trans[1] + 1) << 2) | ((trans[2] + 1) << 4)));
       }
       if (g.lost) {
@@ -863,7 +922,12 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
         const DevSurface surf = P.surf[abs(g.surface) - 1];
         if (surf.bc == BC_VACUUM) {
           // cross_vacuum_bc (src/particle.cpp:631-655)
-          leak += wgt;
+          if (P.n_parts > 1)
+            unsafeAtomicAdd(&S.leak[(h.cell0 >= 0 ? P.cell[h.cell0].part : 0) * LEAK_PART_SLOTS +
+                                    (wave_slot() & (LEAK_PART_SLOTS - 1))],
+              wgt);
+          else
+            leak += wgt;
           wgt = 0.0;
         } else if (surf.bc == BC_REFLECT) {
           // ReflectiveBC::handle_particle + cross_reflective_bc (src/boundary_condition.cpp:37-46,
@@ -914,7 +978,7 @@ __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
       }
       if (trace) {
         hmix(hh, dbits(wgt));
-        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? lowest_cell(g) : -1)));
+        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? lowest_cell(g) - base.cell : -1)));
         if (WW)
           hmix(hh, static_cast<uint64_t>(st.n_bank));
         S.hash[p] = hh;
@@ -1005,11 +1069,13 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
       const double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
       double prob = 0.0;
       int i_nuclide = -1;
+      int j_nuclide = -1; // its entry in the material's list
       for (int j = j0; j < j1; ++j) {
         const double dens = P.mat_dens[j] * 1.0;
         prob += dens * micro[j - j0];
         if (prob >= cutoff) {
           i_nuclide = P.mat_nuc[j];
+          j_nuclide = j;
           break;
         }
       }
@@ -1072,7 +1138,8 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
       if (S.trace_hash) {
         uint64_t hh = S.hash[p];
         hmix(hh, 0x44);
-        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(i_nuclide)));
+        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(P.n_parts > 1 && j_nuclide >= 0 ?
+                                                                P.mat_nuc_local[j_nuclide] : i_nuclide)));
         hmix(hh, absorbed ? 101 : 2);
         hmix(hh, dbits(E));
         hmix(hh, dbits(u.x));
@@ -1142,10 +1209,14 @@ __global__ void __launch_bounds__(BLOCK) k_accumulate(double* value, double* res
 namespace {
 
 template<int NL>
-void init_nl(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st)
+void init_nl(const DevProblem& P, const DevParticles& S, int64_t n, int64_t g_base, int64_t batch, hipStream_t st)
 {
   int blocks = static_cast<int>((n + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_init<NL>, dim3(blocks < 1 ? 1 : blocks), dim3(BLOCK), 0, st, P, S, n, id_base);
+  if (P.n_parts > 1)
+    hipLaunchKernelGGL((k_init<NL, true>), dim3(blocks < 1 ? 1 : blocks), dim3(BLOCK), 0, st, P, S, n, g_base, batch);
+  else
+    hipLaunchKernelGGL((k_init<NL, false>), dim3(blocks < 1 ? 1 : blocks), dim3(BLOCK), 0, st, P, S, n, g_base,
+      batch);
 }
 
 template<int NL, int LPP>
@@ -1160,31 +1231,34 @@ void xs_nl(const DevProblem& P, const DevParticles& S, const int* q, int blocks,
 }
 
 template<int NL, bool LEARNED>
-void xs_lane_l(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int group, hipStream_t st)
+void xs_lane_l(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, int group,
+  hipStream_t st)
 {
   switch (group) {
-  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
-  default: hipLaunchKernelGGL((k_xs_lane<NL, 4, LEARNED>), dim3(blocks), dim3(BLOCK), 0, st, P, S, q); break;
+  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  default: hipLaunchKernelGGL((k_xs_lane<NL, 4, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
   }
 }
 
 template<int NL>
-void xs_lane(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int group, hipStream_t st)
+void xs_lane(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, int group,
+  hipStream_t st)
 {
   if (S.use_learned)
-    xs_lane_l<NL, true>(P, S, q, blocks, group, st);
+    xs_lane_l<NL, true>(P, S, q, blocks, tpb, group, st);
   else
-    xs_lane_l<NL, false>(P, S, q, blocks, group, st);
+    xs_lane_l<NL, false>(P, S, q, blocks, tpb, group, st);
 }
 
 template<int NL>
 void xs_lpp(const DevProblem& P, const DevParticles& S, const int* q, const LaunchConfig& lc, hipStream_t st)
 {
   const int blocks = lc.blocks;
+  const int tpb = lc.block == 64 || lc.block == 128 ? lc.block : BLOCK;
   switch (lc.xs_lanes) {
-  case 1: xs_lane<NL>(P, S, q, blocks, lc.xs_group, st); break;
+  case 1: xs_lane<NL>(P, S, q, blocks, tpb, lc.xs_group, st); break;
   case 2: xs_nl<NL, 2>(P, S, q, blocks, st); break;
   case 4: xs_nl<NL, 4>(P, S, q, blocks, st); break;
   case 8: xs_nl<NL, 8>(P, S, q, blocks, st); break;
@@ -1205,66 +1279,78 @@ void launch_partition(const DevParticles& S, const Part& A, const LaunchConfig& 
 }
 
 template<int NL>
-void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
+void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st,
+  int cycle)
 {
+  const int tpb = lc.block == 64 || lc.block == 128 ? lc.block : BLOCK;
   switch (which) {
   case KERNEL_SORT: {
-    // staged survivors -> q_xs, sorted by key (or only compacted)
+    // staged survivors -> q_xs, sorted by (material, energy bucket) key, by
+    // material only on the cycles between full sorts, or only compacted
+    const bool full = lc.sort_every <= 1 || cycle % lc.sort_every == 0;
+    const int eb_shift = __builtin_ctz(static_cast<unsigned>(S.sort_eb));
     Part A {S.stage_p, S.stage_key, S.qcount + QC_STAGE, S.q_xs, S.qcount + QC_XS, nullptr,
-      lc.sort ? S.sort_nb : 1};
+      !lc.sort ? 1 : (full ? S.sort_nb : S.sort_nb >> eb_shift), full ? 0 : eb_shift};
     launch_partition(S, A, lc, st);
     break;
   }
   case KERNEL_XS: xs_lpp<NL>(P, S, S.q_xs, lc, st); break;
-  case KERNEL_ADVANCE: hipLaunchKernelGGL(k_advance<NL>, dim3(lc.blocks), dim3(BLOCK), 0, st, P, S, S.q_xs); break;
+  case KERNEL_ADVANCE:
+    if (P.n_parts > 1)
+      hipLaunchKernelGGL((k_advance<NL, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S, S.q_xs);
+    else
+      hipLaunchKernelGGL((k_advance<NL, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S, S.q_xs);
+    break;
   case KERNEL_SPLIT: {
     // advance outcomes -> q_sc = [surface crossings | collisions]; the two
     // counts land in QC_SURF and QC_COLL, their sum in QC_STAGE
-    Part A {S.q_xs, S.adv_key, S.qcount + QC_XS, S.q_sc, S.qcount + QC_STAGE, S.qcount + QC_SURF, 2};
+    Part A {S.q_xs, S.adv_key, S.qcount + QC_XS, S.q_sc, S.qcount + QC_STAGE, S.qcount + QC_SURF, 2, 0};
     launch_partition(S, A, lc, st);
     break;
   }
   case KERNEL_SURFACE:
     if (P.ww.on)
-      hipLaunchKernelGGL((k_surface<NL, true>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+      hipLaunchKernelGGL((k_surface<NL, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     else
-      hipLaunchKernelGGL((k_surface<NL, false>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+      hipLaunchKernelGGL((k_surface<NL, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     break;
   default:
     if (P.ww.on)
-      hipLaunchKernelGGL((k_collision<NL, true>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+      hipLaunchKernelGGL((k_collision<NL, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     else
-      hipLaunchKernelGGL((k_collision<NL, false>), dim3(lc.blocks), dim3(BLOCK), 0, st, P, S);
+      hipLaunchKernelGGL((k_collision<NL, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     break;
   }
 }
 
 } // namespace
 
-void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t id_base, hipStream_t st)
+void launch_init(const DevProblem& P, const DevParticles& S, int64_t n, int64_t g_base, int64_t batch,
+  hipStream_t st)
 {
   switch (P.n_levels) {
-  case 1: init_nl<1>(P, S, n, id_base, st); break;
-  case 2: init_nl<2>(P, S, n, id_base, st); break;
-  case 3: init_nl<3>(P, S, n, id_base, st); break;
-  default: init_nl<4>(P, S, n, id_base, st); break;
+  case 1: init_nl<1>(P, S, n, g_base, batch, st); break;
+  case 2: init_nl<2>(P, S, n, g_base, batch, st); break;
+  case 3: init_nl<3>(P, S, n, g_base, batch, st); break;
+  default: init_nl<4>(P, S, n, g_base, batch, st); break;
   }
 }
 
-void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
+void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st,
+  int cycle)
 {
   switch (P.n_levels) {
-  case 1: kernel_nl<1>(which, P, S, lc, st); break;
-  case 2: kernel_nl<2>(which, P, S, lc, st); break;
-  case 3: kernel_nl<3>(which, P, S, lc, st); break;
-  default: kernel_nl<4>(which, P, S, lc, st); break;
+  case 1: kernel_nl<1>(which, P, S, lc, st, cycle); break;
+  case 2: kernel_nl<2>(which, P, S, lc, st, cycle); break;
+  case 3: kernel_nl<3>(which, P, S, lc, st, cycle); break;
+  default: kernel_nl<4>(which, P, S, lc, st, cycle); break;
   }
 }
 
-void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st)
+void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st, int cycle)
 {
   for (int k = KERNEL_SORT; k <= KERNEL_COLLISION; ++k)
-    launch_kernel(k, P, S, lc, st);
+    launch_kernel(k, P, S, lc, st, cycle);
 }
 
 void launch_accumulate(double* value, double* results, int64_t n_bins, double norm, hipStream_t st)

@@ -122,6 +122,11 @@ _bind("omc_macro_batch", [c_int, POINTER(c_double), c_int64, c_int, c_int, POINT
 _bind("omc_math_batch", [c_int, POINTER(c_double), POINTER(c_double), c_int64, POINTER(c_double)])
 _bind("omc_copy_kernel", [c_int64, c_int, POINTER(c_double)])
 _bind("omc_check_model", [c_char_p, POINTER(c_double), c_int, POINTER(c_int), c_int64], c_int, check=False)
+_bind("omc_init_problems", [c_int, POINTER(c_char_p)])
+_bind("omc_n_problems", [], c_int, check=False)
+_bind("omc_problem_info", [c_int, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)])
+_bind("omc_problem_global_tallies", [c_int, POINTER(POINTER(c_double))])
+_bind("omc_check_problem_set", [c_int, POINTER(c_char_p), POINTER(c_double), c_int], c_int, check=False)
 _bind("omc_learned_build", [c_int, c_int, POINTER(c_int)])
 _bind("omc_learned_loss_grad", [c_int, c_int, POINTER(c_float), POINTER(c_double), POINTER(c_double)])
 _bind("omc_learned_model", [c_int, POINTER(c_float), POINTER(c_int)], c_int, check=False)
@@ -252,6 +257,17 @@ class Tally:
         self._index = index.value
         self.id = tally_id
 
+    @classmethod
+    def from_index(cls, index):
+        """The tally at an index of the tallies array (ids repeat across the
+        problems of a problem set; indices do not)."""
+        t = cls.__new__(cls)
+        t._index = int(index)
+        tid = c_int32()
+        _dll.openmc_tally_get_id(t._index, tid)
+        t.id = tid.value
+        return t
+
     @property
     def num_realizations(self):
         n = c_int32()
@@ -369,6 +385,61 @@ def copy_kernel(n_bytes, reps=10):
     ms = c_double()
     _dll.omc_copy_kernel(int(n_bytes), int(reps), ms)
     return ms.value
+
+
+def init_problems(directories, output=True):
+    """Build-specific: initialize a problem set, several independent models
+    (one directory each) whose particles share one event loop.  run() and
+    next_batch() then advance every problem; each keeps its own ids, seeds,
+    source, tally and leakage."""
+    global is_initialized
+    arr = (c_char_p * len(directories))(*[os.fspath(d).encode() for d in directories])
+    if not output:
+        c_int.in_dll(_dll, "verbosity").value = 1
+    _dll.omc_init_problems(len(directories), arr)
+    is_initialized = True
+
+
+def n_problems():
+    return _dll.omc_n_problems()
+
+
+def problem_info(problem):
+    """Particles per batch, first population index and tally index (-1 if
+    none) of one problem of the set."""
+    n, first, tally = c_int64(), c_int64(), c_int32()
+    _dll.omc_problem_info(int(problem), n, first, tally)
+    return {"n_particles": n.value, "first": first.value, "tally_index": tally.value}
+
+
+def problem_tally(problem):
+    """The Tally of one problem of the set (None without one)."""
+    idx = problem_info(problem)["tally_index"]
+    return None if idx < 0 else Tally.from_index(idx)
+
+
+def problem_global_tallies(problem):
+    """Global tallies [4, 3] (sum, sum of squares in columns 1, 2) of one problem."""
+    ptr = POINTER(c_double)()
+    _dll.omc_problem_global_tallies(int(problem), ptr)
+    return np.ctypeslib.as_array(ptr, (4, 3)).copy()
+
+
+def check_problem_set(directories):
+    """Host-only reading of a problem set (no GPU): totals and per-problem offsets."""
+    arr = (c_char_p * len(directories))(*[os.fspath(d).encode() for d in directories])
+    out = (c_double * (16 + 8 * len(directories)))()
+    n = _dll.omc_check_problem_set(len(directories), arr, out, len(out))
+    if n < 0:
+        _error_handler(n, None, None)
+    vals = list(out)[:n]
+    keys = ["n_nuclides", "n_materials", "n_cells", "n_surfaces", "n_universes", "n_lattices", "n_levels",
+            "n_particles", "n_tallies", "energy_min", "energy_max"]
+    head = dict(zip(keys, vals[:len(keys)]))
+    pk = ["first", "n_particles", "root_universe", "mat_begin", "surf_begin", "cell_begin", "tally", "tally_off"]
+    rest = vals[len(keys):]
+    parts = [dict(zip(pk, rest[8 * k:8 * k + 8])) for k in range(len(rest) // 8)]
+    return head, parts
 
 
 def check_model(directory, n_gi=0):

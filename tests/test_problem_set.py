@@ -1,0 +1,142 @@
+"""Problem sets: several independent models sharing one event loop.
+
+Host side (no GPU): the concatenated model keeps each problem's indices,
+particle ranges and tally offsets, holds each nuclide data file once and
+rejects problems that disagree on run-global settings.  GPU side: every
+problem of a set gives, history for history, the hashes of the problem run
+alone and of the oracle, and its tally and leakage within 1e-12 relative.
+"""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from helpers import assert_tallies_close, chdir, gpu_run, oracle_run
+
+SMALL = dict(n_problems=4, n_batches=2, particles=(300, 1500), lattice=(2, 4), nuclides=(6, 12), mesh=(3, 6),
+             grid_lo=2000, grid_hi=4000)
+
+
+@pytest.fixture(scope="module")
+def small_set():
+    from openmc_amd import configs
+    d, dirs, problems, nucs = configs.problem_set_dir(**SMALL)
+    return d, dirs, problems, nucs
+
+
+def test_problem_set_host_layout(small_set):
+    from openmc_amd import lib
+    d, dirs, problems, nucs = small_set
+    head, parts = lib.check_problem_set(dirs)
+    used = sorted({n.name for _, ns in problems for n in ns})
+    assert head["n_nuclides"] == len(used)
+    assert head["n_materials"] == 3 * len(problems)
+    assert head["n_levels"] == 2
+    npart = [m.settings.particles for m, _ in problems]
+    assert head["n_particles"] == sum(npart)
+    assert head["n_tallies"] == len(problems)
+    first = np.concatenate([[0], np.cumsum(npart)[:-1]])
+    cells = surfs = bins = 0
+    for k, (p, (m, _)) in enumerate(zip(parts, problems)):
+        one, _, _ = lib.check_model(dirs[k])
+        assert p["first"] == first[k] and p["n_particles"] == npart[k]
+        assert p["mat_begin"] == 3 * k
+        assert p["cell_begin"] == cells and p["surf_begin"] == surfs
+        assert p["tally"] == k and p["tally_off"] == bins
+        cells += one["n_cells"]
+        surfs += one["n_surfaces"]
+        bins += one["n_tally_bins"]
+        assert one["energy_min"] == head["energy_min"] and one["energy_max"] == head["energy_max"]
+    assert head["n_cells"] == cells and head["n_surfaces"] == surfs
+
+
+def test_problem_set_rejects_mismatched_settings(small_set, tmp_path):
+    from openmc_amd import lib
+    d, dirs, problems, nucs = small_set
+    bad = os.path.join(str(tmp_path), "bad")
+    shutil.copytree(dirs[1], bad)
+    p = os.path.join(bad, "model.xml")
+    text = open(p).read()
+    assert "<batches>2</batches>" in text
+    open(p, "w").write(text.replace("<batches>2</batches>", "<batches>3</batches>"))
+    # keep the relative path to the shared library valid from the copy
+    text = open(p).read().replace("../data/", os.path.join(d, "data") + "/")
+    open(p, "w").write(text)
+    with pytest.raises(lib.OpenMCError, match="must share batches"):
+        lib.check_problem_set([dirs[0], bad])
+
+
+def _set_run(dirs, options=None):
+    from openmc_amd import lib
+    lib.init_problems(dirs, output=False)
+    try:
+        for k, v in (options or {}).items():
+            lib.set_option(k, v)
+        lib.run()
+        h = lib.particle_hashes()
+        out = []
+        for k in range(lib.n_problems()):
+            info = lib.problem_info(k)
+            t = lib.problem_tally(k)
+            out.append({"hashes": h[info["first"]:info["first"] + info["n_particles"]].copy(),
+                        "results": t.results.copy(), "n_real": t.num_realizations,
+                        "global": lib.problem_global_tallies(k)})
+        stats = lib.stats()
+    finally:
+        lib.finalize()
+    return out, stats
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("options", [{}, {"max_in_flight": 1000}], ids=["all-in-flight", "chunked"])
+def test_problem_set_matches_problems_alone(small_set, options):
+    from oracle.problem import write_oracle_problem
+    d, dirs, problems, nucs = small_set
+    opts = dict(options, trace_hash=1)
+    got, _ = _set_run(dirs, opts)
+    for k, (model, used) in enumerate(problems):
+        alone = gpu_run(dirs[k], options={"trace_hash": 1})
+        n = model.settings.particles
+        assert got[k]["hashes"].size == n
+        assert np.array_equal(got[k]["hashes"], alone["hashes"]), "problem %d histories differ" % k
+        assert_tallies_close(got[k]["results"], alone["results"])
+        assert got[k]["n_real"] == alone["n_real"]
+        leak, leak1 = got[k]["global"][3, 1:], np.asarray(alone["global"]).reshape(-1, 3)[3, 1:]
+        assert np.allclose(leak, leak1, rtol=1e-12, atol=0)
+        if k < 2:
+            write_oracle_problem(model, used, dirs[k], data_dir=os.path.join("..", "data"))
+            o = oracle_run(dirs[k])
+            assert np.array_equal(got[k]["hashes"], o.hashes(n)), "problem %d differs from the oracle" % k
+            assert_tallies_close(got[k]["results"], o.results())
+
+
+@pytest.mark.gpu
+def test_autotune_picks_from_space_and_keeps_histories(small_set):
+    """The autotuner times real batches and leaves its choice set; whatever it
+    picks, the histories are those of the default launch shape."""
+    from openmc_amd import autotune, lib
+    d, dirs, problems, nucs = small_set
+    ref, _ = _set_run(dirs, {"trace_hash": 1})
+    space = [("block", [256, 64]), ("sort_every", [1, 3]), ("cycles_per_check", [16, 4])]
+    lib.init_problems(dirs, output=False)
+    try:
+        best, trials = autotune.tune(space=space)
+        assert set(best) >= {"block", "sort_every", "cycles_per_check"}
+        for name, values in space:
+            assert best[name] in values
+        assert len(trials) == 1 + sum(len(v) - 1 for _, v in space)
+        assert all(t["seconds"] > 0 for t in trials)
+        st = lib.stats()
+        assert st["block"] == best["block"] and st["sort_every"] == best["sort_every"]
+        lib.set_option("trace_hash", 1)
+        lib.run()
+        h = lib.particle_hashes()
+        for k in range(lib.n_problems()):
+            info = lib.problem_info(k)
+            assert np.array_equal(h[info["first"]:info["first"] + info["n_particles"]], ref[k]["hashes"])
+            t = lib.problem_tally(k)
+            assert t.num_realizations == ref[k]["n_real"]
+            assert_tallies_close(t.results, ref[k]["results"])
+    finally:
+        lib.finalize()
