@@ -332,6 +332,11 @@ def pmc_summary(dirs, events=None, copy_bytes=None):
         n = max(len(disp[k]), 1)
         lines.append("%-34s %10d %14.3f %16.1f %16.1f" % (k, n, dur.get(k, 0.0), agg[k].get("FETCH_SIZE", 0) / n,
                                                         agg[k].get("WRITE_SIZE", 0) / n))
+    # any other counters, per dispatch
+    for k in sorted(agg):
+        for cn in sorted(agg[k]):
+            if cn not in ("FETCH_SIZE", "WRITE_SIZE"):
+                lines.append("counter %-30s %-28s %18.1f" % (k, cn, agg[k][cn] / max(len(disp[k]), 1)))
     if events and copy_bytes:
         copy = [k for k in agg if k.startswith("k_copy")]
         if copy:

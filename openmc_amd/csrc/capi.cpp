@@ -97,7 +97,12 @@ struct Sim {
   double* d_results = nullptr;
   std::vector<double> h_results; // [all tallies' bins][3]
   int64_t total_bins = 0;
-  double* d_value = nullptr; // VALUE of every tally, each at HostTally::off
+  double* d_value = nullptr; // VALUE of every tally: [n_copies][bins] at HostTally::off * n_copies
+  void* value_alloc = nullptr;
+  int tally_copies = 1; // copies of each tally's VALUE (up to one per XCD; 1 measured fastest)
+  bool wave_search = true; // wavefront-cooperative grid searches (plain lookups)
+  std::vector<DevPart> h_parts;
+  void* parts_alloc = nullptr;
   bool results_dirty = false;
   int n_real = 0;
   std::vector<double> gt = std::vector<double>(3 * N_GLOBAL, 0.0);
@@ -213,6 +218,41 @@ void upload_matnuc(Sim& s)
   s.P.matnuc = static_cast<const DevMatNuc*>(s.matnuc_alloc);
 }
 
+// (Re)allocates the tallies' VALUE arrays with n_copies copies of their bins
+// each (one per XCD: a wavefront adds into its XCD's copy) and points the
+// device tally records at them.
+void set_tally_copies(Sim& s, int n_copies)
+{
+  if (s.value_alloc) {
+    (void)hipFree(s.value_alloc);
+    s.value_alloc = nullptr;
+  }
+  s.d_value = nullptr;
+  s.tally_copies = n_copies;
+  if (s.total_bins > 0) {
+    hip_check(hipMalloc(&s.value_alloc, sizeof(double) * s.total_bins * n_copies), "hipMalloc");
+    hip_check(hipMemset(s.value_alloc, 0, sizeof(double) * s.total_bins * n_copies), "hipMemset");
+    s.d_value = static_cast<double*>(s.value_alloc);
+  }
+  for (size_t k = 0; k < s.h_parts.size(); ++k) {
+    const int t = s.m.parts[k].tally;
+    if (t >= 0) {
+      s.h_parts[k].tally.value = s.d_value + s.m.tallies[t].off * n_copies;
+      s.h_parts[k].tally.n_copies = n_copies;
+    }
+  }
+  const int enabled = s.P.tally.enabled;
+  s.P.tally = s.h_parts[0].tally;
+  s.P.tally.enabled = enabled;
+  if (s.parts_alloc)
+    (void)hipFree(s.parts_alloc);
+  hip_check(hipMalloc(&s.parts_alloc, sizeof(DevPart) * s.h_parts.size()), "hipMalloc");
+  hip_check(hipMemcpy(s.parts_alloc, s.h_parts.data(), sizeof(DevPart) * s.h_parts.size(), hipMemcpyHostToDevice),
+    "hipMemcpy");
+  s.P.part = static_cast<const DevPart*>(s.parts_alloc);
+  s.graph_valid = false;
+}
+
 void upload_problem(Sim& s)
 {
   HostModel& m = s.m;
@@ -313,9 +353,7 @@ void upload_problem(Sim& s)
     t.off = total_bins;
     total_bins += t.n_filter_bins;
   }
-  double* value = nullptr;
   if (total_bins > 0) {
-    value = dev_alloc<double>(s.dev_allocs, total_bins);
     s.d_results = dev_alloc<double>(s.dev_allocs, 3 * total_bins);
     s.h_results.assign(3 * total_bins, 0.0);
   }
@@ -331,7 +369,8 @@ void upload_problem(Sim& s)
     T.n_energy = t.ebins.empty() ? 0 : static_cast<int>(t.ebins.size()) - 1;
     T.ebins = dev_upload(s, t.ebins);
     T.n_bins = t.n_filter_bins;
-    T.value = value + t.off;
+    T.value = nullptr; // set by set_tally_copies
+    T.n_copies = 1;
     return T;
   };
 
@@ -355,13 +394,12 @@ void upload_problem(Sim& s)
     parts.push_back(d);
   }
   P.src = parts[0].src;
-  P.tally = parts[0].tally;
   P.root_universe = parts[0].root_universe;
-  P.part = dev_upload(s, parts);
   P.n_parts = static_cast<int>(parts.size());
   P.mat_nuc_local = P.n_parts > 1 ? dev_upload(s, m.mat_nuc_local) : nullptr;
   s.total_bins = total_bins;
-  s.d_value = value;
+  s.h_parts = parts;
+  set_tally_copies(s, s.tally_copies);
 
   if (m.ww.on) {
     const HostWW& w = m.ww;
@@ -585,6 +623,7 @@ void transport_batch(Sim& s)
   }
   s.S.trace_hash = s.trace_hash ? 1 : 0;
   s.S.use_learned = (s.use_learned && s.P.learned.enabled) ? 1 : 0;
+  s.S.wave_search = s.wave_search ? 1 : 0;
   if (s.trace_hash)
     s.h_hashes.assign(m.n_particles, 0);
   hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double) * leak_slots(s), s.stream), "hipMemsetAsync");
@@ -703,7 +742,8 @@ void transport_batch(Sim& s)
       for (const HostTally& t : m.tallies) {
         const double n_part = static_cast<double>(set ? m.parts[t.part].n_particles : m.n_particles);
         double norm = total_source / (n_part * 1);
-        launch_accumulate(s.d_value + t.off, s.d_results + 3 * t.off, t.n_filter_bins, norm, s.stream);
+        launch_accumulate(s.d_value + t.off * s.tally_copies, s.d_results + 3 * t.off, t.n_filter_bins,
+          s.tally_copies, norm, s.stream);
       }
       s.n_real++;
       s.results_dirty = true;
@@ -969,6 +1009,10 @@ int openmc_finalize(void)
     free_learned(g->learned);
     if (g->matnuc_alloc)
       (void)hipFree(g->matnuc_alloc);
+    if (g->value_alloc)
+      (void)hipFree(g->value_alloc);
+    if (g->parts_alloc)
+      (void)hipFree(g->parts_alloc);
     free_list(g->dev_allocs);
     if (g->h_qcount)
       (void)hipHostFree(g->h_qcount);
@@ -1196,6 +1240,22 @@ int omc_set_option(const char* name, double value)
     }
     g->block_opt = v;
     update_launch_config(*g);
+  } else if (n == "wave_search") {
+    g->wave_search = value != 0;
+  } else if (n == "tally_copies") {
+    int v = static_cast<int>(value);
+    if (v != 1 && v != 2 && v != 4 && v != 8) {
+      set_errmsg("tally_copies must be 1, 2, 4 or 8");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    if (g->n_real > 0 || current_batch > 0) {
+      set_errmsg("tally_copies can only be changed before the first batch");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    return guarded([&]() -> int {
+      set_tally_copies(*g, v);
+      return 0;
+    });
   } else if (n == "sort_every") {
     int v = static_cast<int>(value);
     if (v < 1 || v > 1024) {

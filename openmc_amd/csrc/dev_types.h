@@ -145,8 +145,10 @@ struct DevTally {
   double ll[3];
   double width[3];
   const double* ebins; // n_energy + 1 edges
-  double* value;       // [n_bins] VALUE accumulator for the current batch
+  double* value;       // [n_copies][n_bins] VALUE accumulator for the current batch
   int64_t n_bins;
+  int n_copies;        // 1, or one per XCD (a power of two)
+  int pad_copies;
 };
 
 // Mesh weight windows for neutrons (src/weight_windows.cpp:56-146, 375-411)
@@ -359,6 +361,8 @@ struct DevParticles {
   unsigned long long* counters; // [CNT_N][CNT_SLOTS]
   int trace_hash;
   int use_learned;
+  int wave_search; // plain lookups by wavefront-cooperative search (else per lane)
+  int pad_ws;
 };
 
 } // namespace omc

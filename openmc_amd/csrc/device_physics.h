@@ -935,6 +935,185 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
   }
 }
 
+// Minimum and maximum of an int over the lanes of a wavefront (every lane
+// must call them); the result is wavefront-uniform.
+__device__ __forceinline__ int wave_min_int(int x)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int y = __shfl_xor(x, o);
+    x = y < x ? y : x;
+  }
+  return wave_uniform(x);
+}
+
+__device__ __forceinline__ int wave_max_int(int x)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int y = __shfl_xor(x, o);
+    x = y > x ? y : x;
+  }
+  return wave_uniform(x);
+}
+
+constexpr int WAVE_WINDOW = 64; // grid points one wavefront load covers (one per lane)
+
+// Nuclide::calculate_xs for K nuclides of the wavefront's material, the
+// searches done by the wavefront together.  Every lane of the wavefront must
+// call it: lanes with act == false take part in the loads and cross-lane
+// reads only.
+//
+// The lanes of a sorted calculate-xs queue hold nearby energies of one
+// material, so for each nuclide their log-grid buckets [lo, hi) lie in a
+// short window [lo_w, hi_w) of the grid (lo_w and hi_w are the bucket bounds
+// at the smallest and largest log-grid index of the wavefront: the bucket
+// table is non-decreasing).  When the window has at most 64 points, lane j
+// loads E[lo_w + j] (one coalesced load for the wavefront) and each lane runs
+// lower_bound over exactly its own bucket [lo, hi) on those registers,
+// reading other lanes' values with cross-lane moves.  The index is therefore
+// the one the search in memory gives (include/openmc/search.h:13-20 with the
+// bucket-start case, src/nuclide.cpp:732-742), and the bracket energies
+// E[i], E[i+1] come from the same registers.  A nuclide whose window is wider
+// takes the per-lane search in memory.
+template<int K>
+__device__ __forceinline__ void nuclide_xs_wave(const DevProblem& P, const DevMatNuc* mn, int kk, double E,
+  int i_log, int il_lo, int il_hi, bool act, int& probes, Micro* out)
+{
+  const int lane = static_cast<int>(__lane_id());
+  const double* Eg[K];
+  int64_t off[K];
+  int n[K], lo[K], hi[K], lo_w[K], ig[K], first[K], cnt[K];
+  bool coop[K], in[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const DevMatNuc* r = mn + (k < kk ? k : 0);
+    const int nuc = wave_uniform(r->nuc);
+    off[k] = wave_uniform(r->off);
+    n[k] = wave_uniform(r->n);
+    const double e_first = wave_uniform(r->e_first);
+    const double e_last = wave_uniform(r->e_last);
+    Eg[k] = P.E + off[k];
+    const int* gi = P.grid_index + static_cast<int64_t>(nuc) * (P.n_log_bins + 1);
+    lo_w[k] = wave_uniform(gi[il_lo]);
+    const int hi_w = wave_uniform(gi[il_hi + 1]) + 1;
+    coop[k] = hi_w - lo_w[k] <= WAVE_WINDOW;
+    lo[k] = gi[i_log];
+    hi[k] = gi[i_log + 1] + 1;
+    const bool below = E < e_first;
+    const bool above = !below && E > e_last;
+    in[k] = act && k < kk && !below && !above;
+    ig[k] = below ? 0 : n[k] - 2;
+  }
+  // the window loads, one coalesced load per nuclide
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (coop[k]) {
+      int j = lo_w[k] + lane;
+      j = j > n[k] - 1 ? n[k] - 1 : j;
+      v[k] = Eg[k][j];
+    } else {
+      v[k] = 0.0;
+    }
+  }
+  // lower_bound over [lo, hi): on the window registers (coop) or in memory,
+  // all K searches in lockstep; the loop runs while any lane of the
+  // wavefront still searches, so the cross-lane reads see every lane
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    first[k] = in[k] ? (coop[k] ? lo[k] - lo_w[k] : lo[k]) : 0;
+    cnt[k] = in[k] ? hi[k] - lo[k] : 0;
+    if (in[k])
+      ++probes;
+  }
+  while (true) {
+    bool mine = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      mine = mine || cnt[k] > 0;
+    if (__ballot(mine) == 0ull)
+      break;
+    double vm[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int mid = first[k] + (cnt[k] >> 1);
+      if (coop[k])
+        vm[k] = __shfl(v[k], mid & (WAVE_WINDOW - 1));
+      else
+        vm[k] = cnt[k] > 0 ? Eg[k][mid] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (cnt[k] > 0) {
+        if (!coop[k])
+          ++probes;
+        const int step = cnt[k] >> 1;
+        if (vm[k] < E) {
+          first[k] += step + 1;
+          cnt[k] -= step + 1;
+        } else {
+          cnt[k] = step;
+        }
+      }
+    }
+  }
+  // index and bracket energies
+  double e0[K], e1[K];
+  bool have[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int lb = coop[k] ? lo_w[k] + first[k] : first[k];
+    // E[lb] for the bucket-start case
+    double e_lb;
+    if (coop[k])
+      e_lb = __shfl(v[k], first[k] & (WAVE_WINDOW - 1));
+    else
+      e_lb = in[k] && lb == lo[k] ? Eg[k][lb] : 0.0;
+    if (in[k])
+      ig[k] = lb == lo[k] && e_lb == E ? lb : lb - 1;
+    int i = ig[k];
+    i = i < 0 ? 0 : (i > n[k] - 2 ? n[k] - 2 : i);
+    ig[k] = i;
+    const int r0 = i - lo_w[k];
+    have[k] = coop[k] && r0 >= 0 && r0 + 1 < WAVE_WINDOW;
+    if (coop[k]) {
+      e0[k] = __shfl(v[k], r0 & (WAVE_WINDOW - 1));
+      e1[k] = __shfl(v[k], (r0 + 1) & (WAVE_WINDOW - 1));
+    }
+  }
+  double2 x0[K], x1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = ig[k];
+    if (!have[k]) {
+      e0[k] = Eg[k][i];
+      e1[k] = Eg[k][i + 1];
+    }
+    x0[k] = P.XS[off[k] + i];
+    x1[k] = P.XS[off[k] + i + 1];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k >= kk)
+      continue;
+    int i = ig[k];
+    // duplicate grid point: the upper interval (src/nuclide.cpp:740-742)
+    if (i + 2 < n[k] && e0[k] == e1[k]) {
+      ++i;
+      e0[k] = e1[k];
+      e1[k] = Eg[k][i + 1];
+      x0[k] = x1[k];
+      x1[k] = P.XS[off[k] + i + 1];
+    }
+    const double f = (E - e0[k]) / (e1[k] - e0[k]);
+    out[k].i_grid = i;
+    out[k].f = f;
+    out[k].total = (1.0 - f) * x0[k].x + f * x1[k].x;
+    out[k].absorption = (1.0 - f) * x0[k].y + f * x1[k].y;
+  }
+}
+
 //==============================================================================
 // Tally: RegularMesh ray-march (src/mesh.cpp:1000-1146, 1328-1369)
 //==============================================================================
@@ -989,9 +1168,23 @@ __device__ __forceinline__ int64_t mesh_bin(const M& T, const int* ijk)
   }
 }
 
+// Index of the XCD (accelerator complex die) the wavefront runs on.
+__device__ __forceinline__ int xcd_id()
+{
+  unsigned x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+  return static_cast<int>(x);
+}
+
+// The VALUE array has n_copies copies of the tally's bins; a wavefront adds
+// into the copy of its XCD, so the lines a copy's atomics touch live in one
+// XCD's L2 (k_accumulate sums the copies in copy order at the end of a batch).
 __device__ __forceinline__ void tally_add(const DevTally& T, int64_t bin, double score)
 {
-  unsafeAtomicAdd(&T.value[bin], score);
+  double* v = T.value;
+  if (T.n_copies > 1)
+    v += static_cast<int64_t>(xcd_id() & (T.n_copies - 1)) * T.n_bins;
+  unsafeAtomicAdd(&v[bin], score);
 }
 
 // score_tracklength_tally with [mesh] or [mesh, energy] filters

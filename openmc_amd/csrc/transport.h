@@ -39,7 +39,8 @@ void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig
 // the calculate-xs partition's key under sort_every).
 void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st,
   int cycle = 0);
-void launch_accumulate(double* value, double* results, int64_t n_bins, double norm, hipStream_t st);
+// value holds n_copies copies of the tally's n_bins bins (summed in copy order)
+void launch_accumulate(double* value, double* results, int64_t n_bins, int n_copies, double norm, hipStream_t st);
 
 // Standalone batched lookups (xs_lookup.hip): E[n] -> i_grid[n], xs[n][2]
 void launch_lookup_batch(const DevProblem& P, int i_nuc, const double* E, int64_t n, int learned, int* i_grid,

@@ -684,6 +684,58 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
   return nn;
 }
 
+// particle_xs with the wavefront-cooperative searches (nuclide_xs_wave):
+// every lane of the wavefront calls it; lanes with act == false (another
+// material, or no particle) take part in the cross-lane work and store
+// nothing.  Returns the lookups done by this lane.
+template<int G>
+__device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevParticles& S, int p, PHot* h,
+  int material, double E_in, bool act, bool trace, int& probes)
+{
+  const double E = act ? E_in : P.energy_min;
+  const double lx = detm::log(E / P.energy_min);
+  int i_log = static_cast<int>(lx / P.log_spacing);
+  i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+  // the log-grid range of the participating lanes
+  const int il_lo = wave_min_int(act ? i_log : P.n_log_bins);
+  const int il_hi = wave_max_int(act ? i_log : -1);
+  const int j0 = wave_uniform(P.mat_off[material]);
+  const int nn = wave_uniform(P.mat_off[material + 1]) - j0;
+  double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+  uint64_t hh = 0;
+  if (trace && act) {
+    hh = S.hash[p];
+    hmix(hh, 0x11);
+    hmix(hh, static_cast<uint64_t>(material - part_base(P, h->cell0).mat));
+  }
+  double xs_t = 0.0;
+  double xs_a = 0.0;
+  for (int jj = 0; jj < nn; jj += G) {
+    const int kk = nn - jj < G ? nn - jj : G;
+    Micro m[G];
+    nuclide_xs_wave<G>(P, P.matnuc + j0 + jj, kk, E, i_log, il_lo, il_hi, act, probes, m);
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k < kk) {
+        if (act)
+          micro[jj + k] = m[k].total;
+        const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
+        xs_t += dens * m[k].total;
+        xs_a += dens * m[k].absorption;
+        if (trace)
+          hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
+      }
+    }
+  }
+  if (act) {
+    h->xs_t = xs_t;
+    h->xs_a = xs_a;
+    if (trace)
+      S.hash[p] = hh;
+  }
+  return act ? nn : 0;
+}
+
 // One lane per particle: the lane looks up the material's nuclides G at a
 // time (their searches interleaved) and forms the sums in material order as
 // it goes, so no LDS is needed and occupancy is set by registers alone.
@@ -730,9 +782,15 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
       const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
       const int mat = __builtin_amdgcn_readfirstlane(__shfl(material, leader));
       const bool mine = need && material == mat;
-      if (mine) {
+      if (LEARNED || !S.wave_search) {
+        if (mine) {
+          int probes = 0;
+          n_lookups += particle_xs<G, LEARNED>(P, S, p, h, mat, E, trace, probes);
+          n_probes += probes;
+        }
+      } else {
         int probes = 0;
-        n_lookups += particle_xs<G, LEARNED>(P, S, p, h, mat, E, trace, probes);
+        n_lookups += particle_xs_wave<G>(P, S, p, h, mat, E, mine, trace, probes);
         n_probes += probes;
       }
       todo &= ~__ballot(mine);
@@ -798,7 +856,7 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S,
         hp->time = h.time + dt;
         if (P.tally.enabled) {
           if (MULTI) {
-            const DevTally T = P.part[h.cell0 >= 0 ? P.cell[h.cell0].part : 0].tally;
+            const DevTally& T = P.part[h.cell0 >= 0 ? P.cell[h.cell0].part : 0].tally;
             if (T.n_bins > 0)
               n_segs += score_tracklength(T, r0, r1, g.u(), E, wgt, distance);
           } else {
@@ -1191,12 +1249,22 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
 // Tally accumulation (src/tallies/tally.cpp:821-878)
 //==============================================================================
 
-__global__ void __launch_bounds__(BLOCK) k_accumulate(double* value, double* results, int64_t n_bins, double norm)
+// Tally::accumulate (src/tallies/tally.cpp:821-878) for one tally whose
+// VALUE has n_copies copies of its bins: the copies are summed in copy
+// order, then SUM += VALUE * norm and SUM_SQ += (VALUE * norm)^2, and VALUE
+// is cleared.
+__global__ void __launch_bounds__(BLOCK) k_accumulate(double* value, double* results, int64_t n_bins, int n_copies,
+  double norm)
 {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_bins;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    double val = value[i] * norm;
+    double sum = value[i];
     value[i] = 0.0;
+    for (int c = 1; c < n_copies; ++c) {
+      sum += value[c * n_bins + i];
+      value[c * n_bins + i] = 0.0;
+    }
+    double val = sum * norm;
     results[3 * i + 1] += val;
     results[3 * i + 2] += val * val;
   }
@@ -1353,11 +1421,11 @@ void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig
     launch_kernel(k, P, S, lc, st, cycle);
 }
 
-void launch_accumulate(double* value, double* results, int64_t n_bins, double norm, hipStream_t st)
+void launch_accumulate(double* value, double* results, int64_t n_bins, int n_copies, double norm, hipStream_t st)
 {
   int64_t b = (n_bins + BLOCK - 1) / BLOCK;
   int blocks = static_cast<int>(b > 4096 ? 4096 : (b < 1 ? 1 : b));
-  hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(BLOCK), 0, st, value, results, n_bins, norm);
+  hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(BLOCK), 0, st, value, results, n_bins, n_copies, norm);
 }
 
 } // namespace omc
