@@ -100,7 +100,7 @@ struct Sim {
   double* d_value = nullptr; // VALUE of every tally: [n_copies][bins] at HostTally::off * n_copies
   void* value_alloc = nullptr;
   int tally_copies = 1; // copies of each tally's VALUE (up to one per XCD; 1 measured fastest)
-  bool wave_search = true; // wavefront-cooperative grid searches (plain lookups)
+  bool wave_search = false; // wavefront-cooperative grid searches (plain lookups; measured 4% slower)
   std::vector<DevPart> h_parts;
   void* parts_alloc = nullptr;
   bool results_dirty = false;
@@ -473,7 +473,9 @@ void alloc_particles(Sim& s, int64_t cap)
   const int64_t deep = std::max(1, s.P.n_levels - 1);
   S.hot = dev_alloc<PHot>(L, cap);
   S.lev = dev_alloc<PLevel>(L, static_cast<size_t>(deep * cap));
-  S.max_nn = std::max(1, s.P.max_mat_nuc);
+  // rows padded to a multiple of four values: stores and loads of the row
+  // are 16-byte pairs
+  S.max_nn = (std::max(1, s.P.max_mat_nuc) + 3) & ~3;
   S.micro_t = dev_alloc<double>(L, static_cast<size_t>(S.max_nn) * cap);
   S.lost = dev_alloc<int>(L, cap);
   S.hash = dev_alloc<uint64_t>(L, cap);
@@ -533,7 +535,7 @@ int64_t bytes_per_particle(const Sim& s)
   // and staging arrays, and the weight-window record and bank when on
   const int64_t ww = s.P.ww.on ? static_cast<int64_t>(sizeof(PWW)) + s.bank_cap * sizeof(PSite) : 0;
   return static_cast<int64_t>(sizeof(PHot)) + deep * static_cast<int64_t>(sizeof(PLevel)) +
-         std::max(1, s.P.max_mat_nuc) * 8 + 4 + 8 + 5 * 4 + ww;
+         ((std::max(1, s.P.max_mat_nuc) + 3) & ~3) * 8 + 4 + 8 + 5 * 4 + ww;
 }
 
 void choose_capacity(Sim& s)

@@ -9,6 +9,7 @@
 // order, so each particle's state is kept in as few cache lines as possible.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace omc {
@@ -275,6 +276,8 @@ constexpr int LEAK_PART_SLOTS = 16;
 // fills on this path: find_cell_inner copies it down unchanged,
 // src/geometry.cpp:196-197, and collisions copy it down, src/particle.cpp:396-406).
 struct alignas(128) PHot {
+  // (xs_t, xs_a are stored together as one 16-byte pair: keep them adjacent
+  // at a 16-byte aligned offset)
   double rx, ry, rz; // level-0 position
   double ux, uy, uz; // direction
   double E;
@@ -293,6 +296,8 @@ struct alignas(128) PHot {
   int n_event;
 };
 static_assert(sizeof(PHot) == 128, "one cache line per particle");
+static_assert(offsetof(PHot, xs_a) == offsetof(PHot, xs_t) + 8 && offsetof(PHot, xs_t) % 16 == 0,
+  "xs_t, xs_a form one aligned 16-byte pair");
 
 // Coordinate level >= 1 of one particle (64 bytes).
 struct alignas(64) PLevel {

@@ -640,6 +640,25 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
   wave_count(S, CNT_SEARCH_PROBES, n_probes);
 }
 
+// The kk (<= G) nuclide totals of a group into the particle's micro_t row,
+// two per 16-byte store: the group starts at a multiple of G in a row padded
+// to a multiple of four values, so the pairs are 16-byte aligned.
+template<int G>
+__device__ __forceinline__ void store_micro(double* dst, const Micro* m, int kk)
+{
+  if (G == 1) {
+    dst[0] = m[0].total;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < G; k += 2) {
+    if (k + 1 < kk)
+      *reinterpret_cast<double2*>(dst + k) = double2 {m[k].total, m[k + 1].total};
+    else if (k < kk)
+      dst[k] = m[k].total;
+  }
+}
+
 // Macroscopic cross sections of one particle (material given), nuclides
 // looked up G at a time; sums in material order.  Returns the lookups done.
 // The material must be the same in every active lane of the wavefront.
@@ -665,10 +684,10 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
     const int kk = nn - jj < G ? nn - jj : G;
     Micro m[G];
     nuclide_xs_group<G, LEARNED>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
+    store_micro<G>(micro + jj, m, kk);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
-        micro[jj + k] = m[k].total;
         const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
@@ -677,8 +696,8 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
       }
     }
   }
-  h->xs_t = xs_t;
-  h->xs_a = xs_a;
+  // xs_t and xs_a are adjacent and 16-byte aligned in the hot record: one store
+  *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
   if (trace)
     S.hash[p] = hh;
   return nn;
@@ -714,11 +733,11 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
     const int kk = nn - jj < G ? nn - jj : G;
     Micro m[G];
     nuclide_xs_wave<G>(P, P.matnuc + j0 + jj, kk, E, i_log, il_lo, il_hi, act, probes, m);
+    if (act)
+      store_micro<G>(micro + jj, m, kk);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
-        if (act)
-          micro[jj + k] = m[k].total;
         const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
@@ -728,8 +747,7 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
     }
   }
   if (act) {
-    h->xs_t = xs_t;
-    h->xs_a = xs_a;
+    *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
     if (trace)
       S.hash[p] = hh;
   }
@@ -1124,17 +1142,29 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
       double cutoff = prn(&seed) * h.xs_t;
       const int j0 = P.mat_off[material];
       const int j1 = P.mat_off[material + 1];
-      const double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+      // the row is read two values per 16-byte load (rows are padded to a
+      // multiple of four values)
+      const double2* micro2 = reinterpret_cast<const double2*>(S.micro_t + static_cast<int64_t>(p) * S.max_nn);
       double prob = 0.0;
       int i_nuclide = -1;
       int j_nuclide = -1; // its entry in the material's list
-      for (int j = j0; j < j1; ++j) {
+      for (int j = j0; j < j1; j += 2) {
+        const double2 mm = micro2[(j - j0) >> 1];
         const double dens = P.mat_dens[j] * 1.0;
-        prob += dens * micro[j - j0];
+        prob += dens * mm.x;
         if (prob >= cutoff) {
           i_nuclide = P.mat_nuc[j];
           j_nuclide = j;
           break;
+        }
+        if (j + 1 < j1) {
+          const double dens1 = P.mat_dens[j + 1] * 1.0;
+          prob += dens1 * mm.y;
+          if (prob >= cutoff) {
+            i_nuclide = P.mat_nuc[j + 1];
+            j_nuclide = j + 1;
+            break;
+          }
         }
       }
       bool absorbed = false;

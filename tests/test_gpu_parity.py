@@ -140,7 +140,7 @@ def test_transport_graph_and_direct_launch_identical():
                                   {"sort": 1, "xs_lanes": 1, "xs_group": 8},
                                   {"block": 64, "blocks": 7}, {"block": 128},
                                   {"sort_every": 4, "cycles_per_check": 8}, {"sort_every": 3, "sort_bits": 0},
-                                  {"wave_search": 0}, {"wave_search": 0, "xs_group": 8}, {"tally_copies": 8},
+                                  {"wave_search": 1}, {"wave_search": 1, "xs_group": 2}, {"tally_copies": 8},
                                   {"sort": 0, "xs_group": 8}])
 def test_transport_launch_shapes_identical(opts):
     """Queue sorting and lanes-per-particle change the order of work, never a history."""
