@@ -74,9 +74,12 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
             info.update({"learned_" + k: v for k, v in lib.learned_stats().items()})
             lib.set_option("learned_index", 1)
         lib.set_option("event_timing", 1)
+        lib.set_option("progress_s", 30)
         lib.simulation_init()
-        for _ in range(warmup):
+        tw = time.perf_counter()
+        for k in range(warmup):
             lib.next_batch()
+            log("  warm-up batch %d done (%.2f s)" % (k + 1, time.perf_counter() - tw))
         lib.reset_timers()
         s0 = lib.stats()
         t0 = time.perf_counter()
@@ -151,7 +154,7 @@ def cpu_baseline(d, target_s=10.0):
     path = os.path.join(d, "oracle_problem.txt")
 
     def timed(event_based, n):
-        o = Oracle(path)
+        o = Oracle(path)  # loading the data is not timed
         t = time.perf_counter()
         o.run(event_based=event_based, threads=threads, n_particles=n, n_batches=1, trace_hash=False)
         el = time.perf_counter() - t
@@ -161,9 +164,19 @@ def cpu_baseline(d, target_s=10.0):
 
     modes = {}
     for name, event_based in (("history", False), ("event", True)):
-        probe, _ = timed(event_based, 2000)
-        n = int(max(2000, min(2_000_000, 2000 * target_s / max(probe, 1e-3))))
-        el, c = timed(event_based, n)
+        # probes of growing size until one takes a tenth of the target, then
+        # one sample sized to the target (every step logged)
+        n, el = 64, 0.0
+        while True:
+            el, c = timed(event_based, n)
+            log("  cpu %s probe: %d histories in %.2f s" % (name, n, el))
+            if el >= 0.1 * target_s or n >= 2_000_000:
+                break
+            n = int(min(2_000_000, n * min(16.0, max(2.0, 0.1 * target_s / max(el, 1e-3)))))
+        if el < 0.5 * target_s:
+            n = int(max(n, min(2_000_000, n * target_s / max(el, 1e-3))))
+            el, c = timed(event_based, n)
+            log("  cpu %s sample: %d histories in %.2f s" % (name, n, el))
         modes[name] = {"particles_per_s": round(n / el, 2), "particles": n, "seconds": round(el, 2),
                        "lookups_per_s": round(c["lookups"] / el, 1)}
     best = max(modes, key=lambda k: modes[k]["particles_per_s"])

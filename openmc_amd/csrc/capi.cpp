@@ -100,6 +100,7 @@ struct Sim {
   double* d_value = nullptr; // VALUE of every tally: [n_copies][bins] at HostTally::off * n_copies
   void* value_alloc = nullptr;
   int tally_copies = 1; // copies of each tally's VALUE (up to one per XCD; 1 measured fastest)
+  double progress_s = 0; // seconds between progress lines on stderr during a batch (0: none)
   bool wave_search = false; // wavefront-cooperative grid searches (plain lookups; measured 4% slower)
   std::vector<DevPart> h_parts;
   void* parts_alloc = nullptr;
@@ -658,9 +659,14 @@ void transport_batch(Sim& s)
     // (nothing to stage for the next one).  Every history ends within
     // max_particle_events events (one per cycle), so more cycles than that
     // means a kernel failed to retire particles.
+    // Without weight windows a slot holds one history, so more cycles than
+    // max_particle_events means a kernel failed to retire particles.  With
+    // them a slot runs its whole split tree (the secondaries are revived in
+    // the same slot), which has no such bound.
     long long cycles = 0;
+    auto last_note = std::chrono::steady_clock::now();
     while (true) {
-      if (cycles > m.max_events + 16)
+      if (!s.P.ww.on && cycles > m.max_events + 16)
         throw std::runtime_error("event loop did not drain within max_particle_events cycles");
       run_cycles(s, s.cycles_per_check);
       cycles += s.cycles_per_check;
@@ -672,6 +678,15 @@ void transport_batch(Sim& s)
         flush_kernel_timers(s);
       if (s.h_qcount[QC_STAGE] == 0)
         break;
+      if (s.progress_s > 0) {
+        auto now = std::chrono::steady_clock::now();
+        if (std::chrono::duration<double>(now - last_note).count() >= s.progress_s) {
+          std::fprintf(stderr, "omc: batch %d: %lld cycles, %d particles in flight\n", current_batch, cycles,
+            s.h_qcount[QC_STAGE]);
+          std::fflush(stderr);
+          last_note = now;
+        }
+      }
     }
     if (s.trace_hash)
       hip_check(hipMemcpy(s.h_hashes.data() + offset, s.S.hash, sizeof(uint64_t) * n, hipMemcpyDeviceToHost),
@@ -1242,6 +1257,8 @@ int omc_set_option(const char* name, double value)
     }
     g->block_opt = v;
     update_launch_config(*g);
+  } else if (n == "progress_s") {
+    g->progress_s = value;
   } else if (n == "wave_search") {
     g->wave_search = value != 0;
   } else if (n == "tally_copies") {
