@@ -44,13 +44,23 @@ def xs_lookups(st):
     return st["lookups"] - st["collisions"]
 
 
+def kxs_lookups(st):
+    """The calculate-xs lookups done by the k_xs kernel (not the history-based tail)."""
+    return xs_lookups(st) - st.get("hist_lookups", 0)
+
+
 def kernel_bytes(st):
-    """Algorithmic bytes moved by each event kernel over the measured region."""
+    """Algorithmic bytes moved by each event kernel over the measured region
+    (the events of the history-based tail are credited to k_history)."""
+    hx, hl, ha, hs = st.get("hist_xs_events", 0), st.get("hist_lookups", 0), st.get("hist_adv_events", 0), \
+        st.get("hist_segments", 0)
+    hsu, hc = st.get("hist_surf_events", 0), st.get("hist_coll_events", 0)
     return {
-        "k_xs": B_LOOKUP * xs_lookups(st) + B_EVENT * st["xs_events"],
-        "k_advance": B_SEG * st["segments"] + B_EVENT * st["adv_events"],
-        "k_surface": B_EVENT * st["surf_events"],
-        "k_collision": B_COLL * st["collisions"] + B_EVENT * st["coll_events"],
+        "k_xs": B_LOOKUP * (xs_lookups(st) - hl) + B_EVENT * (st["xs_events"] - hx),
+        "k_advance": B_SEG * (st["segments"] - hs) + B_EVENT * (st["adv_events"] - ha),
+        "k_surface": B_EVENT * (st["surf_events"] - hsu),
+        "k_collision": B_COLL * (st["collisions"] - hc) + B_EVENT * (st["coll_events"] - hc),
+        "k_history": B_LOOKUP * hl + B_SEG * hs + B_COLL * hc + B_EVENT * (hx + ha + hsu + hc),
     }
 
 
@@ -115,7 +125,8 @@ def traffic_per_launch(kernel, st, launches):
         for line in f:
             parts = line.split()
             if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
-                return round(float(parts[3]) * st[UNIT_COUNTER[parts[2]]] / launches, 1)
+                c = UNIT_COUNTER[parts[2]]
+                return round(float(parts[3]) * (st[c] - st.get("hist_" + c, 0)) / launches, 1)
     return None
 
 
@@ -123,10 +134,10 @@ def roofline(info):
     st = info["stats"]
     times = {"k_xs": st["t_xs_ms"], "k_advance": st["t_adv_ms"], "k_surface": st["t_surf_ms"],
              "k_collision": st["t_coll_ms"], "partition_xs": st["t_sort_ms"],
-             "partition_split": st["t_split_ms"], "k_init": st["t_init_ms"]}
+             "partition_split": st["t_split_ms"], "k_init": st["t_init_ms"], "k_history": st.get("t_hist_ms", 0.0)}
     launches = {"k_xs": st["n_xs"], "k_advance": st["n_adv"], "k_surface": st["n_surf"],
                 "k_collision": st["n_coll"], "partition_xs": st["n_sort"], "partition_split": st["n_split"],
-                "k_init": st["n_init"]}
+                "k_init": st["n_init"], "k_history": st.get("n_hist", 0)}
     by = kernel_bytes(st)
     # queue bookkeeping and source sampling carry no algorithmic bytes
     by["partition_xs"] = 0
@@ -288,14 +299,16 @@ def bench_problem_set(args):
         "config": {"workload": "batched-%d-lattice-problems-%.3gpps" % (len(dirs), total), "problems": len(dirs),
                    "particles_per_batch": total, "nuclides": len(nucs),
                    "mode": "event-based, one event loop shared by all problems"},
-        "xs_lookups_per_s": round(xs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
+        "xs_lookups_per_s": round(kxs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
         "sequential": {"particles_per_s": round(total / seq_s, 2), "seconds_per_batch": round(seq_s, 4)},
         "batching_speedup": round(value / (total / seq_s), 3),
         "per_problem": alone,
         "launch_options": rs["options"],
         "kernels": per_kernel,
-        "events": {k: st[k] for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
-                                      "collisions", "segments", "search_probes", "cycles")},
+        "events": {k: st.get(k, 0) for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
+                                             "collisions", "segments", "search_probes", "cycles", "hist_launches",
+                                             "hist_xs_events", "hist_lookups", "hist_adv_events",
+                                             "hist_segments", "hist_surf_events", "hist_coll_events")},
         "roofline": roof,
     }
     if "autotune" in rs:
@@ -446,13 +459,15 @@ def main():
         "config": {"workload": "lattice-17x17-60nuclides-1e7pps-mesh100^3" if args.config == 1 else
                    "config%d" % args.config, "particles_per_batch": n, "nuclides": len(nucs),
                    "grid_points": int(sum(x.n_points for x in nucs)), "mode": "event-based"},
-        "xs_lookups_per_s": round(xs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
+        "xs_lookups_per_s": round(kxs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
         "xs_lookups_per_s_transport": round(xs_lookups(st) / (st["transport_ms"] * 1e-3), 1),
         "events_per_particle": round((st["xs_events"] + st["adv_events"] + st["surf_events"] +
                                       st["coll_events"]) / (n * args.steps), 2),
         "kernels": per_kernel,
-        "events": {k: st[k] for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
-                                      "collisions", "segments", "search_probes", "cycles")},
+        "events": {k: st.get(k, 0) for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
+                                             "collisions", "segments", "search_probes", "cycles", "hist_launches",
+                                             "hist_xs_events", "hist_lookups", "hist_adv_events",
+                                             "hist_segments", "hist_surf_events", "hist_coll_events")},
         "roofline": roof,
     }
     if copy_info:
@@ -463,7 +478,7 @@ def main():
         lst = lr["stats"]
         out["learned"] = {
             "particles_per_s": round(n * args.steps / lr["elapsed_s"], 2),
-            "xs_lookups_per_s": round(xs_lookups(lst) / lst["t_xs_ms"] * 1e3, 1),
+            "xs_lookups_per_s": round(kxs_lookups(lst) / lst["t_xs_ms"] * 1e3, 1),
             "speedup_vs_binary": round(plain["elapsed_s"] / lr["elapsed_s"], 4),
             "probes_per_lookup": round(lst["search_probes"] / max(lst["lookups"], 1), 3),
             "probes_per_lookup_binary": round(st["search_probes"] / max(st["lookups"], 1), 3),

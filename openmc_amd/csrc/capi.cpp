@@ -79,8 +79,8 @@ constexpr int GT_LEAKAGE = 3;
 struct KernelTimer {
   std::vector<hipEvent_t> start, stop;
   std::vector<int> kind;
-  double ms[8] = {0};
-  long long launches[8] = {0};
+  double ms[KERNEL_KINDS] = {0};
+  long long launches[KERNEL_KINDS] = {0};
 };
 
 struct Sim {
@@ -101,6 +101,9 @@ struct Sim {
   void* value_alloc = nullptr;
   int tally_copies = 1; // copies of each tally's VALUE (up to one per XCD; 1 measured fastest)
   double progress_s = 0; // seconds between progress lines on stderr during a batch (0: none)
+  int history_below = 65536; // finish a batch history-based once fewer particles than this are live (0: never)
+  int history_steps = 2048;  // event cycles per particle per history-based launch
+  long long n_hist_launches = 0;
   bool wave_search = false; // wavefront-cooperative grid searches (plain lookups; measured 4% slower)
   std::vector<DevPart> h_parts;
   void* parts_alloc = nullptr;
@@ -615,6 +618,50 @@ void run_cycles(Sim& s, int n_cycles)
   s.n_cycles += n_cycles;
 }
 
+// The tail of a batch, history-based: launches of k_history until no
+// particle is left alive (each launch runs every live particle for at most
+// history_steps cycles, so launches stay short).
+void run_history_tail(Sim& s, std::chrono::steady_clock::time_point& last_note)
+{
+  const int n_staged = s.h_qcount[QC_STAGE];
+  long long rounds = 0;
+  while (true) {
+    s.h_qcount[QC_LIVE] = 0;
+    hip_check(hipMemcpyAsync(s.S.qcount + QC_LIVE, s.h_qcount + QC_LIVE, sizeof(int), hipMemcpyHostToDevice,
+                s.stream),
+      "hipMemcpyAsync");
+    if (s.event_timing)
+      record_kernel(s, KERNEL_HISTORY, true);
+    launch_history(s.P, s.S, n_staged, s.history_steps, s.stream);
+    if (s.event_timing)
+      record_kernel(s, KERNEL_HISTORY, false);
+    hip_check(hipMemcpyAsync(s.h_qcount + QC_LIVE, s.S.qcount + QC_LIVE, sizeof(int), hipMemcpyDeviceToHost,
+                s.stream),
+      "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+    s.n_hist_launches++;
+    ++rounds;
+    if (s.event_timing)
+      flush_kernel_timers(s);
+    if (s.h_qcount[QC_LIVE] == 0)
+      break;
+    // every launch retires or advances each live particle by history_steps
+    // cycles; a history that outlives max_particle_events cycles (without
+    // weight windows) means a failure to retire it
+    if (!s.P.ww.on && rounds * static_cast<long long>(s.history_steps) > s.m.max_events + 16)
+      throw std::runtime_error("history-based tail did not finish within max_particle_events cycles");
+    if (s.progress_s > 0) {
+      auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last_note).count() >= s.progress_s) {
+        std::fprintf(stderr, "omc: batch %d: history-based tail, %lld launches, %d particles alive\n",
+          current_batch, rounds, s.h_qcount[QC_LIVE]);
+        std::fflush(stderr);
+        last_note = now;
+      }
+    }
+  }
+}
+
 void transport_batch(Sim& s)
 {
   HostModel& m = s.m;
@@ -678,6 +725,13 @@ void transport_batch(Sim& s)
         flush_kernel_timers(s);
       if (s.h_qcount[QC_STAGE] == 0)
         break;
+      // few particles left: finish them history-based (the staged entries
+      // of the last cycle are exactly the live particles, each about to
+      // start its next calculate-xs event)
+      if (s.h_qcount[QC_XS] < s.history_below) {
+        run_history_tail(s, last_note);
+        break;
+      }
       if (s.progress_s > 0) {
         auto now = std::chrono::steady_clock::now();
         if (std::chrono::duration<double>(now - last_note).count() >= s.progress_s) {
@@ -1008,7 +1062,7 @@ int openmc_reset_timers(void)
     g->t_transport_ms = 0;
     g->t_batch_wall_s = 0;
     g->active_batches = 0;
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < KERNEL_KINDS; ++k) {
       g->kt.ms[k] = 0;
       g->kt.launches[k] = 0;
     }
@@ -1257,6 +1311,13 @@ int omc_set_option(const char* name, double value)
     }
     g->block_opt = v;
     update_launch_config(*g);
+  } else if (n == "history_below" || n == "history_steps") {
+    int v = static_cast<int>(value);
+    if (v < 0 || (n == "history_steps" && v < 1)) {
+      set_errmsg(n + " out of range");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
+    (n == "history_below" ? g->history_below : g->history_steps) = v;
   } else if (n == "progress_s") {
     g->progress_s = value;
   } else if (n == "wave_search") {
@@ -1308,8 +1369,9 @@ int omc_set_option(const char* name, double value)
 static const char* k_stat_names =
   "transport_ms,active_batches,lookups,collisions,segments,lost,xs_events,adv_events,surf_events,"
   "coll_events,search_probes,cycles,checks,capacity,t_sort_ms,t_xs_ms,t_adv_ms,t_split_ms,t_surf_ms,"
-  "t_coll_ms,t_init_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,wall_s,blocks,xs_lanes,sort,xs_group,"
-  "block,sort_every,cycles_per_check,sort_bits,sort_blocks";
+  "t_coll_ms,t_init_ms,t_hist_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,n_hist,wall_s,blocks,xs_lanes,"
+  "sort,xs_group,block,sort_every,cycles_per_check,sort_bits,sort_blocks,history_below,history_steps,"
+  "hist_launches,hist_xs_events,hist_lookups,hist_adv_events,hist_segments,hist_surf_events,hist_coll_events";
 
 const char* omc_stat_names(void)
 {
@@ -1349,6 +1411,11 @@ int omc_get_stats(double* out, int n)
   v.push_back(g->cycles_per_check);
   v.push_back(g->sort_bits);
   v.push_back(g->lc.sort_blocks);
+  v.push_back(g->history_below);
+  v.push_back(g->history_steps);
+  v.push_back(static_cast<double>(g->n_hist_launches));
+  for (int c = CNT_HIST_XS_EVENTS; c <= CNT_HIST_COLL_EVENTS; ++c)
+    v.push_back(static_cast<double>(g->counters[c]));
   for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
     out[k] = v[k];
   return static_cast<int>(v.size());

@@ -253,6 +253,13 @@ enum Counter : int {
   CNT_SURF_EVENTS,
   CNT_COLL_EVENTS,
   CNT_SEARCH_PROBES,
+  // the part of the counts above done by the history-based tail
+  CNT_HIST_XS_EVENTS,
+  CNT_HIST_LOOKUPS, // calculate-xs lookups only
+  CNT_HIST_ADV_EVENTS,
+  CNT_HIST_SEGMENTS,
+  CNT_HIST_SURF_EVENTS,
+  CNT_HIST_COLL_EVENTS,
   CNT_N
 };
 
@@ -261,7 +268,8 @@ enum Counter : int {
 //   QC_STAGE entries staged for the next calculate-xs partition
 //   QC_SURF  surface crossings at the front of q_sc
 //   QC_COLL  collisions after them (QC_COLL == QC_SURF + 1 is relied on)
-enum QueueCount : int { QC_XS = 0, QC_STAGE, QC_SURF, QC_COLL, QC_N };
+//   QC_LIVE  particles still alive after a history-based tail launch
+enum QueueCount : int { QC_XS = 0, QC_STAGE, QC_SURF, QC_COLL, QC_LIVE, QC_N };
 
 // Counters and the leakage sum are spread over this many slots (a wavefront
 // adds into slot wave_id % CNT_SLOTS) so no address sees more than a few

@@ -17,7 +17,8 @@ enum KernelKind : int {
   KERNEL_SURFACE = 4,
   KERNEL_COLLISION = 5,
   KERNEL_INIT = 6,
-  KERNEL_KINDS = 7
+  KERNEL_HISTORY = 7,   // history-based tail launches
+  KERNEL_KINDS = 8
 };
 
 struct LaunchConfig {
@@ -39,6 +40,11 @@ void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig
 // the calculate-xs partition's key under sort_every).
 void launch_kernel(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st,
   int cycle = 0);
+// History-based processing of the n_staged entries staged at the end of an
+// event cycle: each live particle runs up to max_steps whole cycles in one
+// lane; the number still alive afterwards is added to qcount[QC_LIVE].
+void launch_history(const DevProblem& P, const DevParticles& S, int n_staged, int max_steps, hipStream_t st);
+
 // value holds n_copies copies of the tally's n_bins bins (summed in copy order)
 void launch_accumulate(double* value, double* results, int64_t n_bins, int n_copies, double norm, hipStream_t st);
 

@@ -823,92 +823,100 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
 // advance event (src/particle.cpp:232-286, src/event.cpp:107-128)
 //==============================================================================
 
+// Event counts of one lane, summed per wavefront into the counter slots.
+struct EvCount {
+  unsigned long long events = 0, lost = 0, segs = 0, lookups = 0, probes = 0;
+};
+
+// Particle::event_advance for slot p (src/particle.cpp:232-286): distance to
+// boundary vs collision, move, time, track-length tallies.  Returns the
+// outcome: 0 surface crossing, 1 collision, -1 done.
+template<int NL, bool MULTI>
+__device__ __forceinline__ int advance_one(const DevProblem& P, const DevParticles& S, int p, EvCount& c)
+{
+  const PHot h = S.hot[p];
+  double wgt = h.wgt;
+  if (wgt == 0.0)
+    return -1;
+  c.events++;
+  Geo<NL> g;
+  load_geo<NL>(P, S, p, h, g);
+  const double E = h.E;
+  const double xs_t = h.xs_t;
+  uint64_t seed = h.seed;
+  Boundary b = distance_to_boundary(P, g);
+  double d_coll;
+  if (xs_t == 0.0)
+    d_coll = __builtin_inf();
+  else
+    d_coll = -detm::log(prn(&seed)) / xs_t;
+  double speed = C_LIGHT * sqrt(E * (E + 2 * MASS_NEUTRON_EV)) / (E + MASS_NEUTRON_EV);
+  double distance = b.distance;
+  if (d_coll < distance)
+    distance = d_coll;
+  if (INFTY < distance)
+    distance = INFTY;
+  const V3 r0 = g.r(0);
+  // move_distance (src/particle_data.cpp:86-91): r += distance * u on every level
+  const double mx = g.ux * distance, my = g.uy * distance, mz = g.uz * distance;
+  const V3 r1 = {g.rx[0] + mx, g.ry[0] + my, g.rz[0] + mz};
+  PHot* hp = S.hot + p;
+  hp->rx = r1.x;
+  hp->ry = r1.y;
+  hp->rz = r1.z;
+#pragma unroll
+  for (int l = 1; l < NL; ++l) {
+    if (l < g.n_coord) {
+      PLevel* L = level_rec<NL>(S, p, l);
+      L->rx = g.rx[l] + mx;
+      L->ry = g.ry[l] + my;
+      L->rz = g.rz[l] + mz;
+    }
+  }
+  double dt = distance / speed;
+  hp->time = h.time + dt;
+  if (P.tally.enabled) {
+    if (MULTI) {
+      const DevTally& T = P.part[h.cell0 >= 0 ? P.cell[h.cell0].part : 0].tally;
+      if (T.n_bins > 0)
+        c.segs += score_tracklength(T, r0, r1, g.u(), E, wgt, distance);
+    } else {
+      c.segs += score_tracklength(P.tally, r0, r1, g.u(), E, wgt, distance);
+    }
+  }
+  if (distance == INFTY)
+    wgt = 0.0;
+  if (g.lost) {
+    wgt = 0.0;
+    S.lost[p] = 1;
+    c.lost++;
+  }
+  hp->seed = seed;
+  hp->b_surface = b.surface;
+  hp->b_info = b.coord_level | ((b.trans[0] + 1) << 8) | ((b.trans[1] + 1) << 10) | ((b.trans[2] + 1) << 12);
+  hp->wgt = wgt;
+  if (S.trace_hash) {
+    uint64_t hh = S.hash[p];
+    hmix(hh, 0x22);
+    hmix(hh, dbits(distance));
+    hmix(hh, d_coll > b.distance ? 2 : 1);
+    S.hash[p] = hh;
+  }
+  if (wgt == 0.0)
+    return -1;
+  return d_coll > b.distance ? 0 : 1;
+}
+
 template<int NL, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S, const int* q)
 {
   const int n = S.qcount[QC_XS];
-  unsigned long long n_events = 0, n_segs = 0, n_lost = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    int outcome = -1;
-    {
-      const int p = q[i];
-      const PHot h = S.hot[p];
-      double wgt = h.wgt;
-      if (wgt != 0.0) {
-        n_events++;
-        Geo<NL> g;
-        load_geo<NL>(P, S, p, h, g);
-        const double E = h.E;
-        const double xs_t = h.xs_t;
-        uint64_t seed = h.seed;
-        Boundary b = distance_to_boundary(P, g);
-        double d_coll;
-        if (xs_t == 0.0)
-          d_coll = __builtin_inf();
-        else
-          d_coll = -detm::log(prn(&seed)) / xs_t;
-        double speed = C_LIGHT * sqrt(E * (E + 2 * MASS_NEUTRON_EV)) / (E + MASS_NEUTRON_EV);
-        double distance = b.distance;
-        if (d_coll < distance)
-          distance = d_coll;
-        if (INFTY < distance)
-          distance = INFTY;
-        const V3 r0 = g.r(0);
-        // move_distance (src/particle_data.cpp:86-91): r += distance * u on every level
-        const double mx = g.ux * distance, my = g.uy * distance, mz = g.uz * distance;
-        const V3 r1 = {g.rx[0] + mx, g.ry[0] + my, g.rz[0] + mz};
-        PHot* hp = S.hot + p;
-        hp->rx = r1.x;
-        hp->ry = r1.y;
-        hp->rz = r1.z;
-#pragma unroll
-        for (int l = 1; l < NL; ++l) {
-          if (l < g.n_coord) {
-            PLevel* L = level_rec<NL>(S, p, l);
-            L->rx = g.rx[l] + mx;
-            L->ry = g.ry[l] + my;
-            L->rz = g.rz[l] + mz;
-          }
-        }
-        double dt = distance / speed;
-        hp->time = h.time + dt;
-        if (P.tally.enabled) {
-          if (MULTI) {
-            const DevTally& T = P.part[h.cell0 >= 0 ? P.cell[h.cell0].part : 0].tally;
-            if (T.n_bins > 0)
-              n_segs += score_tracklength(T, r0, r1, g.u(), E, wgt, distance);
-          } else {
-            n_segs += score_tracklength(P.tally, r0, r1, g.u(), E, wgt, distance);
-          }
-        }
-        if (distance == INFTY)
-          wgt = 0.0;
-        if (g.lost) {
-          wgt = 0.0;
-          S.lost[p] = 1;
-          n_lost++;
-        }
-        hp->seed = seed;
-        hp->b_surface = b.surface;
-        hp->b_info = b.coord_level | ((b.trans[0] + 1) << 8) | ((b.trans[1] + 1) << 10) |
-                     ((b.trans[2] + 1) << 12);
-        hp->wgt = wgt;
-        if (S.trace_hash) {
-          uint64_t hh = S.hash[p];
-          hmix(hh, 0x22);
-          hmix(hh, dbits(distance));
-          hmix(hh, d_coll > b.distance ? 2 : 1);
-          S.hash[p] = hh;
-        }
-        if (wgt != 0.0)
-          outcome = d_coll > b.distance ? 0 : 1;
-      }
-    }
-    S.adv_key[i] = outcome;
-  }
-  wave_count(S, CNT_ADV_EVENTS, n_events);
-  wave_count(S, CNT_SEGMENTS, n_segs);
-  wave_count(S, CNT_LOST, n_lost);
+  EvCount c;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    S.adv_key[i] = advance_one<NL, MULTI>(P, S, q[i], c);
+  wave_count(S, CNT_ADV_EVENTS, c.events);
+  wave_count(S, CNT_SEGMENTS, c.segs);
+  wave_count(S, CNT_LOST, c.lost);
 }
 
 //==============================================================================
@@ -953,134 +961,143 @@ __device__ __forceinline__ double revive_from_bank(const DevProblem& P, const De
   return wgt;
 }
 
+// Particle::event_cross_surface for slot p (src/particle.cpp:288-329):
+// lattice crossing, boundary conditions, cell search, then the revive
+// epilogue.  Leakage goes into `leak` (a single model) or the problem's
+// slots (a problem set).  Returns whether the particle (or the secondary it
+// was revived as) is alive; key is its calculate-xs sort key.
+template<int NL, bool WW>
+__device__ __forceinline__ bool surface_one(const DevProblem& P, const DevParticles& S, int p, EvCount& c,
+  double& leak, int& key)
+{
+  c.events++;
+  const PHot h = S.hot[p];
+  Geo<NL> g;
+  load_geo<NL>(P, S, p, h, g);
+  double wgt = h.wgt;
+  const int cell_last0 = g.cell[0];
+  const int info = h.b_info;
+  int trans[3] = {((info >> 8) & 3) - 1, ((info >> 10) & 3) - 1, ((info >> 12) & 3) - 1};
+  g.surface = h.b_surface;
+  g.n_coord = info & 0xff;
+  if (g.n_coord < 1 || g.n_coord > NL) {
+    g.n_coord = 1;
+    g.lost = 1;
+  }
+  uint64_t hh = 0;
+  const bool trace = S.trace_hash != 0;
+  PartBase base {0, 0, 0};
+  if (trace) {
+    base = part_base(P, h.cell0);
+    const int s_local = g.surface > 0 ? g.surface - base.surf : (g.surface < 0 ? g.surface + base.surf : 0);
+    hh = S.hash[p];
+    hmix(hh, 0x33);
+    hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(s_local)));
+    hmix(hh, static_cast<uint64_t>((trans[0] + 1) | ((trans[1] + 1) << 2) | ((trans[2] + 1) << 4)));
+  }
+  if (g.lost) {
+  } else if (trans[0] != 0 || trans[1] != 0 || trans[2] != 0) {
+    cross_lattice(P, g, trans);
+  } else if (g.surface == SURFACE_NONE || abs(g.surface) > P.n_surfaces) {
+    g.lost = 1;
+  } else {
+    const DevSurface surf = P.surf[abs(g.surface) - 1];
+    if (surf.bc == BC_VACUUM) {
+      // cross_vacuum_bc (src/particle.cpp:631-655)
+      if (P.n_parts > 1)
+        unsafeAtomicAdd(&S.leak[(h.cell0 >= 0 ? P.cell[h.cell0].part : 0) * LEAK_PART_SLOTS +
+                                (wave_slot() & (LEAK_PART_SLOTS - 1))],
+          wgt);
+      else
+        leak += wgt;
+      wgt = 0.0;
+    } else if (surf.bc == BC_REFLECT) {
+      // ReflectiveBC::handle_particle + cross_reflective_bc (src/boundary_condition.cpp:37-46,
+      // src/particle.cpp:657-710)
+      V3 u = reflect(g.u(), surf_normal(surf, g.r(0)));
+      u = v_div(u, v_norm(u));
+      if (g.n_coord != 1) {
+        g.lost = 1;
+      } else {
+        g.set_u(u);
+        g.cell[0] = cell_last0;
+        g.surface = -g.surface;
+        g.n_coord = 1;
+        if (!neighbor_list_find_cell(P, g))
+          g.lost = 1;
+      }
+    } else {
+      // Particle::cross_surface (src/particle.cpp:597-628)
+      if (!neighbor_list_find_cell(P, g)) {
+        g.n_coord = 1;
+        if (!exhaustive_find_cell(P, g)) {
+          g.surface = SURFACE_NONE;
+          g.n_coord = 1;
+          g.set_r(0, v_add(g.r(0), v_scale(g.u(), TINY_BIT)));
+          if (!exhaustive_find_cell(P, g))
+            g.lost = 1;
+        }
+      }
+    }
+  }
+  if (g.lost) {
+    wgt = 0.0;
+    S.lost[p] = 1;
+    c.lost++;
+  }
+  PWW st;
+  if (WW) {
+    st = S.ww[p];
+    // Particle::event_cross_surface: weight windows after a surface
+    // (not lattice) crossing when that checkpoint is on (src/particle.cpp:320-322)
+    const bool lattice = trans[0] != 0 || trans[1] != 0 || trans[2] != 0;
+    if (P.ww.ckpt_surface && !lattice) {
+      uint64_t seed = h.seed;
+      const SplitSink sink {S.bank + static_cast<int64_t>(p) * S.bank_cap, S.bank_cap, S.status};
+      apply_weight_windows(P.ww, st, wgt, g.r(0), g.u(), h.E, h.time, g.surface, &seed, sink);
+      S.hot[p].seed = seed;
+    }
+  }
+  if (trace) {
+    hmix(hh, dbits(wgt));
+    hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? lowest_cell(g) - base.cell : -1)));
+    if (WW)
+      hmix(hh, static_cast<uint64_t>(st.n_bank));
+    S.hash[p] = hh;
+  }
+  // event_revive_from_secondary (src/particle.cpp:417-473)
+  int n_event = h.n_event + 1;
+  if (n_event == P.max_events)
+    wgt = 0.0;
+  store_geo<NL>(S, p, g);
+  S.hot[p].n_event = n_event;
+  S.hot[p].wgt = wgt;
+  int mat = g.material;
+  double E = h.E;
+  if (WW) {
+    if (wgt == 0.0 && st.n_bank > 0)
+      wgt = revive_from_bank<NL>(P, S, p, st, mat, E, c.lost);
+    S.ww[p] = st;
+  }
+  key = sort_key(S, mat, E);
+  return wgt != 0.0;
+}
+
 template<int NL, bool WW>
 __global__ void __launch_bounds__(BLOCK) k_surface(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_SURF];
-  unsigned long long n_events = 0, n_lost = 0;
+  EvCount c;
   double leak = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    bool push = false;
-    int key = 0;
     const int p = S.q_sc[i];
-    {
-      n_events++;
-      const PHot h = S.hot[p];
-      Geo<NL> g;
-      load_geo<NL>(P, S, p, h, g);
-      double wgt = h.wgt;
-      const int cell_last0 = g.cell[0];
-      const int info = h.b_info;
-      int trans[3] = {((info >> 8) & 3) - 1, ((info >> 10) & 3) - 1, ((info >> 12) & 3) - 1};
-      g.surface = h.b_surface;
-      g.n_coord = info & 0xff;
-      if (g.n_coord < 1 || g.n_coord > NL) {
-        g.n_coord = 1;
-        g.lost = 1;
-      }
-      uint64_t hh = 0;
-      const bool trace = S.trace_hash != 0;
-      PartBase base {0, 0, 0};
-      if (trace) {
-        base = part_base(P, h.cell0);
-        const int s_local = g.surface > 0 ? g.surface - base.surf : (g.surface < 0 ? g.surface + base.surf : 0);
-        hh = S.hash[p];
-        hmix(hh, 0x33);
-        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(s_local)));
-        hmix(hh, static_cast<uint64_t>((trans[0] + 1) | ((trans[1] + 1) << 2) | ((trans[2] + 1) << 4)));
-      }
-      if (g.lost) {
-      } else if (trans[0] != 0 || trans[1] != 0 || trans[2] != 0) {
-        cross_lattice(P, g, trans);
-      } else if (g.surface == SURFACE_NONE || abs(g.surface) > P.n_surfaces) {
-        g.lost = 1;
-      } else {
-        const DevSurface surf = P.surf[abs(g.surface) - 1];
-        if (surf.bc == BC_VACUUM) {
-          // cross_vacuum_bc (src/particle.cpp:631-655)
-          if (P.n_parts > 1)
-            unsafeAtomicAdd(&S.leak[(h.cell0 >= 0 ? P.cell[h.cell0].part : 0) * LEAK_PART_SLOTS +
-                                    (wave_slot() & (LEAK_PART_SLOTS - 1))],
-              wgt);
-          else
-            leak += wgt;
-          wgt = 0.0;
-        } else if (surf.bc == BC_REFLECT) {
-          // ReflectiveBC::handle_particle + cross_reflective_bc (src/boundary_condition.cpp:37-46,
-          // src/particle.cpp:657-710)
-          V3 u = reflect(g.u(), surf_normal(surf, g.r(0)));
-          u = v_div(u, v_norm(u));
-          if (g.n_coord != 1) {
-            g.lost = 1;
-          } else {
-            g.set_u(u);
-            g.cell[0] = cell_last0;
-            g.surface = -g.surface;
-            g.n_coord = 1;
-            if (!neighbor_list_find_cell(P, g))
-              g.lost = 1;
-          }
-        } else {
-          // Particle::cross_surface (src/particle.cpp:597-628)
-          if (!neighbor_list_find_cell(P, g)) {
-            g.n_coord = 1;
-            if (!exhaustive_find_cell(P, g)) {
-              g.surface = SURFACE_NONE;
-              g.n_coord = 1;
-              g.set_r(0, v_add(g.r(0), v_scale(g.u(), TINY_BIT)));
-              if (!exhaustive_find_cell(P, g))
-                g.lost = 1;
-            }
-          }
-        }
-      }
-      if (g.lost) {
-        wgt = 0.0;
-        S.lost[p] = 1;
-        n_lost++;
-      }
-      PWW st;
-      if (WW) {
-        st = S.ww[p];
-        // Particle::event_cross_surface: weight windows after a surface
-        // (not lattice) crossing when that checkpoint is on (src/particle.cpp:320-322)
-        const bool lattice = trans[0] != 0 || trans[1] != 0 || trans[2] != 0;
-        if (P.ww.ckpt_surface && !lattice) {
-          uint64_t seed = h.seed;
-          const SplitSink sink {S.bank + static_cast<int64_t>(p) * S.bank_cap, S.bank_cap, S.status};
-          apply_weight_windows(P.ww, st, wgt, g.r(0), g.u(), h.E, h.time, g.surface, &seed, sink);
-          S.hot[p].seed = seed;
-        }
-      }
-      if (trace) {
-        hmix(hh, dbits(wgt));
-        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(wgt != 0.0 ? lowest_cell(g) - base.cell : -1)));
-        if (WW)
-          hmix(hh, static_cast<uint64_t>(st.n_bank));
-        S.hash[p] = hh;
-      }
-      // event_revive_from_secondary (src/particle.cpp:417-473)
-      int n_event = h.n_event + 1;
-      if (n_event == P.max_events)
-        wgt = 0.0;
-      store_geo<NL>(S, p, g);
-      S.hot[p].n_event = n_event;
-      S.hot[p].wgt = wgt;
-      int mat = g.material;
-      double E = h.E;
-      if (WW) {
-        if (wgt == 0.0 && st.n_bank > 0)
-          wgt = revive_from_bank<NL>(P, S, p, st, mat, E, n_lost);
-        S.ww[p] = st;
-      }
-      push = wgt != 0.0;
-      key = sort_key(S, mat, E);
-    }
+    int key = 0;
+    const bool push = surface_one<NL, WW>(P, S, p, c, leak, key);
     S.stage_p[i] = p;
     S.stage_key[i] = push ? key : -1;
   }
-  wave_count(S, CNT_SURF_EVENTS, n_events);
-  wave_count(S, CNT_LOST, n_lost);
+  wave_count(S, CNT_SURF_EVENTS, c.events);
+  wave_count(S, CNT_LOST, c.lost);
   for (int o = 32; o > 0; o >>= 1)
     leak += __shfl_xor(leak, o);
   if (lane_id() == 0 && leak != 0.0)
@@ -1120,159 +1137,284 @@ __device__ __forceinline__ V3 sample_cxs_target_velocity(double awr, double E, V
   return v_scale(rotate_angle(u, mu, seed), vt);
 }
 
+// Particle::event_collide for slot p (src/particle.cpp:331-415): sample the
+// nuclide from the particle's micro_t row, analog absorption, elastic
+// scattering with free-gas target motion, weight windows, then the revive
+// epilogue.  Returns whether the particle (or its revived secondary) is
+// alive; key is its calculate-xs sort key.
+template<int NL, bool WW>
+__device__ __forceinline__ bool collision_one(const DevProblem& P, const DevParticles& S, int p, EvCount& c,
+  int& key)
+{
+  c.events++;
+  const PHot h = S.hot[p];
+  double E = h.E;
+  double wgt = h.wgt;
+  uint64_t seed = h.seed;
+  const int material = h.material;
+  V3 u = {h.ux, h.uy, h.uz};
+  // sample_nuclide (src/physics.cpp:494-518)
+  double cutoff = prn(&seed) * h.xs_t;
+  const int j0 = P.mat_off[material];
+  const int j1 = P.mat_off[material + 1];
+  // the row is read two values per 16-byte load (rows are padded to a
+  // multiple of four values)
+  const double2* micro2 = reinterpret_cast<const double2*>(S.micro_t + static_cast<int64_t>(p) * S.max_nn);
+  double prob = 0.0;
+  int i_nuclide = -1;
+  int j_nuclide = -1; // its entry in the material's list
+  for (int j = j0; j < j1; j += 2) {
+    const double2 mm = micro2[(j - j0) >> 1];
+    const double dens = P.mat_dens[j] * 1.0;
+    prob += dens * mm.x;
+    if (prob >= cutoff) {
+      i_nuclide = P.mat_nuc[j];
+      j_nuclide = j;
+      break;
+    }
+    if (j + 1 < j1) {
+      const double dens1 = P.mat_dens[j + 1] * 1.0;
+      prob += dens1 * mm.y;
+      if (prob >= cutoff) {
+        i_nuclide = P.mat_nuc[j + 1];
+        j_nuclide = j + 1;
+        break;
+      }
+    }
+  }
+  bool absorbed = false;
+  if (i_nuclide < 0) {
+    wgt = 0.0;
+    S.lost[p] = 1;
+    c.lost++;
+  } else {
+    // micro cross sections of the sampled nuclide at this energy
+    const double lx = detm::log(E / P.energy_min);
+    int i_log = static_cast<int>(lx / P.log_spacing);
+    i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+    int probes = 0;
+    Micro m = nuclide_xs(P, i_nuclide, E, i_log, lx, S.use_learned != 0, probes);
+    c.lookups++;
+    c.probes += probes;
+    // absorption (src/physics.cpp:638-672), analog
+    if (m.absorption > 0.0) {
+      if (m.absorption > prn(&seed) * m.total) {
+        wgt = 0.0;
+        absorbed = true;
+      }
+    }
+    if (!absorbed) {
+      // scatter (src/physics.cpp:674-752): the data hold elastic scattering only
+      double cut = prn(&seed) * (m.total - m.absorption);
+      (void)cut;
+      const DevNuclide nd = P.nuc[i_nuclide];
+      const double awr = nd.awr;
+      const double kT = nd.kT;
+      // elastic_scatter (src/physics.cpp:754-818)
+      double vel = sqrt(E);
+      V3 v_n = v_scale(u, vel);
+      V3 v_t = {0.0, 0.0, 0.0};
+      // sample_target_velocity (src/physics.cpp:836-988), non-resonant nuclide
+      if (!(E >= P.free_gas_threshold * kT && awr > 1.0))
+        v_t = sample_cxs_target_velocity(awr, E, u, kT, &seed);
+      V3 v_cm = v_div(v_add(v_n, v_scale(v_t, awr)), awr + 1.0);
+      v_n = v_sub(v_n, v_cm);
+      vel = v_norm(v_n);
+      double mu_cm = uniform_distribution(-1., 1., &seed);
+      V3 u_cm = v_div(v_n, vel);
+      v_n = v_scale(rotate_angle(u_cm, mu_cm, &seed), vel);
+      v_n = v_add(v_n, v_cm);
+      E = v_dot(v_n, v_n);
+      vel = sqrt(E);
+      u = v_div(v_n, vel);
+    }
+  }
+  PWW st;
+  if (WW) {
+    st = S.ww[p];
+    // collision(): weight windows after the reaction (src/physics.cpp:64-65)
+    if (P.ww.ckpt_collision) {
+      const SplitSink sink {S.bank + static_cast<int64_t>(p) * S.bank_cap, S.bank_cap, S.status};
+      apply_weight_windows(P.ww, st, wgt, V3 {h.rx, h.ry, h.rz}, u, E, h.time, SURFACE_NONE, &seed, sink);
+    }
+  }
+  if (S.trace_hash) {
+    uint64_t hh = S.hash[p];
+    hmix(hh, 0x44);
+    hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(P.n_parts > 1 && j_nuclide >= 0 ?
+                                                            P.mat_nuc_local[j_nuclide] : i_nuclide)));
+    hmix(hh, absorbed ? 101 : 2);
+    hmix(hh, dbits(E));
+    hmix(hh, dbits(u.x));
+    hmix(hh, dbits(u.y));
+    hmix(hh, dbits(u.z));
+    if (WW) {
+      hmix(hh, dbits(wgt));
+      hmix(hh, static_cast<uint64_t>(st.n_bank));
+    }
+    S.hash[p] = hh;
+  }
+  if (E < P.energy_cutoff)
+    wgt = 0.0;
+  // Particle::event_collide epilogue (src/particle.cpp:388-406); the
+  // direction of the lower levels is the shared one
+  int n_event = h.n_event + 1;
+  if (n_event == P.max_events)
+    wgt = 0.0;
+  PHot* hp = S.hot + p;
+  hp->surface = SURFACE_NONE;
+  hp->material_last = C_NONE;
+  hp->ux = u.x;
+  hp->uy = u.y;
+  hp->uz = u.z;
+  hp->n_event = n_event;
+  hp->E = E;
+  hp->wgt = wgt;
+  hp->seed = seed;
+  int mat = material;
+  if (WW) {
+    if (wgt == 0.0 && st.n_bank > 0)
+      wgt = revive_from_bank<NL>(P, S, p, st, mat, E, c.lost);
+    S.ww[p] = st;
+  }
+  key = sort_key(S, mat, E);
+  return wgt != 0.0;
+}
+
 template<int NL, bool WW>
 __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_COLL];
   const int n_surf = S.qcount[QC_SURF]; // collisions follow the crossings in q_sc
-  unsigned long long n_events = 0, n_lost = 0, n_lookups = 0, n_probes = 0;
+  EvCount c;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    bool push = false;
-    int key = 0;
     const int p = S.q_sc[n_surf + i];
-    {
-      n_events++;
-      const PHot h = S.hot[p];
-      double E = h.E;
-      double wgt = h.wgt;
-      uint64_t seed = h.seed;
-      const int material = h.material;
-      V3 u = {h.ux, h.uy, h.uz};
-      // sample_nuclide (src/physics.cpp:494-518)
-      double cutoff = prn(&seed) * h.xs_t;
-      const int j0 = P.mat_off[material];
-      const int j1 = P.mat_off[material + 1];
-      // the row is read two values per 16-byte load (rows are padded to a
-      // multiple of four values)
-      const double2* micro2 = reinterpret_cast<const double2*>(S.micro_t + static_cast<int64_t>(p) * S.max_nn);
-      double prob = 0.0;
-      int i_nuclide = -1;
-      int j_nuclide = -1; // its entry in the material's list
-      for (int j = j0; j < j1; j += 2) {
-        const double2 mm = micro2[(j - j0) >> 1];
-        const double dens = P.mat_dens[j] * 1.0;
-        prob += dens * mm.x;
-        if (prob >= cutoff) {
-          i_nuclide = P.mat_nuc[j];
-          j_nuclide = j;
-          break;
-        }
-        if (j + 1 < j1) {
-          const double dens1 = P.mat_dens[j + 1] * 1.0;
-          prob += dens1 * mm.y;
-          if (prob >= cutoff) {
-            i_nuclide = P.mat_nuc[j + 1];
-            j_nuclide = j + 1;
-            break;
-          }
-        }
-      }
-      bool absorbed = false;
-      if (i_nuclide < 0) {
-        wgt = 0.0;
-        S.lost[p] = 1;
-        n_lost++;
-      } else {
-        // micro cross sections of the sampled nuclide at this energy
-        const double lx = detm::log(E / P.energy_min);
-        int i_log = static_cast<int>(lx / P.log_spacing);
-        i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
-        int probes = 0;
-        Micro m = nuclide_xs(P, i_nuclide, E, i_log, lx, S.use_learned != 0, probes);
-        n_lookups++;
-        n_probes += probes;
-        // absorption (src/physics.cpp:638-672), analog
-        if (m.absorption > 0.0) {
-          if (m.absorption > prn(&seed) * m.total) {
-            wgt = 0.0;
-            absorbed = true;
-          }
-        }
-        if (!absorbed) {
-          // scatter (src/physics.cpp:674-752): the data hold elastic scattering only
-          double cut = prn(&seed) * (m.total - m.absorption);
-          (void)cut;
-          const DevNuclide nd = P.nuc[i_nuclide];
-          const double awr = nd.awr;
-          const double kT = nd.kT;
-          // elastic_scatter (src/physics.cpp:754-818)
-          double vel = sqrt(E);
-          V3 v_n = v_scale(u, vel);
-          V3 v_t = {0.0, 0.0, 0.0};
-          // sample_target_velocity (src/physics.cpp:836-988), non-resonant nuclide
-          if (!(E >= P.free_gas_threshold * kT && awr > 1.0))
-            v_t = sample_cxs_target_velocity(awr, E, u, kT, &seed);
-          V3 v_cm = v_div(v_add(v_n, v_scale(v_t, awr)), awr + 1.0);
-          v_n = v_sub(v_n, v_cm);
-          vel = v_norm(v_n);
-          double mu_cm = uniform_distribution(-1., 1., &seed);
-          V3 u_cm = v_div(v_n, vel);
-          v_n = v_scale(rotate_angle(u_cm, mu_cm, &seed), vel);
-          v_n = v_add(v_n, v_cm);
-          E = v_dot(v_n, v_n);
-          vel = sqrt(E);
-          u = v_div(v_n, vel);
-        }
-      }
-      PWW st;
-      if (WW) {
-        st = S.ww[p];
-        // collision(): weight windows after the reaction (src/physics.cpp:64-65)
-        if (P.ww.ckpt_collision) {
-          const SplitSink sink {S.bank + static_cast<int64_t>(p) * S.bank_cap, S.bank_cap, S.status};
-          apply_weight_windows(P.ww, st, wgt, V3 {h.rx, h.ry, h.rz}, u, E, h.time, SURFACE_NONE, &seed, sink);
-        }
-      }
-      if (S.trace_hash) {
-        uint64_t hh = S.hash[p];
-        hmix(hh, 0x44);
-        hmix(hh, static_cast<uint64_t>(static_cast<int64_t>(P.n_parts > 1 && j_nuclide >= 0 ?
-                                                                P.mat_nuc_local[j_nuclide] : i_nuclide)));
-        hmix(hh, absorbed ? 101 : 2);
-        hmix(hh, dbits(E));
-        hmix(hh, dbits(u.x));
-        hmix(hh, dbits(u.y));
-        hmix(hh, dbits(u.z));
-        if (WW) {
-          hmix(hh, dbits(wgt));
-          hmix(hh, static_cast<uint64_t>(st.n_bank));
-        }
-        S.hash[p] = hh;
-      }
-      if (E < P.energy_cutoff)
-        wgt = 0.0;
-      // Particle::event_collide epilogue (src/particle.cpp:388-406); the
-      // direction of the lower levels is the shared one
-      int n_event = h.n_event + 1;
-      if (n_event == P.max_events)
-        wgt = 0.0;
-      PHot* hp = S.hot + p;
-      hp->surface = SURFACE_NONE;
-      hp->material_last = C_NONE;
-      hp->ux = u.x;
-      hp->uy = u.y;
-      hp->uz = u.z;
-      hp->n_event = n_event;
-      hp->E = E;
-      hp->wgt = wgt;
-      hp->seed = seed;
-      int mat = material;
-      if (WW) {
-        if (wgt == 0.0 && st.n_bank > 0)
-          wgt = revive_from_bank<NL>(P, S, p, st, mat, E, n_lost);
-        S.ww[p] = st;
-      }
-      push = wgt != 0.0;
-      key = sort_key(S, mat, E);
-    }
+    int key = 0;
+    const bool push = collision_one<NL, WW>(P, S, p, c, key);
     S.stage_p[n_surf + i] = p;
     S.stage_key[n_surf + i] = push ? key : -1;
   }
-  wave_count(S, CNT_COLL_EVENTS, n_events);
-  wave_count(S, CNT_COLLISIONS, n_events);
-  wave_count(S, CNT_LOST, n_lost);
-  wave_count(S, CNT_LOOKUPS, n_lookups);
-  wave_count(S, CNT_SEARCH_PROBES, n_probes);
+  wave_count(S, CNT_COLL_EVENTS, c.events);
+  wave_count(S, CNT_COLLISIONS, c.events);
+  wave_count(S, CNT_LOST, c.lost);
+  wave_count(S, CNT_LOOKUPS, c.lookups);
+  wave_count(S, CNT_SEARCH_PROBES, c.probes);
+}
+
+//==============================================================================
+// History-based tail: one lane per remaining particle
+//==============================================================================
+
+// Particle::event_calculate_xs for slot p by one lane on its own (the
+// material need not be the same across the wavefront): the material's
+// nuclides one at a time, sums in material order, as k_xs_lane computes them.
+template<bool LEARNED>
+__device__ __forceinline__ void xs_one(const DevProblem& P, const DevParticles& S, int p, EvCount& c)
+{
+  PHot* h = S.hot + p;
+  c.events++;
+  const int material = h->material;
+  if (h->wgt == 0.0)
+    return;
+  if (material == MATERIAL_VOID) {
+    *reinterpret_cast<double2*>(&h->xs_t) = double2 {0.0, 0.0};
+    return;
+  }
+  if (material == h->material_last)
+    return;
+  const double E = h->E;
+  const double lx = detm::log(E / P.energy_min);
+  int i_log = static_cast<int>(lx / P.log_spacing);
+  i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
+  const int j0 = P.mat_off[material];
+  const int nn = P.mat_off[material + 1] - j0;
+  double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+  const bool trace = S.trace_hash != 0;
+  uint64_t hh = 0;
+  if (trace) {
+    hh = S.hash[p];
+    hmix(hh, 0x11);
+    hmix(hh, static_cast<uint64_t>(material - part_base(P, h->cell0).mat));
+  }
+  double xs_t = 0.0;
+  double xs_a = 0.0;
+  int probes = 0;
+  for (int j = 0; j < nn; ++j) {
+    const Micro m = nuclide_xs(P, P.mat_nuc[j0 + j], E, i_log, lx, LEARNED, probes);
+    micro[j] = m.total;
+    const double dens = P.mat_dens[j0 + j] * 1.0;
+    xs_t += dens * m.total;
+    xs_a += dens * m.absorption;
+    if (trace)
+      hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m.i_grid)));
+  }
+  *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
+  if (trace)
+    S.hash[p] = hh;
+  c.lookups += nn;
+  c.probes += probes;
+}
+
+// The particles staged at the end of an event cycle (stage_p / stage_key,
+// QC_STAGE entries, key -1 for done), each run by one lane through whole
+// cycles -- calculate-xs, advance, then crossing or collision, and its
+// revived secondaries -- for at most max_steps cycles.  The event functions
+// are the event kernels' own, so the histories are the same.  A particle
+// still alive keeps its entry (with its new key) for the next launch; the
+// count of those is added to QC_LIVE.  Used for the tail of a batch, where
+// few particles are left and a cycle of event kernels costs its launches.
+template<int NL, bool WW, bool MULTI, bool LEARNED>
+__global__ void __launch_bounds__(BLOCK) k_history(DevProblem P, DevParticles S, int max_steps)
+{
+  const int n = S.qcount[QC_STAGE];
+  EvCount cx, ca, cs, cc;
+  double leak = 0.0;
+  unsigned long long live = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int key = S.stage_key[i];
+    if (key < 0)
+      continue;
+    const int p = S.stage_p[i];
+    bool alive = true;
+    for (int step = 0; step < max_steps; ++step) {
+      xs_one<LEARNED>(P, S, p, cx);
+      const int outcome = advance_one<NL, MULTI>(P, S, p, ca);
+      if (outcome < 0) {
+        alive = false;
+        break;
+      }
+      alive = outcome == 0 ? surface_one<NL, WW>(P, S, p, cs, leak, key) : collision_one<NL, WW>(P, S, p, cc, key);
+      if (!alive)
+        break;
+    }
+    S.stage_key[i] = alive ? key : -1;
+    live += alive ? 1 : 0;
+  }
+  wave_count(S, CNT_XS_EVENTS, cx.events);
+  wave_count(S, CNT_ADV_EVENTS, ca.events);
+  wave_count(S, CNT_SURF_EVENTS, cs.events);
+  wave_count(S, CNT_COLL_EVENTS, cc.events);
+  wave_count(S, CNT_COLLISIONS, cc.events);
+  wave_count(S, CNT_SEGMENTS, ca.segs);
+  wave_count(S, CNT_LOST, ca.lost + cs.lost + cc.lost);
+  wave_count(S, CNT_LOOKUPS, cx.lookups + cc.lookups);
+  wave_count(S, CNT_SEARCH_PROBES, cx.probes + cc.probes);
+  wave_count(S, CNT_HIST_XS_EVENTS, cx.events);
+  wave_count(S, CNT_HIST_LOOKUPS, cx.lookups);
+  wave_count(S, CNT_HIST_ADV_EVENTS, ca.events);
+  wave_count(S, CNT_HIST_SEGMENTS, ca.segs);
+  wave_count(S, CNT_HIST_SURF_EVENTS, cs.events);
+  wave_count(S, CNT_HIST_COLL_EVENTS, cc.events);
+  for (int o = 32; o > 0; o >>= 1) {
+    leak += __shfl_xor(leak, o);
+    live += __shfl_xor(live, o);
+  }
+  if (lane_id() == 0) {
+    if (leak != 0.0)
+      unsafeAtomicAdd(&S.leak[wave_slot()], leak);
+    if (live != 0)
+      atomicAdd(&S.qcount[QC_LIVE], static_cast<int>(live));
+  }
 }
 
 //==============================================================================
@@ -1449,6 +1591,41 @@ void launch_cycle(const DevProblem& P, const DevParticles& S, const LaunchConfig
 {
   for (int k = KERNEL_SORT; k <= KERNEL_COLLISION; ++k)
     launch_kernel(k, P, S, lc, st, cycle);
+}
+
+namespace {
+template<int NL, bool WW, bool MULTI>
+void history_m(const DevProblem& P, const DevParticles& S, int n, int max_steps, hipStream_t st)
+{
+  const int blocks = (n + BLOCK - 1) / BLOCK;
+  if (S.use_learned)
+    hipLaunchKernelGGL((k_history<NL, WW, MULTI, true>), dim3(blocks < 1 ? 1 : blocks), dim3(BLOCK), 0, st, P, S,
+      max_steps);
+  else
+    hipLaunchKernelGGL((k_history<NL, WW, MULTI, false>), dim3(blocks < 1 ? 1 : blocks), dim3(BLOCK), 0, st, P, S,
+      max_steps);
+}
+
+template<int NL>
+void history_nl(const DevProblem& P, const DevParticles& S, int n, int max_steps, hipStream_t st)
+{
+  if (P.ww.on)
+    history_m<NL, true, false>(P, S, n, max_steps, st);
+  else if (P.n_parts > 1)
+    history_m<NL, false, true>(P, S, n, max_steps, st);
+  else
+    history_m<NL, false, false>(P, S, n, max_steps, st);
+}
+} // namespace
+
+void launch_history(const DevProblem& P, const DevParticles& S, int n_staged, int max_steps, hipStream_t st)
+{
+  switch (P.n_levels) {
+  case 1: history_nl<1>(P, S, n_staged, max_steps, st); break;
+  case 2: history_nl<2>(P, S, n_staged, max_steps, st); break;
+  case 3: history_nl<3>(P, S, n_staged, max_steps, st); break;
+  default: history_nl<4>(P, S, n_staged, max_steps, st); break;
+  }
 }
 
 void launch_accumulate(double* value, double* results, int64_t n_bins, int n_copies, double norm, hipStream_t st)

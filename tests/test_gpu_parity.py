@@ -141,7 +141,9 @@ def test_transport_graph_and_direct_launch_identical():
                                   {"block": 64, "blocks": 7}, {"block": 128},
                                   {"sort_every": 4, "cycles_per_check": 8}, {"sort_every": 3, "sort_bits": 0},
                                   {"wave_search": 1}, {"wave_search": 1, "xs_group": 2}, {"tally_copies": 8},
-                                  {"sort": 0, "xs_group": 8}])
+                                  {"sort": 0, "xs_group": 8}, {"history_below": 0},
+                                  {"history_below": 1 << 30, "history_steps": 3, "cycles_per_check": 1},
+                                  {"history_below": 200, "history_steps": 1}])
 def test_transport_launch_shapes_identical(opts):
     """Queue sorting and lanes-per-particle change the order of work, never a history."""
     d, model, nucs = prepare(1, n_particles=1500, n_batches=1, grid_lo=2000, grid_hi=6000, n_nuclides=12,
@@ -231,13 +233,14 @@ def test_weight_windows_bank_overflow_is_an_error():
         gpu_run(d, options={"bank_cap": 2})
 
 
-@pytest.mark.parametrize("bank_cap", [64])
-def test_transport_weight_windows_histories(bank_cap):
+@pytest.mark.parametrize("bank_cap,tail", [(64, 65536), (64, 0)], ids=["history-tail", "event-only"])
+def test_transport_weight_windows_histories(bank_cap, tail):
     """Splitting, roulette and LIFO revival from the per-particle bank give the
-    oracle's histories bit for bit (3-layer slab, 12-bin windows)."""
+    oracle's histories bit for bit (3-layer slab, 12-bin windows), with the
+    history-based tail and with event kernels only."""
     d, model, nucs = prepare(4, **WW_SLAB)
     o = oracle_run(d, event_based=True)
-    g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap})
+    g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap, "history_below": tail})
     h_o = o.hashes(600)
     assert np.array_equal(g["hashes"], h_o), "%d of 600 histories differ" % np.sum(g["hashes"] != h_o)
     assert_tallies_close(g["results"], o.results())
