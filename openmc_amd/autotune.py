@@ -9,6 +9,7 @@ history (every one of them is covered by the launch-shape parity tests):
   sort_every        full (material, energy) sort every n cycles, material-only between
   cycles_per_check  event cycles per host check of the queue counts (and per hipGraph)
   xs_group          nuclide lookups interleaved per lane in calculate-xs
+  history_below     the queue threshold below which a batch finishes history-based
 
 ``tune`` times whole batches of the problem (or problem set) that the
 library holds and picks, option by option, the fastest value (coordinate
@@ -34,8 +35,10 @@ DEFAULT_SPACE = [
     ("sort_every", [1, 2, 4]),
     ("cycles_per_check", [16, 8, 32, 64]),
     ("xs_group", [4, 2, 8]),
+    ("history_below", [65536, 0, 16384, 262144, 1048576]),
 ]
-DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": 5, "sort_every": 1, "cycles_per_check": 16, "xs_group": 4}
+DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": 5, "sort_every": 1, "cycles_per_check": 16, "xs_group": 4,
+            "history_below": 65536}
 
 
 def _time_batch():

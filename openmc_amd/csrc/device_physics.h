@@ -769,7 +769,20 @@ __device__ __forceinline__ double wave_uniform(double x)
   return __longlong_as_double(wave_uniform(static_cast<int64_t>(__double_as_longlong(x))));
 }
 
-template<int K, bool LEARNED>
+// The value itself where the lanes need not agree (UNI false: one particle
+// per lane, any material), the wavefront's copy where they do.
+template<bool UNI, class T>
+__device__ __forceinline__ T uni(T x)
+{
+  if constexpr (UNI)
+    return wave_uniform(x);
+  else
+    return x;
+}
+
+// UNI (default) requires the wavefront-uniform material above; with UNI
+// false each lane reads its own material's records, for one particle per lane.
+template<int K, bool LEARNED, bool UNI = true>
 __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevMatNuc* mn, int kk, double E,
   int i_log, double logE, int& probes, Micro* out)
 {
@@ -781,11 +794,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const DevMatNuc* r = mn + (k < kk ? k : 0);
-    nuc[k] = wave_uniform(r->nuc);
-    off[k] = wave_uniform(r->off);
-    n[k] = wave_uniform(r->n);
-    e_first[k] = wave_uniform(r->e_first);
-    e_last[k] = wave_uniform(r->e_last);
+    nuc[k] = uni<UNI>(r->nuc);
+    off[k] = uni<UNI>(r->off);
+    n[k] = uni<UNI>(r->n);
+    e_first[k] = uni<UNI>(r->e_first);
+    e_last[k] = uni<UNI>(r->e_last);
     Eg[k] = P.E + off[k];
   }
   // final bracket values; with the learned index they are first read
@@ -812,7 +825,7 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const DevMatNuc* r = mn + (k < kk ? k : 0);
-      uu[k] = (logE - wave_uniform(r->u_lo)) * wave_uniform(r->inv_w);
+      uu[k] = (logE - uni<UNI>(r->u_lo)) * uni<UNI>(r->inv_w);
       int sgi = uu[k] < 0.0 ? 0 : static_cast<int>(uu[k]);
       seg[k] = sgi >= Lm.n_seg ? Lm.n_seg - 1 : sgi;
       sg[k] = Lm.rec[static_cast<int64_t>(nuc[k]) * Lm.n_seg + seg[k]];

@@ -1339,14 +1339,24 @@ __device__ __forceinline__ void xs_one(const DevProblem& P, const DevParticles& 
   double xs_t = 0.0;
   double xs_a = 0.0;
   int probes = 0;
-  for (int j = 0; j < nn; ++j) {
-    const Micro m = nuclide_xs(P, P.mat_nuc[j0 + j], E, i_log, lx, LEARNED, probes);
-    micro[j] = m.total;
-    const double dens = P.mat_dens[j0 + j] * 1.0;
-    xs_t += dens * m.total;
-    xs_a += dens * m.absorption;
-    if (trace)
-      hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m.i_grid)));
+  // HG nuclides at a time, their searches interleaved: the lane is alone on
+  // its particle, so its own memory-level parallelism is what hides latency
+  constexpr int HG = 4;
+  for (int jj = 0; jj < nn; jj += HG) {
+    const int kk = nn - jj < HG ? nn - jj : HG;
+    Micro m[HG];
+    nuclide_xs_group<HG, LEARNED, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
+    store_micro<HG>(micro + jj, m, kk);
+#pragma unroll
+    for (int k = 0; k < HG; ++k) {
+      if (k < kk) {
+        const double dens = P.matnuc[j0 + jj + k].dens * 1.0;
+        xs_t += dens * m[k].total;
+        xs_a += dens * m[k].absorption;
+        if (trace)
+          hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
+      }
+    }
   }
   *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
   if (trace)
