@@ -64,6 +64,11 @@ def kernel_bytes(st):
     }
 
 
+# statistics that are settings, not counts (reported as they stand, not differenced)
+CONFIG_STATS = ("blocks", "xs_lanes", "sort", "xs_group", "block", "sort_every", "cycles_per_check", "sort_bits",
+                "sort_blocks", "history_below", "history_steps", "fused_advance")
+
+
 def diff(a, b):
     return {k: b[k] - a[k] for k in b}
 
@@ -104,7 +109,8 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
     finally:
         os.chdir(old)
     st = diff(s0, s1)
-    info.update({"elapsed_s": t1 - t0, "stats": st, "n_particles": n})
+    info.update({"elapsed_s": t1 - t0, "stats": st, "n_particles": n,
+                 "config_stats": {k: s1.get(k, 0) for k in CONFIG_STATS}})
     return info
 
 
@@ -143,18 +149,27 @@ def roofline(info):
     by["partition_xs"] = 0
     by["partition_split"] = 0
     by["k_init"] = 0
+    fused = info.get("config_stats", {}).get("fused_advance", 0) > 0
+    if fused:
+        # one launch does calculate-xs and advance: its bytes are both
+        by["k_xs"] += by["k_advance"]
+        by["k_advance"] = 0
     dom = max(times, key=times.get)
     # average algorithmic bytes per launch over the average launch duration
     achieved = by[dom] / (times[dom] * 1e-3) / 1e9
     traffic = traffic_per_launch(dom, st, launches[dom])
+    if fused and dom == "k_xs" and traffic is not None:
+        adv = traffic_per_launch("k_advance", st, launches[dom])
+        traffic = None if adv is None else round(traffic + adv, 1)
     per_kernel = {k: {"ms": times[k], "launches": launches[k], "avg_us": 1e3 * times[k] / max(launches[k], 1),
                       "GB_per_s": by[k] / max(times[k], 1e-9) * 1e-6} for k in times}
     total_bytes = sum(by.values())
     whole = total_bytes / (st["transport_ms"] * 1e-3) / 1e9
+    name = "k_xs_lane (calculate-xs + advance)" if fused and dom == "k_xs" else dom
     return dom, {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                  "algorithmic_bytes_per_launch": round(by[dom] / max(launches[dom], 1), 1),
-                 "kernel": dom, "whole_loop_GB_per_s": round(whole, 2)}, per_kernel
+                 "kernel": name, "whole_loop_GB_per_s": round(whole, 2)}, per_kernel
 
 
 def cpu_baseline(d, target_s=10.0):
@@ -225,7 +240,8 @@ def run_problem_set(dirs, steps, warmup, options=None, tune_path=None):
         t1 = time.perf_counter()
         s1 = lib.stats()
         info["options"] = {k: s1[k] for k in ("blocks", "block", "sort_every", "cycles_per_check", "sort_bits",
-                                              "xs_group")}
+                                              "xs_group", "history_below", "fused_advance")}
+        info["config_stats"] = {k: s1.get(k, 0) for k in CONFIG_STATS}
         lib.simulation_finalize()
         info["n_particles"] = [lib.problem_info(k)["n_particles"] for k in range(lib.n_problems())]
     finally:

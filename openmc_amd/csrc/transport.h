@@ -29,6 +29,7 @@ struct LaunchConfig {
   int xs_group;    // nuclide lookups interleaved per lane (1, 2, 4 or 8) when xs_lanes == 1
   int block;       // threads per workgroup of the event kernels (64, 128 or 256)
   int sort_every;  // order by (material, energy) every this many cycles, by material in between
+  int fuse_advance; // advance in the calculate-xs launch (xs_lanes 1, xs_group 4)
 };
 
 // Source particles for population indices g_base + [0, n) of batch `batch`

@@ -754,70 +754,6 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
   return act ? nn : 0;
 }
 
-// One lane per particle: the lane looks up the material's nuclides G at a
-// time (their searches interleaved) and forms the sums in material order as
-// it goes, so no LDS is needed and occupancy is set by registers alone.
-// A wavefront handles one material at a time (the queue is sorted by
-// material, so almost always in one pass): the material is then
-// wavefront-uniform and its nuclide list, grid offsets and sizes are read
-// with scalar loads into scalar registers.
-template<int NL, int G, bool LEARNED>
-__global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S, const int* q)
-{
-  const int n = S.qcount[QC_XS];
-  const bool trace = S.trace_hash != 0;
-  unsigned long long n_lookups = 0, n_events = 0, n_probes = 0;
-  // every lane of a wavefront runs the same number of iterations, so the
-  // wavefront votes below see all of its lanes that hold a particle
-  const int stride = gridDim.x * blockDim.x;
-  const int first = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
-  for (int base = first; base < n; base += stride) {
-    const int i = base + lane_id();
-    bool need = false;
-    int material = MATERIAL_VOID;
-    int p = 0;
-    PHot* h = nullptr;
-    double E = 0.0;
-    if (i < n) {
-      p = q[i];
-      n_events++;
-      h = S.hot + p;
-      material = h->material;
-      if (h->wgt != 0.0) {
-        if (material == MATERIAL_VOID) {
-          h->xs_t = 0.0;
-          h->xs_a = 0.0;
-        } else if (material != h->material_last) {
-          need = true;
-          E = h->E;
-        }
-      }
-    }
-    // one pass per distinct material among the lanes that need cross
-    // sections (one pass for almost every wavefront of a sorted queue)
-    unsigned long long todo = __ballot(need);
-    while (todo != 0ull) {
-      const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-      const int mat = __builtin_amdgcn_readfirstlane(__shfl(material, leader));
-      const bool mine = need && material == mat;
-      if (LEARNED || !S.wave_search) {
-        if (mine) {
-          int probes = 0;
-          n_lookups += particle_xs<G, LEARNED>(P, S, p, h, mat, E, trace, probes);
-          n_probes += probes;
-        }
-      } else {
-        int probes = 0;
-        n_lookups += particle_xs_wave<G>(P, S, p, h, mat, E, mine, trace, probes);
-        n_probes += probes;
-      }
-      todo &= ~__ballot(mine);
-    }
-  }
-  wave_count(S, CNT_LOOKUPS, n_lookups);
-  wave_count(S, CNT_XS_EVENTS, n_events);
-  wave_count(S, CNT_SEARCH_PROBES, n_probes);
-}
 
 //==============================================================================
 // advance event (src/particle.cpp:232-286, src/event.cpp:107-128)
@@ -917,6 +853,97 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevProblem P, DevParticles S,
   wave_count(S, CNT_ADV_EVENTS, c.events);
   wave_count(S, CNT_SEGMENTS, c.segs);
   wave_count(S, CNT_LOST, c.lost);
+}
+
+// One lane per particle: the lane looks up the material's nuclides G at a
+// time (their searches interleaved) and forms the sums in material order as
+// it goes, so no LDS is needed and occupancy is set by registers alone.
+// A wavefront handles one material at a time (the queue is sorted by
+// material, so almost always in one pass): the material is then
+// wavefront-uniform and its nuclide list, grid offsets and sizes are read
+// with scalar loads into scalar registers.
+//
+// With ADV != 0 the same kernel then advances each lane's particle
+// (advance_one; ADV 2 for a problem set): the particle's records are still in
+// the caches, and the cycle has one launch less.
+template<int NL, int G, bool LEARNED, int ADV, bool WAVE>
+__device__ __forceinline__ void xs_lane_body(const DevProblem& P, const DevParticles& S, const int* q)
+{
+  const int n = S.qcount[QC_XS];
+  const bool trace = S.trace_hash != 0;
+  unsigned long long n_lookups = 0, n_events = 0, n_probes = 0;
+  EvCount ca;
+  // every lane of a wavefront runs the same number of iterations, so the
+  // wavefront votes below see all of its lanes that hold a particle
+  const int stride = gridDim.x * blockDim.x;
+  const int first = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  for (int base = first; base < n; base += stride) {
+    const int i = base + lane_id();
+    bool need = false;
+    int material = MATERIAL_VOID;
+    int p = 0;
+    PHot* h = nullptr;
+    double E = 0.0;
+    if (i < n) {
+      p = q[i];
+      n_events++;
+      h = S.hot + p;
+      material = h->material;
+      if (h->wgt != 0.0) {
+        if (material == MATERIAL_VOID) {
+          h->xs_t = 0.0;
+          h->xs_a = 0.0;
+        } else if (material != h->material_last) {
+          need = true;
+          E = h->E;
+        }
+      }
+    }
+    // one pass per distinct material among the lanes that need cross
+    // sections (one pass for almost every wavefront of a sorted queue)
+    unsigned long long todo = __ballot(need);
+    while (todo != 0ull) {
+      const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      const int mat = __builtin_amdgcn_readfirstlane(__shfl(material, leader));
+      const bool mine = need && material == mat;
+      if (!WAVE) {
+        if (mine) {
+          int probes = 0;
+          n_lookups += particle_xs<G, LEARNED>(P, S, p, h, mat, E, trace, probes);
+          n_probes += probes;
+        }
+      } else {
+        int probes = 0;
+        n_lookups += particle_xs_wave<G>(P, S, p, h, mat, E, mine, trace, probes);
+        n_probes += probes;
+      }
+      todo &= ~__ballot(mine);
+    }
+    if (ADV != 0 && i < n)
+      S.adv_key[i] = advance_one<NL, ADV == 2>(P, S, p, ca);
+  }
+  wave_count(S, CNT_LOOKUPS, n_lookups);
+  wave_count(S, CNT_XS_EVENTS, n_events);
+  wave_count(S, CNT_SEARCH_PROBES, n_probes);
+  if (ADV != 0) {
+    wave_count(S, CNT_ADV_EVENTS, ca.events);
+    wave_count(S, CNT_SEGMENTS, ca.segs);
+    wave_count(S, CNT_LOST, ca.lost);
+  }
+}
+
+template<int NL, int G, bool LEARNED, bool WAVE>
+__global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S, const int* q)
+{
+  xs_lane_body<NL, G, LEARNED, 0, WAVE>(P, S, q);
+}
+
+// The fused kernel is held to three wavefronts per SIMD (168 VGPRs), the
+// occupancy of the calculate-xs kernel alone.
+template<int NL, bool LEARNED, int ADV>
+__global__ void __launch_bounds__(BLOCK, 3) k_xs_adv(DevProblem P, DevParticles S, const int* q)
+{
+  xs_lane_body<NL, 4, LEARNED, ADV, false>(P, S, q);
 }
 
 //==============================================================================
@@ -1480,15 +1507,32 @@ void xs_nl(const DevProblem& P, const DevParticles& S, const int* q, int blocks,
   hipLaunchKernelGGL((k_xs<NL, LPP>), dim3(b), dim3(BLOCK), lds, st, P, S, q);
 }
 
-template<int NL, bool LEARNED>
+template<int NL, bool LEARNED, bool WAVE>
 void xs_lane_l(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, int group,
   hipStream_t st)
 {
   switch (group) {
-  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
-  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
-  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
-  default: hipLaunchKernelGGL((k_xs_lane<NL, 4, LEARNED>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  case 1: hipLaunchKernelGGL((k_xs_lane<NL, 1, LEARNED, WAVE>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  case 2: hipLaunchKernelGGL((k_xs_lane<NL, 2, LEARNED, WAVE>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  case 8: hipLaunchKernelGGL((k_xs_lane<NL, 8, LEARNED, WAVE>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  default: hipLaunchKernelGGL((k_xs_lane<NL, 4, LEARNED, WAVE>), dim3(blocks), dim3(tpb), 0, st, P, S, q); break;
+  }
+}
+
+// calculate-xs and advance in one launch (xs_group 4, one lane per particle)
+template<int NL>
+void xs_adv(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, hipStream_t st)
+{
+  if (P.n_parts > 1) {
+    if (S.use_learned)
+      hipLaunchKernelGGL((k_xs_adv<NL, true, 2>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+    else
+      hipLaunchKernelGGL((k_xs_adv<NL, false, 2>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+  } else {
+    if (S.use_learned)
+      hipLaunchKernelGGL((k_xs_adv<NL, true, 1>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+    else
+      hipLaunchKernelGGL((k_xs_adv<NL, false, 1>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
   }
 }
 
@@ -1497,9 +1541,11 @@ void xs_lane(const DevProblem& P, const DevParticles& S, const int* q, int block
   hipStream_t st)
 {
   if (S.use_learned)
-    xs_lane_l<NL, true>(P, S, q, blocks, tpb, group, st);
+    xs_lane_l<NL, true, false>(P, S, q, blocks, tpb, group, st);
+  else if (S.wave_search)
+    xs_lane_l<NL, false, true>(P, S, q, blocks, tpb, group, st);
   else
-    xs_lane_l<NL, false>(P, S, q, blocks, tpb, group, st);
+    xs_lane_l<NL, false, false>(P, S, q, blocks, tpb, group, st);
 }
 
 template<int NL>
@@ -1528,6 +1574,11 @@ void launch_partition(const DevParticles& S, const Part& A, const LaunchConfig& 
   hipLaunchKernelGGL(k_part_scatter, dim3(sb), dim3(BLOCK), 0, st, S, A);
 }
 
+bool fused_advance(const LaunchConfig& lc)
+{
+  return lc.fuse_advance && lc.xs_lanes == 1 && lc.xs_group == 4;
+}
+
 template<int NL>
 void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const LaunchConfig& lc, hipStream_t st,
   int cycle)
@@ -1544,8 +1595,15 @@ void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const Laun
     launch_partition(S, A, lc, st);
     break;
   }
-  case KERNEL_XS: xs_lpp<NL>(P, S, S.q_xs, lc, st); break;
+  case KERNEL_XS:
+    if (fused_advance(lc))
+      xs_adv<NL>(P, S, S.q_xs, lc.blocks, tpb, st);
+    else
+      xs_lpp<NL>(P, S, S.q_xs, lc, st);
+    break;
   case KERNEL_ADVANCE:
+    if (fused_advance(lc))
+      break; // done by the calculate-xs launch
     if (P.n_parts > 1)
       hipLaunchKernelGGL((k_advance<NL, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S, S.q_xs);
     else
