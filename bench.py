@@ -392,10 +392,13 @@ def pmc_summary(dirs, events=None, copy_bytes=None):
                 if base not in KERNEL_UNITS:
                     continue
                 name, unit = KERNEL_UNITS[base]
-                if not events.get(UNIT_COUNTER[unit]):
+                c = UNIT_COUNTER[unit]
+                # the kernel's own events (not those of the history-based tail)
+                n_ev = events.get(c, 0) - events.get("hist_" + c, 0)
+                if n_ev <= 0:
                     continue
                 b = (agg[k]["FETCH_SIZE"] * 1024.0 * fetch_scale + agg[k]["WRITE_SIZE"] * 1024.0 * write_scale)
-                lines.append("traffic %s %s %.2f" % (name, unit, b / events[UNIT_COUNTER[unit]]))
+                lines.append("traffic %s %s %.2f" % (name, unit, b / n_ev))
     return "\n".join(lines)
 
 

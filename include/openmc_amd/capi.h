@@ -117,6 +117,17 @@ int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat
  * the concatenated log-grid tables to grid_index_out.  Returns the number of
  * values available in out (>= 0) or an error code. */
 int omc_check_model(const char* dir, double* out, int n_out, int* grid_index_out, int64_t n_gi);
+/* Nuclear data in the reference's HDF5 layout (openmc.data.IncidentNeutron
+ * export_to_hdf5 / Nuclide::Nuclide): cross_sections.xml may name .h5 files,
+ * read through libhdf5 opened at run time ($OPENMC_HDF5_LIBRARY, else
+ * libhdf5.so, else /opt/conda/lib/libhdf5.so).  The writer stores elastic
+ * scattering (MT 2) and capture (MT 102) at one temperature [K]; the reader
+ * returns the grid size and fills energy/total/absorption/elastic (n_max
+ * points) and out2 = {awr, kT}.  No GPU needed. */
+int omc_write_nuclide_hdf5(const char* path, const char* name, double awr, double kT, int temperature,
+  const double* energy, const double* elastic, const double* capture, int64_t n);
+int omc_read_nuclide(const char* path, const char* name, double temperature, int64_t n_max, double* energy,
+  double* total, double* absorption, double* elastic, double* out2);
 /* Streaming copy of n_bytes, `reps` times: calibrates HBM byte counters and
  * the achievable copy bandwidth; *ms = time per copy. */
 int omc_copy_kernel(int64_t n_bytes, int reps, double* ms);

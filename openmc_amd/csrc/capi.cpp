@@ -17,6 +17,7 @@
 #include <unistd.h>
 #include <vector>
 
+#include "hdf5_io.h"
 #include "host_model.h"
 #include "learned.h"
 #include "openmc_amd/capi.h"
@@ -1591,6 +1592,45 @@ int omc_check_problem_set(int n, const char* const* dirs, double* out, int n_out
     for (int k = 0; k < n_out && k < static_cast<int>(v.size()); ++k)
       out[k] = v[k];
     return static_cast<int>(v.size());
+  });
+}
+
+// Writes one nuclide (elastic scattering and capture at one temperature) in
+// the reference's HDF5 layout (openmc.data.IncidentNeutron.export_to_hdf5).
+int omc_write_nuclide_hdf5(const char* path, const char* name, double awr, double kT, int temperature,
+  const double* energy, const double* elastic, const double* capture, int64_t n)
+{
+  return guarded([&]() -> int {
+    if (!path || !name || n < 2)
+      throw std::runtime_error("omc_write_nuclide_hdf5: need a path, a name and two or more grid points");
+    write_nuclide_hdf5(path, name, awr, kT, temperature, energy, elastic, capture, static_cast<size_t>(n));
+    return 0;
+  });
+}
+
+// Host-only read of one nuclide's data file (.nuc or HDF5) as the library
+// reads it: energy grid and the derived total, absorption and elastic
+// columns (up to n_max points); out2 = {awr, kT}.  Returns the grid size.
+int omc_read_nuclide(const char* path, const char* name, double temperature, int64_t n_max, double* energy,
+  double* total, double* absorption, double* elastic, double* out2)
+{
+  return guarded([&]() -> int {
+    HostNuclide nuc;
+    const std::string p = path ? path : "";
+    if (is_hdf5_path(p))
+      read_nuclide_hdf5(nuc, p, name ? name : "", temperature, 10.0);
+    else
+      throw std::runtime_error("omc_read_nuclide reads HDF5 files");
+    const int64_t n = static_cast<int64_t>(nuc.energy.size());
+    for (int64_t k = 0; k < n && k < n_max; ++k) {
+      energy[k] = nuc.energy[k];
+      total[k] = nuc.total[k];
+      absorption[k] = nuc.absorption[k];
+      elastic[k] = nuc.elastic[k];
+    }
+    out2[0] = nuc.awr;
+    out2[1] = nuc.kT;
+    return static_cast<int>(n);
   });
 }
 

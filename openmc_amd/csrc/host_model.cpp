@@ -1,4 +1,5 @@
 #include "host_model.h"
+#include "hdf5_io.h"
 
 #include <algorithm>
 #include <cmath>
@@ -461,8 +462,12 @@ void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_
         nuc.pooled = pt->second;
       }
     }
-    if (nuc.pooled < 0)
-      read_nuclide_file(nuc, it->second);
+    if (nuc.pooled < 0) {
+      if (is_hdf5_path(it->second))
+        read_nuclide_hdf5(nuc, it->second, kv.first, m.temperature_default, m.temperature_tolerance);
+      else
+        read_nuclide_file(nuc, it->second);
+    }
     nuc.name = kv.first;
     nuc.path = it->second;
   }

@@ -99,6 +99,8 @@ struct HostModel {
   int64_t max_events = 1000000;
   double free_gas_threshold = 400.0;
   double energy_cutoff = 0.0;
+  double temperature_default = 293.6;  // K (src/settings.cpp), for HDF5 data
+  double temperature_tolerance = 10.0; // K
   std::string run_mode = "fixed source";
 
   std::vector<HostNuclide> nuclides;
