@@ -331,10 +331,11 @@ def cache_root():
     return os.environ.get("OPENMC_AMD_DATA", os.path.join(tempfile.gettempdir(), "openmc_amd_data"))
 
 
-def write_problem(model, nuclides, directory):
-    """Write nuclide files, cross_sections.xml and model.xml into directory."""
+def write_problem(model, nuclides, directory, data_format="nuc"):
+    """Write nuclide files (data_format "nuc" or "h5"), cross_sections.xml and
+    model.xml into directory."""
     os.makedirs(directory, exist_ok=True)
-    xs = data.write_library(nuclides, os.path.join(directory, "data"))
+    xs = data.write_library(nuclides, os.path.join(directory, "data"), fmt=data_format)
     model.materials.cross_sections = os.path.relpath(xs, directory)
     model.export_to_model_xml(os.path.join(directory, "model.xml"))
     return directory

@@ -58,6 +58,14 @@ class SyntheticNuclide:
             f.write(self.elastic.astype("<f8").tobytes())
             f.write(self.capture.astype("<f8").tobytes())
 
+    def export_hdf5(self, path):
+        """Write the nuclide in the reference's HDF5 layout
+        (openmc.data.IncidentNeutron.export_to_hdf5): elastic scattering (MT 2)
+        and capture (MT 102) at its temperature, through libopenmc.so."""
+        from . import lib
+        lib.write_nuclide_hdf5(path, self.name, self.awr, self.kTs[0], int(round(self.kTs[0] / K_BOLTZMANN)),
+                               self.energy, self.elastic, self.capture)
+
     @classmethod
     def load(cls, path):
         with open(path, "rb") as f:
@@ -142,13 +150,17 @@ class DataLibrary:
         ET.ElementTree(root).write(path, xml_declaration=True, encoding="utf-8")
 
 
-def write_library(nuclides, directory):
-    """Export nuclides into ``directory`` and write its cross_sections.xml."""
+def write_library(nuclides, directory, fmt="nuc"):
+    """Export nuclides into ``directory`` and write its cross_sections.xml;
+    fmt "nuc" (this package's binary files) or "h5" (the reference's HDF5)."""
     os.makedirs(directory, exist_ok=True)
     lib = DataLibrary()
     for nuc in nuclides:
-        fn = nuc.name + ".nuc"
-        nuc.export(os.path.join(directory, fn))
+        fn = nuc.name + "." + fmt
+        if fmt == "h5":
+            nuc.export_hdf5(os.path.join(directory, fn))
+        else:
+            nuc.export(os.path.join(directory, fn))
         lib.register_file(fn, nuc.name)
     xml_path = os.path.join(directory, "cross_sections.xml")
     lib.export_to_xml(xml_path)

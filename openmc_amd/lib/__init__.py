@@ -127,6 +127,10 @@ _bind("omc_n_problems", [], c_int, check=False)
 _bind("omc_problem_info", [c_int, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)])
 _bind("omc_problem_global_tallies", [c_int, POINTER(POINTER(c_double))])
 _bind("omc_check_problem_set", [c_int, POINTER(c_char_p), POINTER(c_double), c_int], c_int, check=False)
+_bind("omc_write_nuclide_hdf5", [c_char_p, c_char_p, c_double, c_double, c_int, POINTER(c_double),
+                                 POINTER(c_double), POINTER(c_double), c_int64])
+_bind("omc_read_nuclide", [c_char_p, c_char_p, c_double, c_int64, POINTER(c_double), POINTER(c_double),
+                           POINTER(c_double), POINTER(c_double), POINTER(c_double)], c_int, check=False)
 _bind("omc_learned_build", [c_int, c_int, POINTER(c_int)])
 _bind("omc_learned_loss_grad", [c_int, c_int, POINTER(c_float), POINTER(c_double), POINTER(c_double)])
 _bind("omc_learned_model", [c_int, POINTER(c_float), POINTER(c_int)], c_int, check=False)
@@ -440,6 +444,34 @@ def check_problem_set(directories):
     rest = vals[len(keys):]
     parts = [dict(zip(pk, rest[8 * k:8 * k + 8])) for k in range(len(rest) // 8)]
     return head, parts
+
+
+def write_nuclide_hdf5(path, name, awr, kT, temperature, energy, elastic, capture):
+    """Build-specific: one nuclide (MT 2 and MT 102 at one temperature [K]) in
+    the reference's HDF5 layout."""
+    E = np.ascontiguousarray(energy, dtype=np.float64)
+    el = np.ascontiguousarray(elastic, dtype=np.float64)
+    cap = np.ascontiguousarray(capture, dtype=np.float64)
+    if not (E.size == el.size == cap.size):
+        raise ValueError("energy, elastic and capture need the same length")
+    _dll.omc_write_nuclide_hdf5(os.fspath(path).encode(), name.encode(), float(awr), float(kT), int(temperature),
+                                E.ctypes.data_as(POINTER(c_double)), el.ctypes.data_as(POINTER(c_double)),
+                                cap.ctypes.data_as(POINTER(c_double)), E.size)
+
+
+def read_nuclide(path, name, temperature=293.6, n_max=1 << 22):
+    """Build-specific, host only: a nuclide from an HDF5 file as the library
+    reads it: dict(energy, total, absorption, elastic, awr, kT)."""
+    bufs = [np.zeros(n_max) for _ in range(4)]
+    out2 = (c_double * 2)()
+    n = _dll.omc_read_nuclide(os.fspath(path).encode(), name.encode(), float(temperature), n_max,
+                              *[b.ctypes.data_as(POINTER(c_double)) for b in bufs], out2)
+    if n < 0:
+        _error_handler(n, None, None)
+    keys = ["energy", "total", "absorption", "elastic"]
+    out = {k: b[:n].copy() for k, b in zip(keys, bufs)}
+    out["awr"], out["kT"] = out2[0], out2[1]
+    return out
 
 
 def check_model(directory, n_gi=0):
