@@ -128,6 +128,22 @@ int omc_write_nuclide_hdf5(const char* path, const char* name, double awr, doubl
   const double* energy, const double* elastic, const double* capture, int64_t n);
 int omc_read_nuclide(const char* path, const char* name, double temperature, int64_t n_max, double* energy,
   double* total, double* absorption, double* elastic, double* out2);
+/* Statepoints in the reference's layout (openmc_statepoint_write,
+ * include/openmc/capi.h:161; src/state_point.cpp:36-360): written by
+ * openmc_next_batch at the <state_point batches> (default the last batch),
+ * without the source bank, as a fixed-source run does.  filename null: the
+ * default <output path>statepoint.<batch>.h5; write_source null: true (a
+ * fixed-source run keeps no source bank: the dataset is created with one
+ * zero site per particle).  omc_problem_statepoint_write writes one problem
+ * of a problem set (no reference counterpart).  omc_h5_read reads a dataset
+ * (attribute null) or attribute back: kind 0 doubles, 1 int64, 2 raw bytes of
+ * fixed-length strings; *n = values (bytes) available, buf filled when cap
+ * holds them. */
+int openmc_statepoint_write(const char* filename, bool* write_source);
+bool openmc_is_statepoint_batch(void);
+int omc_problem_statepoint_write(int problem, const char* filename, bool* write_source);
+int omc_h5_read(const char* path, const char* object, const char* attribute, int kind, void* buf, size_t cap,
+  size_t* n);
 /* Streaming copy of n_bytes, `reps` times: calibrates HBM byte counters and
  * the achievable copy bandwidth; *ms = time per copy. */
 int omc_copy_kernel(int64_t n_bytes, int reps, double* ms);

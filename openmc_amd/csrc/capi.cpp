@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <stdexcept>
 #include <string>
 #include <unistd.h>
@@ -144,6 +145,11 @@ struct Sim {
   // stats
   double t_transport_ms = 0, t_batch_wall_s = 0;
   int active_batches = 0;
+  // wall-clock timers of the statepoint's runtime group [s]
+  std::string path_input; // the model directory given to openmc_init
+  std::chrono::steady_clock::time_point t_start = std::chrono::steady_clock::now();
+  double t_init_s = 0, t_read_s = 0, t_sim_s = 0, t_active_s = 0, t_tallies_s = 0, t_statepoint_s = 0;
+  bool warned_statepoint = false;
   unsigned long long counters[CNT_N] = {0};
   long long n_cycles = 0, n_checks = 0;
   KernelTimer kt;
@@ -350,6 +356,11 @@ void upload_problem(Sim& s)
     part.disc_prob = m.disc_prob;
     part.disc_alias = m.disc_alias;
     part.domain_ids = m.domain_ids;
+    part.dir = s.path_input;
+    part.meshes = m.meshes;
+    part.filters = m.filters;
+    part.statepoint_batches = m.statepoint_batches;
+    part.path_output = m.path_output;
     m.parts.push_back(std::move(part));
   }
 
@@ -784,11 +795,14 @@ void transport_batch(Sim& s)
   hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  s.t_sim_s += std::chrono::duration<double>(w1 - w0).count();
   if (active) {
     s.t_transport_ms += ms;
     s.t_batch_wall_s += std::chrono::duration<double>(w1 - w0).count();
+    s.t_active_s += std::chrono::duration<double>(w1 - w0).count();
     s.active_batches++;
   }
+  const auto wt = std::chrono::steady_clock::now();
 
   s.total_weight = total_weight;
 
@@ -823,6 +837,7 @@ void transport_batch(Sim& s)
       s.results_dirty = true;
     }
   }
+  s.t_tallies_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - wt).count();
   hip_check(hipMemcpyAsync(s.h_counter_slots.data(), s.S.counters,
               sizeof(unsigned long long) * CNT_N * CNT_SLOTS, hipMemcpyDeviceToHost, s.stream),
     "hipMemcpyAsync");
@@ -843,6 +858,185 @@ void sync_results(Sim& s)
   hip_check(hipMemcpy(s.h_results.data(), s.d_results, sizeof(double) * s.h_results.size(), hipMemcpyDeviceToHost),
     "hipMemcpy");
   s.results_dirty = false;
+}
+
+// The statepoint of problem `part` (the model itself when there is one), in
+// the reference's layout: openmc_statepoint_write (src/state_point.cpp:36-360)
+// with meshes_to_hdf5 / Mesh::to_hdf5 / RegularMesh::to_hdf5_inner
+// (src/mesh.cpp:578-598, 1412-1418, 3874-3892), Filter::to_statepoint and
+// the mesh and energy filters' own (src/tallies/filter_mesh.cpp:72-82,
+// filter_energy.cpp:83-87), write_tally_results (src/hdf5_interface.cpp:689-708)
+// and the source bank's site type (src/state_point.cpp:557-589).  A fixed-source
+// run keeps no source bank, so when asked for it the dataset is created with
+// one site per particle and left unwritten, as the reference leaves it.
+void write_statepoint(Sim& s, int part, const std::string& filename, bool write_source)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  const HostModel& m = s.m;
+  const HostPart& hp = m.parts.at(static_cast<size_t>(part));
+  sync_results(s);
+  H5Writer w(filename);
+  const int64_t f = w.root();
+  w.attr_str(f, "filetype", "statepoint");
+  const int version[2] = {18, 1};     // VERSION_STATEPOINT (include/openmc/constants.h:28)
+  const int openmc_ver[3] = {0, 15, 3}; // the reference release whose layout this is
+  w.attr(f, "version", H5Writer::INT, version, 2, true);
+  w.attr(f, "openmc_version", H5Writer::INT, openmc_ver, 3, true);
+  {
+    char ts[64];
+    std::time_t now = std::time(nullptr);
+    std::strftime(ts, sizeof(ts), "%Y-%m-%d %H:%M:%S", std::localtime(&now));
+    w.attr_str(f, "date_and_time", ts);
+  }
+  w.attr_str(f, "path", hp.dir);
+  const int64_t seed = static_cast<int64_t>(s.P.master_seed);
+  const unsigned long stride = static_cast<unsigned long>(s.P.stride);
+  w.data(f, "seed", H5Writer::INT64, &seed);
+  w.data(f, "stride", H5Writer::SIZE, &stride);
+  w.data_str(f, "energy_mode", "continuous-energy");
+  w.data_str(f, "run_mode", "fixed source");
+  const int8_t no = 0, src_flag = write_source ? 1 : 0;
+  w.attr(f, "photon_transport", H5Writer::BOOL, &no);
+  const int64_t np = hp.n_particles;
+  const int nb = m.n_batches, cb = current_batch;
+  w.data(f, "n_particles", H5Writer::INT64, &np);
+  w.data(f, "n_batches", H5Writer::INT, &nb);
+  w.data(f, "current_batch", H5Writer::INT, &cb);
+  w.attr(f, "source_present", H5Writer::BOOL, &src_flag);
+
+  const int64_t tg = w.group(f, "tallies");
+  {
+    const int64_t mg = w.group(tg, "meshes");
+    const int n_meshes = static_cast<int>(hp.meshes.size());
+    w.attr(mg, "n_meshes", H5Writer::INT, &n_meshes);
+    if (n_meshes > 0) {
+      std::vector<int> ids;
+      for (const auto& e : hp.meshes) {
+        const HostMesh& me = e.second;
+        const int64_t g1 = w.group(mg, "mesh " + std::to_string(e.first));
+        w.data_str(g1, "type", "regular");
+        w.attr(g1, "id", H5Writer::INT, &e.first);
+        const size_t nd = static_cast<size_t>(me.n_dim);
+        w.data(g1, "dimension", H5Writer::INT, me.shape, {nd});
+        w.data(g1, "lower_left", H5Writer::DOUBLE, me.ll, {nd});
+        w.data(g1, "upper_right", H5Writer::DOUBLE, me.ur, {nd});
+        w.data(g1, "width", H5Writer::DOUBLE, me.width, {nd});
+        ids.push_back(e.first);
+      }
+      w.attr(mg, "ids", H5Writer::INT, ids.data(), ids.size(), true);
+    }
+  }
+  {
+    const int64_t fg = w.group(tg, "filters");
+    const unsigned long n_filters = hp.filters.size();
+    w.attr(fg, "n_filters", H5Writer::SIZE, &n_filters);
+    if (n_filters > 0) {
+      std::vector<int> ids;
+      for (const HostFilter& fl : hp.filters)
+        ids.push_back(fl.id);
+      w.attr(fg, "ids", H5Writer::INT, ids.data(), ids.size(), true);
+      for (const HostFilter& fl : hp.filters) {
+        const int64_t g1 = w.group(fg, "filter " + std::to_string(fl.id));
+        w.data_str(g1, "type", fl.type);
+        w.data(g1, "n_bins", H5Writer::INT, &fl.n_bins);
+        if (fl.type == "mesh")
+          w.data(g1, "bins", H5Writer::INT, &fl.mesh_id);
+        else
+          w.data(g1, "bins", H5Writer::DOUBLE, fl.bins.data(), {fl.bins.size()});
+      }
+    }
+  }
+  const HostTally* t = hp.tally >= 0 ? &m.tallies[static_cast<size_t>(hp.tally)] : nullptr;
+  const unsigned long n_tallies = t ? 1 : 0;
+  w.attr(tg, "n_tallies", H5Writer::SIZE, &n_tallies);
+  int64_t tally_group = -1;
+  if (t) {
+    w.attr(tg, "ids", H5Writer::INT, &t->id, 1, true);
+    tally_group = w.group(tg, "tally " + std::to_string(t->id));
+    w.data_str(tally_group, "name", t->name);
+    const int zero = 0, one = 1;
+    w.attr(tally_group, "internal", H5Writer::INT, &zero);
+    w.attr(tally_group, "multiply_density", H5Writer::INT, &one);
+    w.attr(tally_group, "higher_moments", H5Writer::INT, &zero);
+    w.data_str(tally_group, "estimator", t->estimator);
+    const int n_real = s.n_real;
+    w.data(tally_group, "n_realizations", H5Writer::INT, &n_real);
+    const unsigned long n_tf = t->filter_ids.size();
+    w.data(tally_group, "n_filters", H5Writer::SIZE, &n_tf);
+    if (n_tf > 0)
+      w.data(tally_group, "filters", H5Writer::INT, t->filter_ids.data(), {t->filter_ids.size()});
+    w.data_strs(tally_group, "nuclides", {"total"});
+    const unsigned long n_scores = 1;
+    w.data(tally_group, "n_score_bins", H5Writer::SIZE, &n_scores);
+    w.data_strs(tally_group, "score_bins", {"flux"});
+  }
+  // global tallies [N_GLOBAL][3] and the results' sum and sum of squares
+  const std::vector<double>& gt = m.parts.size() > 1 && !s.gt_part.empty() ? s.gt_part[static_cast<size_t>(part)]
+                                                                            : s.gt;
+  w.data(f, "global_tallies", H5Writer::DOUBLE, gt.data(), {static_cast<size_t>(N_GLOBAL), 3});
+  const bool tallies_active = current_batch > m.n_inactive && !m.tallies.empty();
+  const int present = tallies_active ? 1 : 0;
+  w.attr(f, "tallies_present", H5Writer::INT, &present);
+  if (t && tallies_active) {
+    const size_t nbins = static_cast<size_t>(t->n_filter_bins);
+    std::vector<double> r(2 * nbins);
+    const double* src = s.h_results.data() + 3 * t->off;
+    for (size_t b = 0; b < nbins; ++b) {
+      r[2 * b] = src[3 * b + 1];
+      r[2 * b + 1] = src[3 * b + 2];
+    }
+    w.data(tally_group, "results", H5Writer::DOUBLE, r.data(), {nbins, 1, 2});
+  }
+  const int n_real_global = n_realizations;
+  w.data(f, "n_realizations", H5Writer::INT, &n_real_global);
+
+  const int64_t rg = w.group(f, "runtime");
+  const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - s.t_start).count();
+  const struct {
+    const char* name;
+    double v;
+  } rt[] = {{"total initialization", s.t_init_s}, {"reading cross sections", s.t_read_s},
+    {"simulation", s.t_sim_s}, {"transport", s.t_sim_s}, {"active batches", s.t_active_s},
+    {"accumulating tallies", s.t_tallies_s}, {"total", total}, {"writing statepoints", s.t_statepoint_s}};
+  for (const auto& e : rt)
+    w.data(rg, e.name, H5Writer::DOUBLE, &e.v);
+  if (write_source)
+    w.source_bank(f, "source_bank", hp.n_particles);
+  s.t_statepoint_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The default statepoint name: <output path>statepoint.<batch, zero-padded to
+// the width of n_batches>.h5 (src/state_point.cpp:42-52); the output path of
+// a problem of a problem set is relative to its directory.
+std::string statepoint_name(const Sim& s, int part)
+{
+  const HostPart& hp = s.m.parts.at(static_cast<size_t>(part));
+  const int width = static_cast<int>(std::to_string(s.m.n_batches).size());
+  char num[32];
+  std::snprintf(num, sizeof(num), "%0*d", width, current_batch);
+  std::string dir = hp.path_output;
+  if (s.m.parts.size() > 1 && (dir.empty() || dir[0] != '/'))
+    dir = (hp.dir.empty() ? std::string(".") : hp.dir) + "/" + dir;
+  return dir + "statepoint." + num + ".h5";
+}
+
+// Statepoints due at this batch (finalize_batch, src/simulation.cpp:432-444):
+// fixed source, so without the source bank.  Without libhdf5 they are
+// skipped with one warning rather than failing the run.
+void write_due_statepoints(Sim& s)
+{
+  for (size_t k = 0; k < s.m.parts.size(); ++k) {
+    const auto& b = s.m.parts[k].statepoint_batches;
+    if (std::find(b.begin(), b.end(), current_batch) == b.end())
+      continue;
+    if (!hdf5_available()) {
+      if (!s.warned_statepoint)
+        std::fprintf(stderr, "omc: warning: no libhdf5, statepoints are not written\n");
+      s.warned_statepoint = true;
+      return;
+    }
+    write_statepoint(s, static_cast<int>(k), statepoint_name(s, static_cast<int>(k)), false);
+  }
 }
 
 // Device set-up shared by openmc_init and omc_init_problems.
@@ -913,19 +1107,26 @@ int openmc_init(int argc, char* argv[], const void* intracomm)
     if (g && g->initialized)
       openmc_finalize();
     std::string dir = ".";
+    bool dir_given = false;
     for (int i = 1; i < argc; ++i) {
       std::string a = argv[i] ? argv[i] : "";
       if (a == "-e" || a == "--event")
         event_based = true;
       else if (a == "-s" || a == "--threads")
         ++i; // host threads: the GPU path ignores them
-      else if (!a.empty() && a[0] != '-')
+      else if (!a.empty() && a[0] != '-') {
         dir = a;
+        dir_given = true;
+      }
     }
     delete g;
     g = new Sim();
+    g->path_input = dir_given ? dir : "";
+    const auto t0 = std::chrono::steady_clock::now();
     read_model(g->m, dir);
+    g->t_read_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     finish_init(*g);
+    g->t_init_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
   });
 }
@@ -946,8 +1147,11 @@ int omc_init_problems(int n, const char* const* dirs)
       d.push_back(dirs[k] ? dirs[k] : ".");
     delete g;
     g = new Sim();
+    const auto t0 = std::chrono::steady_clock::now();
     read_problem_set(g->m, d);
+    g->t_read_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     finish_init(*g);
+    g->t_init_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
   });
 }
@@ -991,6 +1195,7 @@ int openmc_next_batch(int* status)
     }
     ++current_batch;
     transport_batch(*g);
+    write_due_statepoints(*g);
     if (status)
       *status = current_batch >= g->m.n_batches ? 1 : 0;
     return 0;
@@ -1631,6 +1836,69 @@ int omc_read_nuclide(const char* path, const char* name, double temperature, int
     out2[0] = nuc.awr;
     out2[1] = nuc.kT;
     return static_cast<int>(n);
+  });
+}
+
+// openmc_statepoint_write (include/openmc/capi.h:161, src/state_point.cpp:36-360):
+// a null filename gives the default name of the current batch, a null
+// write_source means true.
+int openmc_statepoint_write(const char* filename, bool* write_source)
+{
+  if (!g || !g->initialized) {
+    set_errmsg("OpenMC has not been initialized.");
+    return OPENMC_E_ALLOCATE;
+  }
+  return guarded([&]() -> int {
+    const std::string name = filename ? std::string(filename) : statepoint_name(*g, 0);
+    const auto dot = name.find_last_of('.');
+    if (dot == std::string::npos || name.substr(dot + 1) != "h5")
+      std::fprintf(stderr, "omc: warning: openmc_statepoint_write was passed a file extension differing from .h5, "
+                           "but an hdf5 file will be written.\n");
+    write_statepoint(*g, 0, name, write_source ? *write_source : true);
+    return 0;
+  });
+}
+
+// openmc_is_statepoint_batch (include/openmc/capi.h:84, src/simulation.cpp)
+bool openmc_is_statepoint_batch(void)
+{
+  if (!g || !g->initialized || g->m.parts.empty())
+    return false;
+  const auto& b = g->m.parts[0].statepoint_batches;
+  return std::find(b.begin(), b.end(), current_batch) != b.end();
+}
+
+// The statepoint of one problem of a problem set (its own tally, global
+// tallies and meshes); filename null: the default name in the problem's
+// directory.  No reference counterpart (the reference runs one model per
+// process).
+int omc_problem_statepoint_write(int problem, const char* filename, bool* write_source)
+{
+  if (!g || !g->initialized || problem < 0 || problem >= static_cast<int>(g->m.parts.size())) {
+    set_errmsg("Index in problems array is out of bounds.");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  return guarded([&]() -> int {
+    const std::string name = filename ? std::string(filename) : statepoint_name(*g, problem);
+    write_statepoint(*g, problem, name, write_source ? *write_source : true);
+    return 0;
+  });
+}
+
+// Reads a dataset (attribute null or empty) or an attribute of an object of
+// an HDF5 file: kind 0 doubles, 1 int64, 2 the raw bytes of fixed-length
+// strings.  *n = the number of values (bytes for kind 2); buf is filled only
+// when cap holds them.  For reading statepoints back without h5py.
+int omc_h5_read(const char* path, const char* object, const char* attribute, int kind, void* buf, size_t cap,
+  size_t* n)
+{
+  return guarded([&]() -> int {
+    if (!path || kind < 0 || kind > 2)
+      throw std::runtime_error("omc_h5_read: need a path and kind 0, 1 or 2");
+    const size_t r = h5_read(path, object ? object : "", attribute ? attribute : "", kind, buf, cap);
+    if (n)
+      *n = r;
+    return 0;
   });
 }
 

@@ -13,6 +13,9 @@
 //       <T>K/xs                 cross section, attr threshold_idx (0-based
 //                               index of its first energy point)
 //
+// The same file writes the statepoint's HDF5 objects (H5Writer) with the
+// conventions of src/hdf5_interface.cpp.
+//
 // libhdf5 is opened at run time (dlopen), so the library loads and runs on
 // .nuc data where no HDF5 is installed; only reading or writing an .h5 file
 // needs it.  Its location: $OPENMC_HDF5_LIBRARY, else libhdf5.so on the
@@ -77,7 +80,16 @@ struct H5 {
   herr_t (*tset_size)(hid_t, size_t) = nullptr;
   herr_t (*tclose)(hid_t) = nullptr;
   herr_t (*eset_auto)(hid_t, void*, void*) = nullptr;
+  hid_t (*tcreate)(int, size_t) = nullptr;
+  herr_t (*tinsert)(hid_t, const char*, size_t, hid_t) = nullptr;
+  hid_t (*dget_type)(hid_t) = nullptr;
+  size_t (*tget_size)(hid_t) = nullptr;
+  int (*tget_class)(hid_t) = nullptr;
+  hid_t (*aopen_by_name)(hid_t, const char*, const char*, hid_t, hid_t) = nullptr;
+  hid_t (*aget_space)(hid_t) = nullptr;
+  hid_t (*aget_type)(hid_t) = nullptr;
   hid_t t_double = -1, t_int = -1, t_int64 = -1, t_str = -1;
+  hid_t t_int8 = -1, t_ulong = -1, t_i64 = -1;
 };
 
 template<class F>
@@ -145,6 +157,14 @@ H5& h5()
   bind(so, t.tset_size, "H5Tset_size");
   bind(so, t.tclose, "H5Tclose");
   bind(so, t.eset_auto, "H5Eset_auto2");
+  bind(so, t.tcreate, "H5Tcreate");
+  bind(so, t.tinsert, "H5Tinsert");
+  bind(so, t.dget_type, "H5Dget_type");
+  bind(so, t.tget_size, "H5Tget_size");
+  bind(so, t.tget_class, "H5Tget_class");
+  bind(so, t.aopen_by_name, "H5Aopen_by_name");
+  bind(so, t.aget_space, "H5Aget_space");
+  bind(so, t.aget_type, "H5Aget_type");
   if (t.open() < 0)
     throw std::runtime_error("H5open failed");
   // the predefined types are globals, valid after H5open
@@ -152,6 +172,9 @@ H5& h5()
   t.t_int = global_type(so, "H5T_NATIVE_INT_g");
   t.t_int64 = global_type(so, "H5T_NATIVE_LLONG_g");
   t.t_str = global_type(so, "H5T_C_S1_g");
+  t.t_int8 = global_type(so, "H5T_NATIVE_INT8_g");
+  t.t_ulong = global_type(so, "H5T_NATIVE_ULONG_g");
+  t.t_i64 = global_type(so, "H5T_NATIVE_INT64_g");
   t.eset_auto(0, nullptr, nullptr); // errors come back as return codes, not printed
   t.so = so;
   h = t;
@@ -244,6 +267,16 @@ void write_vector(hid_t group, const char* name, const double* v, size_t n)
 }
 
 } // namespace
+
+bool hdf5_available()
+{
+  try {
+    h5();
+    return true;
+  } catch (const std::exception&) {
+    return false;
+  }
+}
 
 bool is_hdf5_path(const std::string& path)
 {
@@ -381,6 +414,184 @@ void write_nuclide_hdf5(const std::string& path, const std::string& name, double
     Obj ds(h.dopen(tg.id, "xs", H5P_DEFAULT_), h.dclose, "xs");
     write_attr(ds.id, "threshold_idx", h.t_int, &zero);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Writer for the reference's output files (hdf5_interface.cpp conventions:
+// bool as INT8, size_t as ULONG, int64_t as INT64, a string as a fixed-length
+// C string of its own length, an array of strings padded to the longest + 1,
+// and no object at all for an empty string)
+
+H5Writer::H5Writer(const std::string& path)
+{
+  H5& h = h5();
+  file_ = h.fcreate(path.c_str(), H5F_ACC_TRUNC_, H5P_DEFAULT_, H5P_DEFAULT_);
+  check(file_, "creating " + path);
+}
+
+H5Writer::~H5Writer()
+{
+  H5& h = h5();
+  for (auto it = groups_.rbegin(); it != groups_.rend(); ++it)
+    h.gclose(*it);
+  if (file_ >= 0)
+    h.fclose(file_);
+}
+
+int64_t H5Writer::group(int64_t parent, const std::string& name)
+{
+  H5& h = h5();
+  const hid_t g = h.gcreate(parent, name.c_str(), H5P_DEFAULT_, H5P_DEFAULT_, H5P_DEFAULT_);
+  check(g, "creating group " + name);
+  groups_.push_back(g);
+  return g;
+}
+
+namespace {
+
+void write_any(hid_t obj, const std::string& name, hid_t type, int ndim, const hsize_t* dims, const void* data,
+  bool attribute)
+{
+  H5& h = h5();
+  Obj sp(ndim > 0 ? h.screate_simple(ndim, dims, nullptr) : h.screate(0 /* H5S_SCALAR */), h.sclose, "dataspace");
+  if (attribute) {
+    Obj a(h.acreate(obj, name.c_str(), type, sp.id, H5P_DEFAULT_, H5P_DEFAULT_), h.aclose, "attribute " + name);
+    check(h.awrite(a.id, type, data), "writing attribute " + name);
+  } else {
+    Obj d(h.dcreate(obj, name.c_str(), type, sp.id, H5P_DEFAULT_, H5P_DEFAULT_, H5P_DEFAULT_), h.dclose,
+      "dataset " + name);
+    check(h.dwrite(d.id, type, 0, 0, H5P_DEFAULT_, data), "writing " + name);
+  }
+}
+
+hid_t h5type(H5Writer::Kind k)
+{
+  H5& h = h5();
+  switch (k) {
+  case H5Writer::BOOL: return h.t_int8;
+  case H5Writer::INT: return h.t_int;
+  case H5Writer::INT64: return h.t_i64;
+  case H5Writer::SIZE: return h.t_ulong;
+  default: return h.t_double;
+  }
+}
+
+} // namespace
+
+void H5Writer::attr(int64_t obj, const std::string& name, Kind k, const void* v, size_t n, bool array)
+{
+  const hsize_t dims[1] = {static_cast<hsize_t>(n)};
+  write_any(obj, name, h5type(k), array ? 1 : 0, dims, v, true);
+}
+
+void H5Writer::data(int64_t obj, const std::string& name, Kind k, const void* v, const std::vector<size_t>& shape)
+{
+  std::vector<hsize_t> dims(shape.begin(), shape.end());
+  write_any(obj, name, h5type(k), static_cast<int>(dims.size()), dims.data(), v, false);
+}
+
+void H5Writer::attr_str(int64_t obj, const std::string& name, const std::string& s)
+{
+  if (s.empty())
+    return;
+  H5& h = h5();
+  Obj t(h.tcopy(h.t_str), h.tclose, "string type");
+  check(h.tset_size(t.id, s.size()), "string size");
+  write_any(obj, name, t.id, 0, nullptr, s.c_str(), true);
+}
+
+void H5Writer::data_str(int64_t obj, const std::string& name, const std::string& s)
+{
+  if (s.empty())
+    return;
+  H5& h = h5();
+  Obj t(h.tcopy(h.t_str), h.tclose, "string type");
+  check(h.tset_size(t.id, s.size()), "string size");
+  write_any(obj, name, t.id, 0, nullptr, s.c_str(), false);
+}
+
+void H5Writer::data_strs(int64_t obj, const std::string& name, const std::vector<std::string>& v)
+{
+  size_t m = 1;
+  for (const auto& s : v)
+    m = std::max(m, s.size() + 1);
+  std::vector<char> buf(v.size() * m, '\0');
+  for (size_t i = 0; i < v.size(); ++i)
+    std::copy(v[i].begin(), v[i].end(), buf.begin() + static_cast<std::ptrdiff_t>(i * m));
+  H5& h = h5();
+  Obj t(h.tcopy(h.t_str), h.tclose, "string type");
+  check(h.tset_size(t.id, m), "string size");
+  const hsize_t dims[1] = {static_cast<hsize_t>(v.size())};
+  write_any(obj, name, t.id, 1, dims, buf.data(), false);
+}
+
+void H5Writer::source_bank(int64_t obj, const std::string& name, int64_t n_sites)
+{
+  // the file layout of a source site (h5banktype(false), src/state_point.cpp:557-589):
+  // r and u as {x, y, z} doubles, E, time, wgt, then delayed_group, surf_id, particle
+  H5& h = h5();
+  constexpr int COMPOUND = 6; // H5T_COMPOUND
+  Obj pos(h.tcreate(COMPOUND, 24), h.tclose, "position type");
+  check(h.tinsert(pos.id, "x", 0, h.t_double), "position type");
+  check(h.tinsert(pos.id, "y", 8, h.t_double), "position type");
+  check(h.tinsert(pos.id, "z", 16, h.t_double), "position type");
+  Obj site(h.tcreate(COMPOUND, 2 * 24 + 3 * 8 + 3 * 4), h.tclose, "source site type");
+  check(h.tinsert(site.id, "r", 0, pos.id), "source site type");
+  check(h.tinsert(site.id, "u", 24, pos.id), "source site type");
+  check(h.tinsert(site.id, "E", 48, h.t_double), "source site type");
+  check(h.tinsert(site.id, "time", 56, h.t_double), "source site type");
+  check(h.tinsert(site.id, "wgt", 64, h.t_double), "source site type");
+  check(h.tinsert(site.id, "delayed_group", 72, h.t_int), "source site type");
+  check(h.tinsert(site.id, "surf_id", 76, h.t_int), "source site type");
+  check(h.tinsert(site.id, "particle", 80, h.t_int), "source site type");
+  const hsize_t dims[1] = {static_cast<hsize_t>(n_sites)};
+  Obj sp(h.screate_simple(1, dims, nullptr), h.sclose, "dataspace");
+  // created and left unwritten: its sites read back as the fill value (zeros)
+  Obj d(h.dcreate(obj, name.c_str(), site.id, sp.id, H5P_DEFAULT_, H5P_DEFAULT_, H5P_DEFAULT_), h.dclose,
+    "dataset " + name);
+}
+
+size_t h5_read(const std::string& path, const std::string& object, const std::string& attribute, int kind,
+  void* buf, size_t cap)
+{
+  H5& h = h5();
+  Obj f(h.fopen(path.c_str(), H5F_ACC_RDONLY_, H5P_DEFAULT_), h.fclose, "opening " + path);
+  const bool is_attr = !attribute.empty();
+  hid_t id, space, type;
+  herr_t (*close)(hid_t);
+  if (is_attr) {
+    id = h.aopen_by_name(f.id, object.empty() ? "." : object.c_str(), attribute.c_str(), H5P_DEFAULT_,
+      H5P_DEFAULT_);
+    check(id, "attribute " + attribute + " of " + object);
+    close = h.aclose;
+  } else {
+    id = h.dopen(f.id, object.c_str(), H5P_DEFAULT_);
+    check(id, "dataset " + object);
+    close = h.dclose;
+  }
+  Obj o(id, close, object);
+  Obj sp(is_attr ? h.aget_space(id) : h.dget_space(id), h.sclose, "dataspace");
+  const int64_t n = h.sget_npoints(sp.id);
+  check(n, "dataspace size");
+  Obj t(is_attr ? h.aget_type(id) : h.dget_type(id), h.tclose, "type");
+  constexpr int STRING = 3; // H5T_STRING
+  if (kind == 2) {
+    if (h.tget_class(t.id) != STRING)
+      throw std::runtime_error(object + " " + attribute + " is not a string");
+    const size_t size = h.tget_size(t.id);
+    const size_t total = static_cast<size_t>(n) * size;
+    if (buf && total <= cap && total > 0)
+      check(is_attr ? h.aread(id, t.id, buf) : h.dread(id, t.id, 0, 0, H5P_DEFAULT_, buf), "reading " + object);
+    return total;
+  }
+  const size_t count = static_cast<size_t>(n);
+  if (buf && count <= cap && count > 0) {
+    if (h.tget_class(t.id) == STRING)
+      throw std::runtime_error(object + " " + attribute + " is a string");
+    const hid_t mt = kind == 0 ? h.t_double : h.t_i64;
+    check(is_attr ? h.aread(id, mt, buf) : h.dread(id, mt, 0, 0, H5P_DEFAULT_, buf), "reading " + object);
+  }
+  return count;
 }
 
 } // namespace omc

@@ -197,6 +197,16 @@ bool to_bool(const std::string& v)
   return v == "true" || v == "1" || v == "True";
 }
 
+// model::meshes holds each mesh once, settings' before tallies' (read_meshes,
+// src/mesh.cpp)
+void add_mesh(HostModel& m, int id, const HostMesh& me)
+{
+  for (const auto& e : m.meshes)
+    if (e.first == id)
+      throw std::runtime_error("Two or more meshes use the same unique ID: " + std::to_string(id));
+  m.meshes.emplace_back(id, me);
+}
+
 // <weight_windows>, <weight_windows_on>, <weight_window_checkpoints>,
 // <max_history_splits> (src/settings.cpp:1203-1260, src/weight_windows.cpp:165-229)
 void read_weight_windows(HostModel& m, const XmlNode& s)
@@ -204,8 +214,11 @@ void read_weight_windows(HostModel& m, const XmlNode& s)
   HostWW& w = m.ww;
   auto wws = s.children_named("weight_windows");
   std::unordered_map<int, HostMesh> meshes;
-  for (const XmlNode* mn : s.children_named("mesh"))
-    meshes[std::stoi(mn->value("id"))] = parse_regular_mesh(*mn);
+  for (const XmlNode* mn : s.children_named("mesh")) {
+    const int id = std::stoi(mn->value("id"));
+    meshes[id] = parse_regular_mesh(*mn);
+    add_mesh(m, id, meshes[id]);
+  }
   const XmlNode* sel = nullptr;
   for (const XmlNode* n : wws) {
     if (n->value("particle_type") != "neutron")
@@ -274,6 +287,19 @@ void read_settings(HostModel& m, const XmlNode& s)
   }
   m.n_particles = std::stoll(s.value("particles"));
   m.n_batches = std::stoi(s.value("batches"));
+  // statepoints: the listed batches, else the last one
+  m.statepoint_batches.clear();
+  if (const XmlNode* sp = s.child("state_point"); sp && sp->has("batches")) {
+    for (long long b : to_ints(sp->value("batches")))
+      m.statepoint_batches.push_back(static_cast<int>(b));
+  } else {
+    m.statepoint_batches.push_back(m.n_batches);
+  }
+  if (const XmlNode* out = s.child("output"); out && out->has("path")) {
+    m.path_output = out->value("path");
+    if (!m.path_output.empty() && m.path_output.back() != '/')
+      m.path_output += '/';
+  }
   if (s.has("inactive"))
     m.n_inactive = std::stoi(s.value("inactive"));
   if (s.has("seed"))
@@ -756,8 +782,11 @@ void read_geometry(HostModel& m, const XmlNode& g)
 void read_tallies(HostModel& m, const XmlNode& t)
 {
   std::unordered_map<int, HostMesh> meshes;
-  for (const XmlNode* mn : t.children_named("mesh"))
-    meshes[std::stoi(mn->value("id"))] = parse_regular_mesh(*mn);
+  for (const XmlNode* mn : t.children_named("mesh")) {
+    const int id = std::stoi(mn->value("id"));
+    meshes[id] = parse_regular_mesh(*mn);
+    add_mesh(m, id, meshes[id]);
+  }
   struct Filt {
     std::string type;
     std::vector<double> bins;
@@ -776,10 +805,26 @@ void read_tallies(HostModel& m, const XmlNode& t)
       throw std::runtime_error("filter type '" + f.type + "' is outside the GPU path");
     }
     filters[std::stoi(fn->value("id"))] = f;
+    HostFilter hf;
+    hf.id = std::stoi(fn->value("id"));
+    hf.type = f.type;
+    hf.mesh_id = f.mesh;
+    hf.bins = f.bins;
+    if (f.type == "mesh") {
+      auto mt = meshes.find(f.mesh);
+      if (mt == meshes.end())
+        throw std::runtime_error("mesh filter refers to unknown mesh");
+      hf.n_bins = static_cast<int>(mt->second.n_bins());
+    } else {
+      hf.n_bins = static_cast<int>(f.bins.size()) - 1;
+    }
+    m.filters.push_back(hf);
   }
   for (const XmlNode* tn : t.children_named("tally")) {
     HostTally ht;
     ht.id = std::stoi(tn->value("id"));
+    if (tn->attr("name"))
+      ht.name = *tn->attr("name");
     auto scores = split_ws(tn->value("scores"));
     if (scores.size() != 1 || scores[0] != "flux")
       throw std::runtime_error("only the flux score is supported by the GPU path");
@@ -1054,6 +1099,10 @@ void read_problem_set(HostModel& out, const std::vector<std::string>& dirs)
     part.disc_prob = m.disc_prob;
     part.disc_alias = m.disc_alias;
     part.domain_ids = m.domain_ids;
+    part.meshes = m.meshes;
+    part.filters = m.filters;
+    part.statepoint_batches = m.statepoint_batches;
+    part.path_output = m.path_output;
 
     // tally
     if (!m.tallies.empty()) {

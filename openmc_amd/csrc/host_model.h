@@ -28,6 +28,7 @@ struct HostNuclide {
 
 struct HostTally {
   int id = 0;
+  std::string name;
   bool has_mesh = false;
   int mesh_id = 0;
   int n_dim = 3;
@@ -49,6 +50,16 @@ struct HostMesh {
   int shape[3] = {1, 1, 1};
   double ll[3] = {0, 0, 0}, ur[3] = {0, 0, 0}, width[3] = {0, 0, 0};
   int64_t n_bins() const { return static_cast<int64_t>(shape[0]) * shape[1] * shape[2]; }
+};
+
+// A tally filter as defined in the model (all of them, in definition order,
+// for the statepoint: src/tallies/filter.cpp, filter_mesh.cpp, filter_energy.cpp)
+struct HostFilter {
+  int id = 0;
+  std::string type; // "mesh" or "energy"
+  int mesh_id = 0;  // mesh filter
+  std::vector<double> bins; // energy filter
+  int n_bins = 0;
 };
 
 // Mesh weight windows for neutrons (src/weight_windows.cpp:165-229 reads the
@@ -83,6 +94,11 @@ struct HostPart {
   DevSource src {};
   std::vector<double> disc_x, disc_p, disc_prob;
   std::vector<int> disc_alias, domain_ids;
+  // what the problem's statepoint lists
+  std::vector<std::pair<int, HostMesh>> meshes;
+  std::vector<HostFilter> filters;
+  std::vector<int> statepoint_batches;
+  std::string path_output;
 };
 
 struct HostModel {
@@ -130,6 +146,13 @@ struct HostModel {
   std::vector<int> disc_alias, domain_ids;
 
   std::vector<HostTally> tallies;
+  // every mesh of the model in definition order (settings before tallies)
+  std::vector<std::pair<int, HostMesh>> meshes;
+  std::vector<HostFilter> filters;
+  // <state_point batches> (default: the last batch) and <output><path>
+  // (src/settings.cpp:823-843, 1050-1057)
+  std::vector<int> statepoint_batches;
+  std::string path_output;
 
   double energy_min = 0, energy_max = 0, log_spacing = 0;
   std::vector<int> grid_index; // [n_nuc * (n_log_bins + 1)]

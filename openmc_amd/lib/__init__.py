@@ -14,8 +14,8 @@ nothing here falls back to a CPU implementation.
 """
 import os
 from contextlib import contextmanager
-from ctypes import (CDLL, POINTER, c_bool, c_char, c_char_p, c_double, c_float, c_int, c_int32,
-                    c_int64, c_size_t, c_uint64, c_void_p, create_string_buffer)
+from ctypes import (CDLL, POINTER, byref, c_bool, c_char, c_char_p, c_double, c_float, c_int, c_int32,
+                    c_int64, c_size_t, c_uint64, c_void_p, cast, create_string_buffer)
 
 import numpy as np
 
@@ -131,6 +131,10 @@ _bind("omc_write_nuclide_hdf5", [c_char_p, c_char_p, c_double, c_double, c_int, 
                                  POINTER(c_double), POINTER(c_double), c_int64])
 _bind("omc_read_nuclide", [c_char_p, c_char_p, c_double, c_int64, POINTER(c_double), POINTER(c_double),
                            POINTER(c_double), POINTER(c_double), POINTER(c_double)], c_int, check=False)
+_bind("openmc_statepoint_write", [c_char_p, POINTER(c_bool)])
+_bind("openmc_is_statepoint_batch", [], c_bool, check=False)
+_bind("omc_problem_statepoint_write", [c_int, c_char_p, POINTER(c_bool)])
+_bind("omc_h5_read", [c_char_p, c_char_p, c_char_p, c_int, c_void_p, c_size_t, POINTER(c_size_t)])
 _bind("omc_learned_build", [c_int, c_int, POINTER(c_int)])
 _bind("omc_learned_loss_grad", [c_int, c_int, POINTER(c_float), POINTER(c_double), POINTER(c_double)])
 _bind("omc_learned_model", [c_int, POINTER(c_float), POINTER(c_int)], c_int, check=False)
@@ -457,6 +461,60 @@ def write_nuclide_hdf5(path, name, awr, kT, temperature, energy, elastic, captur
     _dll.omc_write_nuclide_hdf5(os.fspath(path).encode(), name.encode(), float(awr), float(kT), int(temperature),
                                 E.ctypes.data_as(POINTER(c_double)), el.ctypes.data_as(POINTER(c_double)),
                                 cap.ctypes.data_as(POINTER(c_double)), E.size)
+
+
+def statepoint_write(filename=None, write_source=True):
+    """Write a statepoint file (openmc/lib/core.py:566-580).
+
+    Parameters
+    ----------
+    filename : str or None
+        Path to the statepoint to write. If None is passed, a default name that
+        contains the current batch will be written.
+    write_source : bool
+        Whether or not to include the source bank in the statepoint.
+    """
+    if filename is not None:
+        filename = c_char_p(os.fspath(filename).encode())
+    _dll.openmc_statepoint_write(filename, c_bool(write_source))
+
+
+def is_statepoint_batch():
+    """Whether the current batch is one of the statepoint batches."""
+    return bool(_dll.openmc_is_statepoint_batch())
+
+
+def problem_statepoint_write(problem, filename=None, write_source=True):
+    """Build-specific: the statepoint of one problem of a problem set."""
+    if filename is not None:
+        filename = c_char_p(os.fspath(filename).encode())
+    _dll.omc_problem_statepoint_write(int(problem), filename, c_bool(write_source))
+
+
+def h5_read(path, obj, attribute=None, kind="f8"):
+    """Build-specific, host only: a dataset (or an attribute of `obj`) of an
+    HDF5 file through the library's libhdf5, without h5py.  kind "f8" and
+    "i8" give numpy arrays (flattened), "str" a list of strings."""
+    k = {"f8": 0, "i8": 1, "str": 2}[kind]
+    n = c_size_t(0)
+    args = [os.fspath(path).encode(), obj.encode(), attribute.encode() if attribute else None, k]
+    _dll.omc_h5_read(*args, None, 0, byref(n))
+    if k == 2:
+        buf = (c_char * max(n.value, 1))()
+        _dll.omc_h5_read(*args, cast(buf, c_void_p), n.value, byref(n))
+        return [w.decode() for w in bytes(buf.raw[:n.value]).split(b"\0") if w]
+    out = np.zeros(n.value, dtype=np.float64 if k == 0 else np.int64)
+    _dll.omc_h5_read(*args, out.ctypes.data_as(c_void_p), out.size, byref(n))
+    return out
+
+
+def h5_count(path, obj, attribute=None):
+    """Build-specific, host only: the number of elements of a dataset or
+    attribute (of any type, compound included)."""
+    n = c_size_t(0)
+    _dll.omc_h5_read(os.fspath(path).encode(), obj.encode(), attribute.encode() if attribute else None, 1, None, 0,
+                     byref(n))
+    return n.value
 
 
 def read_nuclide(path, name, temperature=293.6, n_max=1 << 22):

@@ -370,6 +370,8 @@ class Settings:
         self.weight_windows_on = None
         self.weight_window_checkpoints = {}
         self.max_history_splits = None
+        self.statepoint = {}  # {"batches": [...]} (openmc/settings.py:1477-1483)
+        self.output = None    # {"path": ...} (openmc/settings.py:1462-1470)
 
     def to_xml_element(self):
         el = ET.Element("settings")
@@ -407,6 +409,14 @@ class Settings:
                     ET.SubElement(ck, key).text = str(bool(self.weight_window_checkpoints[key])).lower()
         if self.max_history_splits is not None:
             ET.SubElement(el, "max_history_splits").text = str(int(self.max_history_splits))
+        if self.output is not None:
+            oe = ET.SubElement(el, "output")
+            for key, value in sorted(self.output.items()):
+                ET.SubElement(oe, key).text = str(value).lower() if key in ("summary", "tallies") else value
+        if self.statepoint:
+            se = ET.SubElement(el, "state_point")
+            if "batches" in self.statepoint:
+                ET.SubElement(se, "batches").text = " ".join(str(x) for x in self.statepoint["batches"])
         return el
 
 
