@@ -66,7 +66,7 @@ def kernel_bytes(st):
 
 # statistics that are settings, not counts (reported as they stand, not differenced)
 CONFIG_STATS = ("blocks", "xs_lanes", "sort", "xs_group", "block", "sort_every", "cycles_per_check", "sort_bits",
-                "sort_blocks", "history_below", "history_steps", "fused_advance")
+                "sort_blocks", "history_below", "history_steps", "fused_advance", "recompute_micro")
 
 
 def diff(a, b):
@@ -240,7 +240,7 @@ def run_problem_set(dirs, steps, warmup, options=None, tune_path=None):
         t1 = time.perf_counter()
         s1 = lib.stats()
         info["options"] = {k: s1[k] for k in ("blocks", "block", "sort_every", "cycles_per_check", "sort_bits",
-                                              "xs_group", "history_below", "fused_advance")}
+                                              "xs_group", "history_below", "fused_advance", "recompute_micro")}
         info["config_stats"] = {k: s1.get(k, 0) for k in CONFIG_STATS}
         lib.simulation_finalize()
         info["n_particles"] = [lib.problem_info(k)["n_particles"] for k in range(lib.n_problems())]

@@ -93,8 +93,11 @@ void openmc_set_stride(uint64_t new_stride);
  * "sort_every" (full material+energy sort every n cycles, material-only in
  * between), "sort_blocks", "xs_lanes", "xs_group", "bank_cap", "block",
  * "wave_search" (wavefront-cooperative grid search, 0/1), "tally_copies"
- * (1/2/4/8 copies of each tally's VALUE, one per XCD), "progress_s" (a
- * progress line on stderr every this many seconds of a batch, 0 = none). */
+ * (1/2/4/8 copies of each tally's VALUE, one per XCD), "recompute_micro"
+ * (collisions repeat the nuclide lookups instead of reading per-particle
+ * micro-total rows that calculate-xs would write: 1, 0, or -1 = when the
+ * largest material has at most 64 nuclides), "progress_s" (a progress line on
+ * stderr every this many seconds of a batch, 0 = none). */
 int omc_set_option(const char* name, double value);
 /* Run statistics of the simulation so far (see omc_stat_names()). */
 int omc_get_stats(double* out, int n);

@@ -11,6 +11,8 @@ history (every one of them is covered by the launch-shape parity tests):
   xs_group          nuclide lookups interleaved per lane in calculate-xs
   history_below     the queue threshold below which a batch finishes history-based
   fuse_advance      advance in the calculate-xs launch (xs_group 4)
+  recompute_micro   collisions repeat the nuclide lookups instead of reading
+                    the micro-total rows (-1: chosen by material size)
 
 ``tune`` times whole batches of the problem (or problem set) that the
 library holds and picks, option by option, the fastest value (coordinate
@@ -38,9 +40,10 @@ DEFAULT_SPACE = [
     ("xs_group", [4, 2, 8]),
     ("history_below", [65536, 0, 16384, 262144, 1048576]),
     ("fuse_advance", [0, 1]),
+    ("recompute_micro", [-1, 0, 1]),
 ]
 DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": 5, "sort_every": 1, "cycles_per_check": 16, "xs_group": 4,
-            "history_below": 65536, "fuse_advance": 0}
+            "history_below": 65536, "fuse_advance": 0, "recompute_micro": -1}
 
 
 def _time_batch():

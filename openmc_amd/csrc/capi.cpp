@@ -75,6 +75,9 @@ void hip_check(hipError_t e, const char* what)
     throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// recompute_micro's automatic choice: up to this many nuclides in the
+// largest material
+constexpr int RECOMPUTE_MAX_NUC = 64;
 constexpr int N_GLOBAL = 4; // K_COLLISION, K_ABSORPTION, K_TRACKLENGTH, LEAKAGE
 constexpr int GT_LEAKAGE = 3;
 
@@ -106,6 +109,7 @@ struct Sim {
   int history_below = 65536; // finish a batch history-based once fewer particles than this are live (0: never)
   int history_steps = 2048;  // event cycles per particle per history-based launch
   long long n_hist_launches = 0;
+  int recompute_micro = -1; // collisions repeat the lookups instead of reading micro_t rows (-1: by material size)
   bool wave_search = false; // wavefront-cooperative grid searches (plain lookups; measured 4% slower)
   std::vector<DevPart> h_parts;
   void* parts_alloc = nullptr;
@@ -688,6 +692,10 @@ void transport_batch(Sim& s)
   s.S.trace_hash = s.trace_hash ? 1 : 0;
   s.S.use_learned = (s.use_learned && s.P.learned.enabled) ? 1 : 0;
   s.S.wave_search = s.wave_search ? 1 : 0;
+  // auto: look the micro totals up again in the collision when materials are
+  // small (measured faster with 20 nuclides a material, slower with 300)
+  s.S.recompute_micro = s.recompute_micro < 0 ? (s.P.max_mat_nuc <= RECOMPUTE_MAX_NUC ? 1 : 0)
+                                               : (s.recompute_micro ? 1 : 0);
   if (s.trace_hash)
     s.h_hashes.assign(m.n_particles, 0);
   hip_check(hipMemsetAsync(s.S.leak, 0, sizeof(double) * leak_slots(s), s.stream), "hipMemsetAsync");
@@ -1531,6 +1539,8 @@ int omc_set_option(const char* name, double value)
   } else if (n == "fuse_advance") {
     g->fuse_advance = value != 0;
     update_launch_config(*g);
+  } else if (n == "recompute_micro") {
+    g->recompute_micro = value < 0 ? -1 : (value != 0 ? 1 : 0);
   } else if (n == "wave_search") {
     g->wave_search = value != 0;
   } else if (n == "tally_copies") {
@@ -1583,7 +1593,7 @@ static const char* k_stat_names =
   "t_coll_ms,t_init_ms,t_hist_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,n_hist,wall_s,blocks,xs_lanes,"
   "sort,xs_group,block,sort_every,cycles_per_check,sort_bits,sort_blocks,history_below,history_steps,"
   "hist_launches,hist_xs_events,hist_lookups,hist_adv_events,hist_segments,hist_surf_events,hist_coll_events,"
-  "fused_advance";
+  "fused_advance,recompute_micro";
 
 const char* omc_stat_names(void)
 {
@@ -1629,6 +1639,7 @@ int omc_get_stats(double* out, int n)
   for (int c = CNT_HIST_XS_EVENTS; c <= CNT_HIST_COLL_EVENTS; ++c)
     v.push_back(static_cast<double>(g->counters[c]));
   v.push_back(g->lc.fuse_advance && g->lc.xs_lanes == 1 && g->lc.xs_group == 4 ? 1.0 : 0.0);
+  v.push_back(g->S.recompute_micro);
   for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
     out[k] = v[k];
   return static_cast<int>(v.size());

@@ -375,7 +375,7 @@ struct DevParticles {
   int trace_hash;
   int use_learned;
   int wave_search; // plain lookups by wavefront-cooperative search (else per lane)
-  int pad_ws;
+  int recompute_micro; // collisions look the micro totals up again (no micro_t rows)
 };
 
 } // namespace omc

@@ -605,7 +605,8 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
       for (int jj = sub; jj < nn; jj += LPP) {
         const int j = j0 + jj;
         Micro m = nuclide_xs(P, P.mat_nuc[j], E, i_log, lx, learned, probes);
-        micro[jj] = m.total;
+        if (!S.recompute_micro)
+          micro[jj] = m.total;
         const double dens = P.mat_dens[j] * 1.0;
         prod[2 * jj] = dens * m.total;
         prod[2 * jj + 1] = dens * m.absorption;
@@ -684,7 +685,9 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
     const int kk = nn - jj < G ? nn - jj : G;
     Micro m[G];
     nuclide_xs_group<G, LEARNED>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
-    store_micro<G>(micro + jj, m, kk);
+    if (!S.recompute_micro)
+      if (!S.recompute_micro)
+        store_micro<G>(micro + jj, m, kk);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
@@ -1168,8 +1171,9 @@ __device__ __forceinline__ V3 sample_cxs_target_velocity(double awr, double E, V
 // nuclide from the particle's micro_t row, analog absorption, elastic
 // scattering with free-gas target motion, weight windows, then the revive
 // epilogue.  Returns whether the particle (or its revived secondary) is
-// alive; key is its calculate-xs sort key.
-template<int NL, bool WW>
+// alive; key is its calculate-xs sort key.  RECOMP: the micro totals are
+// looked up again instead of read from the row (no rows are written then).
+template<int NL, bool WW, bool RECOMP>
 __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevParticles& S, int p, EvCount& c,
   int& key)
 {
@@ -1184,30 +1188,69 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
   double cutoff = prn(&seed) * h.xs_t;
   const int j0 = P.mat_off[material];
   const int j1 = P.mat_off[material + 1];
-  // the row is read two values per 16-byte load (rows are padded to a
-  // multiple of four values)
-  const double2* micro2 = reinterpret_cast<const double2*>(S.micro_t + static_cast<int64_t>(p) * S.max_nn);
+  const double lx = detm::log(E / P.energy_min);
+  int i_log = static_cast<int>(lx / P.log_spacing);
+  i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
   double prob = 0.0;
   int i_nuclide = -1;
   int j_nuclide = -1; // its entry in the material's list
-  for (int j = j0; j < j1; j += 2) {
-    const double2 mm = micro2[(j - j0) >> 1];
-    const double dens = P.mat_dens[j] * 1.0;
-    prob += dens * mm.x;
-    if (prob >= cutoff) {
-      i_nuclide = P.mat_nuc[j];
-      j_nuclide = j;
-      break;
-    }
-    if (j + 1 < j1) {
-      const double dens1 = P.mat_dens[j + 1] * 1.0;
-      prob += dens1 * mm.y;
-      if (prob >= cutoff) {
-        i_nuclide = P.mat_nuc[j + 1];
-        j_nuclide = j + 1;
-        break;
+  Micro m {};
+  int probes = 0;
+  if constexpr (RECOMP) {
+    // the micro totals again, CG nuclides at a time (their searches
+    // interleaved), until the sum passes the cutoff: the same lookups as
+    // calculate-xs, so the same values
+    constexpr int CG = 2; // measured: 2 beats 1 and 4 (4 costs 195 VGPRs)
+    const bool learned = S.use_learned != 0;
+    for (int jj = 0; jj < j1 - j0 && i_nuclide < 0; jj += CG) {
+      const int kk = j1 - j0 - jj < CG ? j1 - j0 - jj : CG;
+      Micro mg[CG];
+      int pr = 0;
+      if (learned)
+        nuclide_xs_group<CG, true, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, pr, mg);
+      else
+        nuclide_xs_group<CG, false, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, pr, mg);
+      probes += pr;
+#pragma unroll
+      for (int k = 0; k < CG; ++k) {
+        if (k < kk && i_nuclide < 0) {
+          const int j = j0 + jj + k;
+          const double dens = P.mat_dens[j] * 1.0;
+          prob += dens * mg[k].total;
+          if (prob >= cutoff) {
+            i_nuclide = P.mat_nuc[j];
+            j_nuclide = j;
+            m = mg[k];
+          }
+        }
       }
     }
+  } else {
+    // the row is read two values per 16-byte load (rows are padded to a
+    // multiple of four values)
+    const double2* micro2 = reinterpret_cast<const double2*>(S.micro_t + static_cast<int64_t>(p) * S.max_nn);
+    for (int j = j0; j < j1; j += 2) {
+      const double2 mm = micro2[(j - j0) >> 1];
+      const double dens = P.mat_dens[j] * 1.0;
+      prob += dens * mm.x;
+      if (prob >= cutoff) {
+        i_nuclide = P.mat_nuc[j];
+        j_nuclide = j;
+        break;
+      }
+      if (j + 1 < j1) {
+        const double dens1 = P.mat_dens[j + 1] * 1.0;
+        prob += dens1 * mm.y;
+        if (prob >= cutoff) {
+          i_nuclide = P.mat_nuc[j + 1];
+          j_nuclide = j + 1;
+          break;
+        }
+      }
+    }
+    // micro cross sections of the sampled nuclide at this energy
+    if (i_nuclide >= 0)
+      m = nuclide_xs(P, i_nuclide, E, i_log, lx, S.use_learned != 0, probes);
   }
   bool absorbed = false;
   if (i_nuclide < 0) {
@@ -1215,12 +1258,6 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
     S.lost[p] = 1;
     c.lost++;
   } else {
-    // micro cross sections of the sampled nuclide at this energy
-    const double lx = detm::log(E / P.energy_min);
-    int i_log = static_cast<int>(lx / P.log_spacing);
-    i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
-    int probes = 0;
-    Micro m = nuclide_xs(P, i_nuclide, E, i_log, lx, S.use_learned != 0, probes);
     c.lookups++;
     c.probes += probes;
     // absorption (src/physics.cpp:638-672), analog
@@ -1308,7 +1345,7 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
   return wgt != 0.0;
 }
 
-template<int NL, bool WW>
+template<int NL, bool WW, bool RECOMP>
 __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_COLL];
@@ -1317,7 +1354,7 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int p = S.q_sc[n_surf + i];
     int key = 0;
-    const bool push = collision_one<NL, WW>(P, S, p, c, key);
+    const bool push = collision_one<NL, WW, RECOMP>(P, S, p, c, key);
     S.stage_p[n_surf + i] = p;
     S.stage_key[n_surf + i] = push ? key : -1;
   }
@@ -1373,7 +1410,8 @@ __device__ __forceinline__ void xs_one(const DevProblem& P, const DevParticles& 
     const int kk = nn - jj < HG ? nn - jj : HG;
     Micro m[HG];
     nuclide_xs_group<HG, LEARNED, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
-    store_micro<HG>(micro + jj, m, kk);
+    if (!S.recompute_micro)
+      store_micro<HG>(micro + jj, m, kk);
 #pragma unroll
     for (int k = 0; k < HG; ++k) {
       if (k < kk) {
@@ -1420,7 +1458,12 @@ __global__ void __launch_bounds__(BLOCK) k_history(DevProblem P, DevParticles S,
         alive = false;
         break;
       }
-      alive = outcome == 0 ? surface_one<NL, WW>(P, S, p, cs, leak, key) : collision_one<NL, WW>(P, S, p, cc, key);
+      if (outcome == 0)
+        alive = surface_one<NL, WW>(P, S, p, cs, leak, key);
+      else if (S.recompute_micro)
+        alive = collision_one<NL, WW, true>(P, S, p, cc, key);
+      else
+        alive = collision_one<NL, WW, false>(P, S, p, cc, key);
       if (!alive)
         break;
     }
@@ -1623,10 +1666,14 @@ void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const Laun
       hipLaunchKernelGGL((k_surface<NL, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     break;
   default:
-    if (P.ww.on)
-      hipLaunchKernelGGL((k_collision<NL, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
+    if (P.ww.on && S.recompute_micro)
+      hipLaunchKernelGGL((k_collision<NL, true, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
+    else if (P.ww.on)
+      hipLaunchKernelGGL((k_collision<NL, true, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
+    else if (S.recompute_micro)
+      hipLaunchKernelGGL((k_collision<NL, false, true>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     else
-      hipLaunchKernelGGL((k_collision<NL, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
+      hipLaunchKernelGGL((k_collision<NL, false, false>), dim3(lc.blocks), dim3(tpb), 0, st, P, S);
     break;
   }
 }

@@ -144,7 +144,10 @@ def test_transport_graph_and_direct_launch_identical():
                                   {"sort": 0, "xs_group": 8}, {"history_below": 0},
                                   {"history_below": 1 << 30, "history_steps": 3, "cycles_per_check": 1},
                                   {"history_below": 200, "history_steps": 1}, {"fuse_advance": 0},
-                                  {"fuse_advance": 1, "wave_search": 1, "history_below": 0}])
+                                  {"fuse_advance": 1, "wave_search": 1, "history_below": 0},
+                                  {"recompute_micro": 1}, {"recompute_micro": 1, "fuse_advance": 1},
+                                  {"recompute_micro": 1, "history_below": 1 << 30, "history_steps": 3},
+                                  {"recompute_micro": 0}, {"recompute_micro": 0, "history_below": 0}])
 def test_transport_launch_shapes_identical(opts):
     """Queue sorting and lanes-per-particle change the order of work, never a history."""
     d, model, nucs = prepare(1, n_particles=1500, n_batches=1, grid_lo=2000, grid_hi=6000, n_nuclides=12,
@@ -234,14 +237,16 @@ def test_weight_windows_bank_overflow_is_an_error():
         gpu_run(d, options={"bank_cap": 2})
 
 
-@pytest.mark.parametrize("bank_cap,tail", [(64, 65536), (64, 0)], ids=["history-tail", "event-only"])
-def test_transport_weight_windows_histories(bank_cap, tail):
+@pytest.mark.parametrize("bank_cap,tail,recompute", [(64, 65536, -1), (64, 0, -1), (64, 65536, 0)],
+                         ids=["history-tail", "event-only", "micro-rows"])
+def test_transport_weight_windows_histories(bank_cap, tail, recompute):
     """Splitting, roulette and LIFO revival from the per-particle bank give the
     oracle's histories bit for bit (3-layer slab, 12-bin windows), with the
     history-based tail and with event kernels only."""
     d, model, nucs = prepare(4, **WW_SLAB)
     o = oracle_run(d, event_based=True)
-    g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap, "history_below": tail})
+    g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap, "history_below": tail,
+                            "recompute_micro": recompute})
     h_o = o.hashes(600)
     assert np.array_equal(g["hashes"], h_o), "%d of 600 histories differ" % np.sum(g["hashes"] != h_o)
     assert_tallies_close(g["results"], o.results())
