@@ -158,9 +158,13 @@ def roofline(info):
     # average algorithmic bytes per launch over the average launch duration
     achieved = by[dom] / (times[dom] * 1e-3) / 1e9
     traffic = traffic_per_launch(dom, st, launches[dom])
-    if fused and dom == "k_xs" and traffic is not None:
-        adv = traffic_per_launch("k_advance", st, launches[dom])
-        traffic = None if adv is None else round(traffic + adv, 1)
+    if fused and dom == "k_xs":
+        # the fused kernel's own profile, else the two kernels' profiles added
+        traffic = traffic_per_launch("k_xs_adv", st, launches[dom])
+        if traffic is None:
+            xs = traffic_per_launch("k_xs", st, launches[dom])
+            adv = traffic_per_launch("k_advance", st, launches[dom])
+            traffic = None if xs is None or adv is None else round(xs + adv, 1)
     per_kernel = {k: {"ms": times[k], "launches": launches[k], "avg_us": 1e3 * times[k] / max(launches[k], 1),
                       "GB_per_s": by[k] / max(times[k], 1e-9) * 1e-6} for k in times}
     total_bytes = sum(by.values())
@@ -337,6 +341,7 @@ def bench_problem_set(args):
 
 # rocprof kernel name -> (name in this script's kernel table, unit of work)
 KERNEL_UNITS = {"k_xs_lane": ("k_xs", "xs_event"), "k_xs": ("k_xs", "xs_event"),
+                "k_xs_adv": ("k_xs_adv", "xs_event"),
                 "k_advance": ("k_advance", "adv_event"), "k_surface": ("k_surface", "surf_event"),
                 "k_collision": ("k_collision", "coll_event")}
 

@@ -96,7 +96,9 @@ void openmc_set_stride(uint64_t new_stride);
  * (1/2/4/8 copies of each tally's VALUE, one per XCD), "recompute_micro"
  * (collisions repeat the nuclide lookups instead of reading per-particle
  * micro-total rows that calculate-xs would write: 1, 0, or -1 = when the
- * largest material has at most 64 nuclides), "progress_s" (a progress line on
+ * largest material has at most 64 nuclides), "fuse_advance" (advance in the
+ * calculate-xs launch: 1, 0, or -1 = when collisions repeat the lookups),
+ * "progress_s" (a progress line on
  * stderr every this many seconds of a batch, 0 = none). */
 int omc_set_option(const char* name, double value);
 /* Run statistics of the simulation so far (see omc_stat_names()). */
