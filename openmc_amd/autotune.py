@@ -10,7 +10,7 @@ history (every one of them is covered by the launch-shape parity tests):
   cycles_per_check  event cycles per host check of the queue counts (and per hipGraph)
   xs_group          nuclide lookups interleaved per lane in calculate-xs
   history_below     the queue threshold below which a batch finishes history-based
-  fuse_advance      advance in the calculate-xs launch (xs_group 4; -1: when
+  fuse_advance      advance in the calculate-xs launch (xs_group 2 or 4; -1: when
                     the collisions repeat the lookups)
   recompute_micro   collisions repeat the nuclide lookups instead of reading
                     the micro-total rows (-1: chosen by material size)
@@ -38,12 +38,12 @@ DEFAULT_SPACE = [
     ("sort_bits", [5, 3, 1, 0]),
     ("sort_every", [1, 2, 4]),
     ("cycles_per_check", [16, 8, 32, 64]),
-    ("xs_group", [4, 2, 8]),
+    ("xs_group", [-1, 2, 4, 8]),
     ("history_below", [65536, 0, 16384, 262144, 1048576]),
     ("fuse_advance", [-1, 0, 1]),
     ("recompute_micro", [-1, 0, 1]),
 ]
-DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": 5, "sort_every": 1, "cycles_per_check": 16, "xs_group": 4,
+DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": 5, "sort_every": 1, "cycles_per_check": 16, "xs_group": -1,
             "history_below": 65536, "fuse_advance": -1, "recompute_micro": -1}
 
 

@@ -943,10 +943,10 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
 
 // The fused kernel is held to three wavefronts per SIMD (168 VGPRs), the
 // occupancy of the calculate-xs kernel alone.
-template<int NL, bool LEARNED, int ADV>
+template<int NL, bool LEARNED, int ADV, int G>
 __global__ void __launch_bounds__(BLOCK, 3) k_xs_adv(DevProblem P, DevParticles S, const int* q)
 {
-  xs_lane_body<NL, 4, LEARNED, ADV, false>(P, S, q);
+  xs_lane_body<NL, G, LEARNED, ADV, false>(P, S, q);
 }
 
 //==============================================================================
@@ -1563,20 +1563,29 @@ void xs_lane_l(const DevProblem& P, const DevParticles& S, const int* q, int blo
 }
 
 // calculate-xs and advance in one launch (xs_group 4, one lane per particle)
-template<int NL>
-void xs_adv(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, hipStream_t st)
+template<int NL, int G>
+void xs_adv_g(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, hipStream_t st)
 {
   if (P.n_parts > 1) {
     if (S.use_learned)
-      hipLaunchKernelGGL((k_xs_adv<NL, true, 2>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+      hipLaunchKernelGGL((k_xs_adv<NL, true, 2, G>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
     else
-      hipLaunchKernelGGL((k_xs_adv<NL, false, 2>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+      hipLaunchKernelGGL((k_xs_adv<NL, false, 2, G>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
   } else {
     if (S.use_learned)
-      hipLaunchKernelGGL((k_xs_adv<NL, true, 1>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+      hipLaunchKernelGGL((k_xs_adv<NL, true, 1, G>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
     else
-      hipLaunchKernelGGL((k_xs_adv<NL, false, 1>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
+      hipLaunchKernelGGL((k_xs_adv<NL, false, 1, G>), dim3(blocks), dim3(tpb), 0, st, P, S, q);
   }
+}
+
+template<int NL>
+void xs_adv(const DevProblem& P, const DevParticles& S, const int* q, int blocks, int tpb, int G, hipStream_t st)
+{
+  if (G == 2)
+    xs_adv_g<NL, 2>(P, S, q, blocks, tpb, st);
+  else
+    xs_adv_g<NL, 4>(P, S, q, blocks, tpb, st);
 }
 
 template<int NL>
@@ -1619,7 +1628,7 @@ void launch_partition(const DevParticles& S, const Part& A, const LaunchConfig& 
 
 bool fused_advance(const LaunchConfig& lc)
 {
-  return lc.fuse_advance && lc.xs_lanes == 1 && lc.xs_group == 4;
+  return lc.fuse_advance && lc.xs_lanes == 1 && (lc.xs_group == 4 || lc.xs_group == 2);
 }
 
 template<int NL>
@@ -1640,7 +1649,7 @@ void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const Laun
   }
   case KERNEL_XS:
     if (fused_advance(lc))
-      xs_adv<NL>(P, S, S.q_xs, lc.blocks, tpb, st);
+      xs_adv<NL>(P, S, S.q_xs, lc.blocks, tpb, lc.xs_group, st);
     else
       xs_lpp<NL>(P, S, S.q_xs, lc, st);
     break;

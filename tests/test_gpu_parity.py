@@ -147,7 +147,9 @@ def test_transport_graph_and_direct_launch_identical():
                                   {"fuse_advance": 1, "wave_search": 1, "history_below": 0},
                                   {"recompute_micro": 1}, {"recompute_micro": 1, "fuse_advance": 1},
                                   {"recompute_micro": 1, "history_below": 1 << 30, "history_steps": 3},
-                                  {"recompute_micro": 0}, {"recompute_micro": 0, "history_below": 0}])
+                                  {"recompute_micro": 0}, {"recompute_micro": 0, "history_below": 0},
+                                  {"fuse_advance": 1, "xs_group": 2}, {"fuse_advance": 1, "xs_group": 2,
+                                                                       "recompute_micro": 0}])
 def test_transport_launch_shapes_identical(opts):
     """Queue sorting and lanes-per-particle change the order of work, never a history."""
     d, model, nucs = prepare(1, n_particles=1500, n_batches=1, grid_lo=2000, grid_hi=6000, n_nuclides=12,
