@@ -45,6 +45,18 @@ extern int n_coord_levels;
 extern bool DAGMC_ENABLED;
 extern bool LIBMESH_ENABLED;
 extern bool UWUW_ENABLED;
+/* openmc/lib/settings.py:27-40, 62: the reference's defaults
+ * (src/settings.cpp); weight_windows_on reflects the model and
+ * path_statepoint_c names the last statepoint written. */
+extern bool cmfd_run;
+extern bool entropy_on;
+extern int32_t max_lost_particles;
+extern bool need_depletion_rx;
+extern bool output_summary;
+extern double rel_max_lost_particles;
+extern bool restart_run;
+extern bool weight_windows_on;
+extern char* path_statepoint_c;
 
 /* include/openmc/capi.h:83 -- reads model.xml from the cwd (or argv's path);
  * "-e" selects event-based transport; the GPU path runs either way. */
@@ -80,6 +92,210 @@ int64_t openmc_get_seed(void);
 void openmc_set_seed(int64_t new_seed);
 uint64_t openmc_get_stride(void);
 void openmc_set_stride(uint64_t new_stride);
+
+/* The rest of include/openmc/capi.h that the modules of openmc/lib bind at import
+ * (implemented in capi_model.cpp).  Getters answer from the model loaded by
+ * openmc_init with the reference's error codes; functions that would change
+ * the model after its upload to the GPU, or that belong to subsystems outside
+ * the fixed-source path (volumes, plotting, CMFD, restart, DAGMC,
+ * eigenvalue), return OPENMC_E_UNASSIGNED with a message.  Typed getters of
+ * filter types the path does not have return OPENMC_E_INVALID_TYPE, as the
+ * reference does for a filter of another type. */
+/* sizes: src/cell.cpp:1820, src/material.cpp:1599, src/mesh.cpp:3900,
+ * src/nuclide.cpp:1105, src/tallies/filter.cpp:62 */
+int cells_size(void);
+size_t n_materials(void);
+int n_meshes(void);
+size_t nuclides_size(void);
+size_t tally_filters_size(void);
+bool openmc_master(void);                                   /* src/message_passing.cpp:16 */
+int openmc_get_n_batches(int* n_batches, bool get_max_batches); /* src/settings.cpp:1319 */
+int openmc_set_n_batches(int32_t n_batches, bool set_max_batches, bool add_statepoint_batch); /* :1285 */
+/* nuclides: include/openmc/capi.h:92,108,118; src/nuclide.cpp:1205 */
+int openmc_get_nuclide_index(const char* name, int* index);
+int openmc_nuclide_name(int index, const char** name);
+int openmc_load_nuclide(const char* name, const double* temps, int n);
+int openmc_nuclide_collapse_rate(int index, int MT, double temperature, const double* energy, const double* flux,
+  int n, double* xs);
+/* materials: include/openmc/capi.h:88,110-127; src/material.cpp:1443 */
+int openmc_get_material_index(int32_t id, int32_t* index);
+int openmc_material_get_id(int32_t index, int32_t* id);
+int openmc_material_set_id(int32_t index, int32_t id);
+int openmc_material_get_name(int32_t index, const char** name);
+int openmc_material_set_name(int32_t index, const char* name);
+int openmc_material_get_densities(int32_t index, const int** nuclides, const double** densities, int* n);
+int openmc_material_get_density(int32_t index, double* density);
+int openmc_material_get_temperature(int32_t index, double* temperature);
+int openmc_material_get_volume(int32_t index, double* volume);
+int openmc_material_set_volume(int32_t index, double volume);
+int openmc_material_get_depletable(int32_t index, bool* depletable);
+int openmc_material_set_depletable(int32_t index, bool depletable);
+int openmc_material_get_fissionable(int32_t index, bool* fissionable);
+int openmc_material_set_density(int32_t index, double density, const char* units);
+int openmc_material_set_densities(int32_t index, int n, const char** name, const double* density);
+int openmc_material_add_nuclide(int32_t index, const char* name, double density);
+int openmc_extend_materials(int32_t n, int32_t* index_start, int32_t* index_end);
+/* cells: include/openmc/capi.h:13-35,62-65,73 */
+int openmc_get_cell_index(int32_t id, int32_t* index);
+int openmc_cell_get_id(int32_t index, int32_t* id);
+int openmc_cell_set_id(int32_t index, int32_t id);
+int openmc_cell_get_name(int32_t index, const char** name);
+int openmc_cell_set_name(int32_t index, const char* name);
+int openmc_cell_get_fill(int32_t index, int* type, int32_t** indices, int32_t* n);
+int openmc_cell_get_num_instances(int32_t index, int32_t* num_instances);
+int openmc_cell_get_temperature(int32_t index, const int32_t* instance, double* T);
+int openmc_cell_get_density(int32_t index, const int32_t* instance, double* rho);
+int openmc_cell_get_translation(int32_t index, double xyz[]);
+int openmc_cell_get_rotation(int32_t index, double rot[], size_t* n);
+int openmc_cell_set_fill(int32_t index, int type, int32_t n, const int32_t* indices);
+int openmc_cell_set_temperature(int32_t index, double T, const int32_t* instance, bool set_contained);
+int openmc_cell_set_density(int32_t index, double rho, const int32_t* instance, bool set_contained);
+int openmc_cell_set_translation(int32_t index, const double xyz[]);
+int openmc_cell_set_rotation(int32_t index, const double rot[], size_t rot_len);
+int openmc_cell_bounding_box(const int32_t index, double* llc, double* urc);
+int openmc_global_bounding_box(double* llc, double* urc);
+int openmc_find_cell(const double* xyz, int32_t* index, int32_t* instance);
+int openmc_extend_cells(int32_t n, int32_t* index_start, int32_t* index_end);
+int openmc_dagmc_universe_get_cell_ids(int32_t univ_id, int32_t* ids, size_t* n);
+int openmc_dagmc_universe_get_num_cells(int32_t univ_id, size_t* n);
+/* meshes: include/openmc/capi.h:89,130-150; src/mesh.cpp:2285,2410,2450,2670-2700 */
+int openmc_get_mesh_index(int32_t id, int32_t* index);
+int openmc_mesh_get_id(int32_t index, int32_t* id);
+int openmc_mesh_set_id(int32_t index, int32_t id);
+int openmc_mesh_get_type(int32_t index, char* type);
+int openmc_mesh_get_n_elements(int32_t index, size_t* n);
+int openmc_mesh_get_volumes(int32_t index, double* volumes);
+int openmc_mesh_bounding_box(int32_t index, double* ll, double* ur);
+int openmc_regular_mesh_get_dimension(int32_t index, int** dims, int* n);
+int openmc_regular_mesh_get_params(int32_t index, double** ll, double** ur, double** width, int* n);
+int openmc_regular_mesh_set_dimension(int32_t index, int n, const int* dims);
+int openmc_regular_mesh_set_params(int32_t index, int n, const double* ll, const double* ur, const double* width);
+int openmc_rectilinear_mesh_get_grid(int32_t index, double** grid_x, int* nx, double** grid_y, int* ny,
+  double** grid_z, int* nz);
+int openmc_rectilinear_mesh_set_grid(int32_t index, const double* grid_x, const int nx, const double* grid_y,
+  const int ny, const double* grid_z, const int nz);
+int openmc_cylindrical_mesh_get_grid(int32_t index, double** grid_x, int* nx, double** grid_y, int* ny,
+  double** grid_z, int* nz);
+int openmc_cylindrical_mesh_set_grid(int32_t index, const double* grid_x, const int nx, const double* grid_y,
+  const int ny, const double* grid_z, const int nz);
+int openmc_spherical_mesh_get_grid(int32_t index, double** grid_x, int* nx, double** grid_y, int* ny,
+  double** grid_z, int* nz);
+int openmc_spherical_mesh_set_grid(int32_t index, const double* grid_x, const int nx, const double* grid_y,
+  const int ny, const double* grid_z, const int nz);
+int openmc_mesh_get_plot_bins(int32_t index, const void* origin, const void* width, int basis, int* pixels,
+  int32_t* data);
+int openmc_mesh_material_volumes(int32_t index, int nx, int ny, int nz, int max_mats, int32_t* materials,
+  double* volumes);
+int openmc_extend_meshes(int32_t n, const char* type, int32_t* index_start, int32_t* index_end);
+/* filters: include/openmc/capi.h:36-47,56-59,86,103-106,128-134,151-159,
+ * 244-249; src/tallies/filter_mesh.cpp:253-275, filter_meshborn.cpp:37-55,
+ * filter_meshsurface.cpp:103-109 */
+int openmc_get_filter_index(int32_t id, int32_t* index);
+int openmc_filter_get_id(int32_t index, int32_t* id);
+int openmc_filter_set_id(int32_t index, int32_t id);
+int openmc_filter_get_type(int32_t index, char* type);
+int openmc_filter_get_num_bins(int32_t index, int* n_bins);
+int openmc_new_filter(const char* type, int32_t* index);
+int openmc_mesh_filter_get_mesh(int32_t index, int32_t* index_mesh);
+int openmc_mesh_filter_set_mesh(int32_t index, int32_t index_mesh);
+int openmc_mesh_filter_get_translation(int32_t index, double translation[3]);
+int openmc_mesh_filter_set_translation(int32_t index, double translation[3]);
+int openmc_mesh_filter_get_rotation(int32_t index, double rot[], size_t* n);
+int openmc_mesh_filter_set_rotation(int32_t index, const double rot[], size_t rot_len);
+int openmc_energy_filter_get_bins(int32_t index, const double** energies, size_t* n);
+int openmc_energy_filter_set_bins(int32_t index, size_t n, const double* energies);
+int openmc_cell_filter_get_bins(int32_t index, const int32_t** cells, int32_t* n);
+int openmc_material_filter_get_bins(int32_t index, const int32_t** bins, size_t* n);
+int openmc_material_filter_set_bins(int32_t index, size_t n, const int32_t* bins);
+int openmc_particle_filter_get_bins(int32_t index, int bins[]);
+int openmc_energyfunc_filter_get_energy(int32_t index, size_t* n, const double** energy);
+int openmc_energyfunc_filter_get_y(int32_t index, size_t* n, const double** y);
+int openmc_energyfunc_filter_set_data(int32_t index, size_t n, const double* energies, const double* y);
+int openmc_energyfunc_filter_get_interpolation(int32_t index, int* interp);
+int openmc_energyfunc_filter_set_interpolation(int32_t index, const char* interp);
+int openmc_legendre_filter_get_order(int32_t index, int* order);
+int openmc_legendre_filter_set_order(int32_t index, int order);
+int openmc_spatial_legendre_filter_get_order(int32_t index, int* order);
+int openmc_spatial_legendre_filter_set_order(int32_t index, int order);
+int openmc_sphharm_filter_get_order(int32_t index, int* order);
+int openmc_sphharm_filter_set_order(int32_t index, int order);
+int openmc_zernike_filter_get_order(int32_t index, int* order);
+int openmc_zernike_filter_set_order(int32_t index, int order);
+int openmc_meshborn_filter_get_mesh(int32_t index, int32_t* index_mesh);
+int openmc_meshborn_filter_set_mesh(int32_t index, int32_t index_mesh);
+int openmc_meshborn_filter_get_translation(int32_t index, double translation[3]);
+int openmc_meshborn_filter_set_translation(int32_t index, double translation[3]);
+int openmc_meshsurface_filter_get_mesh(int32_t index, int32_t* index_mesh);
+int openmc_meshsurface_filter_set_mesh(int32_t index, int32_t index_mesh);
+int openmc_meshsurface_filter_get_translation(int32_t index, double translation[3]);
+int openmc_meshsurface_filter_set_translation(int32_t index, double translation[3]);
+/* tallies: include/openmc/capi.h:52,133,164-185 */
+int openmc_tally_get_filters(int32_t index, const int32_t** indices, size_t* n);
+int openmc_tally_get_scores(int32_t index, int** scores, int* n);
+int openmc_tally_get_nuclides(int32_t index, int** nuclides, int* n);
+int openmc_tally_get_type(int32_t index, int32_t* type);
+int openmc_tally_get_writable(int32_t index, bool* writable);
+int openmc_tally_get_multiply_density(int32_t index, bool* value);
+int openmc_tally_reset(int32_t index);
+int openmc_tally_set_id(int32_t index, int32_t id);
+int openmc_tally_set_active(int32_t index, bool active);
+int openmc_tally_set_writable(int32_t index, bool writable);
+int openmc_tally_set_multiply_density(int32_t index, bool value);
+int openmc_tally_set_estimator(int32_t index, const char* estimator);
+int openmc_tally_set_type(int32_t index, const char* type);
+int openmc_tally_set_filters(int32_t index, size_t n, const int32_t* indices);
+int openmc_tally_set_scores(int32_t index, int n, const char** scores);
+int openmc_tally_set_nuclides(int32_t index, int n, const char** nuclides);
+int openmc_extend_tallies(int32_t n, int32_t* index_start, int32_t* index_end);
+int openmc_remove_tally(int32_t index);
+/* weight windows: include/openmc/capi.h:186-236 (one set, index 0, id 1) */
+size_t openmc_weight_windows_size(void);
+int openmc_get_weight_windows_index(int32_t id, int32_t* idx);
+int openmc_weight_windows_get_id(int32_t index, int32_t* id);
+int openmc_weight_windows_get_mesh(int32_t index, int32_t* mesh_idx);
+int openmc_weight_windows_get_energy_bounds(int32_t index, const double** e_bounds, size_t* e_bounds_size);
+int openmc_weight_windows_get_particle(int32_t index, int* particle);
+int openmc_weight_windows_get_bounds(int32_t index, const double** lower_bounds, const double** upper_bounds,
+  size_t* size);
+int openmc_weight_windows_get_survival_ratio(int32_t index, double* ratio);
+int openmc_weight_windows_get_max_lower_bound_ratio(int32_t index, double* lb_ratio);
+int openmc_weight_windows_get_weight_cutoff(int32_t index, double* cutoff);
+int openmc_weight_windows_get_max_split(int32_t index, int* max_split);
+int openmc_weight_windows_set_id(int32_t index, int32_t id);
+int openmc_weight_windows_set_mesh(int32_t index, int32_t mesh_idx);
+int openmc_weight_windows_set_energy_bounds(int32_t index, double* e_bounds, size_t e_bounds_size);
+int openmc_weight_windows_set_particle(int32_t index, int particle);
+int openmc_weight_windows_set_bounds(int32_t index, const double* lower_bounds, const double* upper_bounds,
+  size_t size);
+int openmc_weight_windows_set_survival_ratio(int32_t index, double ratio);
+int openmc_weight_windows_set_max_lower_bound_ratio(int32_t index, double lb_ratio);
+int openmc_weight_windows_set_weight_cutoff(int32_t index, double cutoff);
+int openmc_weight_windows_set_max_split(int32_t index, int max_split);
+int openmc_weight_windows_update_magic(int32_t ww_idx, int32_t tally_idx, const char* value, double threshold,
+  double ratio);
+int openmc_extend_weight_windows(int32_t n, int32_t* index_start, int32_t* index_end);
+int openmc_weight_windows_export(const char* filename);
+int openmc_weight_windows_import(const char* filename);
+/* outside the path: include/openmc/capi.h:12,71,72,79,137-139,143,147,158,
+ * 250-276 */
+int openmc_calculate_volumes(void);
+int openmc_plot_geometry(void);
+int openmc_id_map(const void* slice, int32_t* data_out);
+int openmc_property_map(const void* slice, double* data_out);
+int openmc_properties_export(const char* filename);
+int openmc_properties_import(const char* filename);
+int openmc_get_keff(double k_combined[]);
+int openmc_source_bank(void** ptr, int64_t* n);
+int openmc_sample_external_source(size_t n, uint64_t* seed, void* sites);
+int openmc_statepoint_load(const char* filename);
+void openmc_cmfd_reweight(const bool feedback, const double* cmfd_src);
+void openmc_initialize_mesh_egrid(const int meshtally_id, const int* cmfd_indices, const double norm);
+void openmc_initialize_linsolver(const int* indptr, int len_indptr, const int* indices, int n_elements, int dim,
+  double spectral, const int* map, bool use_all_threads);
+int openmc_run_linsolver(const double* A_data, const double* b, double* x, double tol);
+/* Zernike polynomials (include/openmc/math_functions.h:94,117) */
+void calc_zn(int n, double rho, double phi, double zn[]);
+void calc_zn_rad(int n, double rho, double zn_rad[]);
 
 /* Build-specific (no counterpart in the reference) ---------------------- */
 
@@ -173,16 +389,22 @@ int omc_problem_global_tallies(int problem, double** ptr);
  * per-problem offsets (layout in capi.cpp). */
 int omc_check_problem_set(int n, const char* const* dirs, double* out, int n_out);
 /* Learned index: fit the per-nuclide piecewise-linear models on the GPU with
- * n_seg segments and `epochs` gradient steps; *max_err gets the largest
- * recorded error bound over all nuclides. */
+ * n_seg segments and `epochs` gradient steps; a nuclide whose recorded index
+ * error exceeds the bound (option "learned_err_bound", default and maximum
+ * 64) anywhere is fitted again with twice as many segments until it is within
+ * it.  *max_err gets the largest recorded error bound over all nuclides. */
 int omc_learned_build(int n_seg, int epochs, int* max_err);
 /* Loss and gradient of one nuclide's model at given FP32 parameters
  * (params[n_seg][2]); loss[n_seg], grad[n_seg][2].  For gradient checks. */
 int omc_learned_loss_grad(int i_nuc, int n_seg, const float* params, double* loss, double* grad);
-/* Training summary: n_seg, max error bound, mean error bound, loss before the
- * first gradient step, after the last one, after recentring. */
+/* Training summary: base n_seg, max error bound, mean error bound, loss before
+ * the first gradient step, after the last one, after recentring. */
 int omc_learned_stats(double* out6);
-/* The trained model of one nuclide: params[n_seg][2], err[n_seg]; returns n_seg. */
+/* More of it: training rounds, the bound enforced, total segments, fewest and
+ * most segments of a nuclide. */
+int omc_learned_stats2(double* out5);
+/* The trained model of one nuclide: returns its segment count n; with params
+ * and err not null fills params[n][2] and err[n]. */
 int omc_learned_model(int i_nuc, float* params, int* err);
 
 #ifdef __cplusplus

@@ -22,7 +22,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["xml.cpp", "host_model.cpp", "hdf5_io.cpp", "capi.cpp", "transport.hip", "xs_lookup.hip", "learned.hip"]
+SOURCES = ["xml.cpp", "host_model.cpp", "hdf5_io.cpp", "capi.cpp", "capi_model.cpp", "transport.hip", "xs_lookup.hip", "learned.hip"]
 
 
 def _run(cmd):

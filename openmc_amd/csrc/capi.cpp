@@ -18,6 +18,7 @@
 #include <unistd.h>
 #include <vector>
 
+#include "capi_internal.h"
 #include "hdf5_io.h"
 #include "host_model.h"
 #include "learned.h"
@@ -115,6 +116,7 @@ struct Sim {
   void* parts_alloc = nullptr;
   bool results_dirty = false;
   int n_real = 0;
+  std::vector<int> tally_real_base; // n_real when each tally was last reset on its own
   std::vector<double> gt = std::vector<double>(3 * N_GLOBAL, 0.0);
   std::vector<std::vector<double>> gt_part; // global tallies of each problem of a problem set
   double total_weight = 0;
@@ -125,6 +127,7 @@ struct Sim {
   // options
   bool trace_hash = false;
   bool use_learned = false;
+  int learned_err_bound = LEARNED_MAX_ERR; // largest index error a learned segment may record
   bool event_timing = false;
   bool use_graph = true;
   int cycles_per_check = 16;
@@ -221,6 +224,8 @@ void upload_matnuc(Sim& s)
     r.dens = m.mat_dens[j];
     r.u_lo = lrn ? s.learned.h_ulo[k] : 0.0;
     r.inv_w = lrn ? s.learned.h_invw[k] : 0.0;
+    r.seg_off = lrn ? s.learned.h_segoff[k] : 0;
+    r.seg_n = lrn ? s.learned.h_segn[k] : 1;
   }
   if (s.matnuc_alloc) {
     (void)hipFree(s.matnuc_alloc);
@@ -896,6 +901,7 @@ void write_statepoint(Sim& s, int part, const std::string& filename, bool write_
   const HostPart& hp = m.parts.at(static_cast<size_t>(part));
   sync_results(s);
   H5Writer w(filename);
+  api_note_statepoint(filename);
   const int64_t f = w.root();
   w.attr_str(f, "filetype", "statepoint");
   const int version[2] = {18, 1};     // VERSION_STATEPOINT (include/openmc/constants.h:28)
@@ -1073,6 +1079,7 @@ void finish_init(Sim& s)
   upload_problem(s);
   if (s.m.event_based)
     event_based = true;
+  weight_windows_on = s.m.ww.on;
   n_particles = s.m.n_particles;
   n_inactive = s.m.n_inactive;
   n_coord_levels = s.m.n_levels;
@@ -1112,6 +1119,36 @@ hipStream_t stream_state()
 const HostModel& model_state()
 {
   return g->m;
+}
+
+HostModel* api_model()
+{
+  return g && g->initialized ? &g->m : nullptr;
+}
+
+bool api_simulation_running()
+{
+  return g && g->sim_initialized;
+}
+
+void api_set_errmsg(const std::string& msg)
+{
+  set_errmsg(msg);
+}
+
+void api_tally_reset(int index)
+{
+  if (!g || !g->initialized || index < 0 || index >= static_cast<int>(g->m.tallies.size()))
+    throw std::runtime_error("Index in tallies array is out of bounds.");
+  if (g->h_results.empty())
+    return;
+  sync_results(*g);
+  const HostTally& t = g->m.tallies[index];
+  const size_t off = 3 * static_cast<size_t>(t.off), n = 3 * static_cast<size_t>(t.n_filter_bins);
+  hip_check(hipMemset(g->d_results + off, 0, sizeof(double) * n), "hipMemset");
+  std::fill(g->h_results.begin() + off, g->h_results.begin() + off + n, 0.0);
+  g->tally_real_base.resize(g->m.tallies.size(), 0);
+  g->tally_real_base[index] = g->n_real;
 }
 
 } // namespace omc
@@ -1195,7 +1232,13 @@ int openmc_simulation_init(void)
       g->P.n_particles = n_particles;
     }
     choose_capacity(*g);
+    // src/simulation.cpp:120-126: batch counter back to zero and
+    // openmc_reset(), so a second run under one openmc_init starts from
+    // empty SUM/SUM_SQ and zero realizations
     current_batch = 0;
+    int err = openmc_reset();
+    if (err)
+      return err;
     g->sim_initialized = true;
     return 0;
   });
@@ -1264,6 +1307,7 @@ int openmc_reset(void)
     for (auto& v : g->gt_part)
       std::fill(v.begin(), v.end(), 0.0);
     g->n_real = 0;
+    g->tally_real_base.assign(g->m.tallies.size(), 0);
     n_realizations = 0;
     g->results_dirty = false;
     return 0;
@@ -1355,7 +1399,7 @@ int openmc_tally_get_n_realizations(int32_t index, int32_t* n)
     set_errmsg("Index in tallies array is out of bounds.");
     return OPENMC_E_OUT_OF_BOUNDS;
   }
-  *n = g->n_real;
+  *n = g->n_real - (index < static_cast<int32_t>(g->tally_real_base.size()) ? g->tally_real_base[index] : 0);
   return 0;
 }
 
@@ -1365,7 +1409,8 @@ int openmc_tally_get_estimator(int32_t index, int* estimator)
     set_errmsg("Index in tallies array is out of bounds.");
     return OPENMC_E_OUT_OF_BOUNDS;
   }
-  *estimator = 2; // TallyEstimator::TRACKLENGTH (include/openmc/constants.h)
+  // TallyEstimator {ANALOG, TRACKLENGTH, COLLISION} (include/openmc/constants.h:299)
+  *estimator = 1;
   return 0;
 }
 
@@ -1489,6 +1534,11 @@ int omc_set_option(const char* name, double value)
     g->trace_hash = value != 0;
   else if (n == "learned_index")
     g->use_learned = value != 0;
+  else if (n == "learned_err_bound") {
+    if (value < 0 || value > LEARNED_MAX_ERR)
+      throw std::runtime_error("learned_err_bound must be in [0, " + std::to_string(LEARNED_MAX_ERR) + "]");
+    g->learned_err_bound = static_cast<int>(value);
+  }
   else if (n == "event_timing")
     g->event_timing = value != 0;
   else if (n == "use_graph")
@@ -1977,7 +2027,7 @@ int omc_learned_build(int n_seg, int epochs, int* max_err)
   if (!g || !g->initialized)
     return OPENMC_E_ALLOCATE;
   return guarded([&]() -> int {
-    int err = build_learned_index(g->learned, g->m, g->P, n_seg, epochs, g->stream);
+    int err = build_learned_index(g->learned, g->m, g->P, n_seg, epochs, g->stream, g->learned_err_bound);
     upload_matnuc(*g);
     if (max_err)
       *max_err = err;

@@ -99,7 +99,9 @@ struct alignas(64) DevMatNuc {
   double e_last;
   double dens;  // atom density (material order)
   double u_lo;  // learned index: log(E_first / E_min) (0 until built)
-  double inv_w; // learned index: n_seg / (log(E_last / E_min) - u_lo)
+  double inv_w; // learned index: seg_n / (log(E_last / E_min) - u_lo)
+  int seg_off;  // learned index: first segment of the nuclide in DevLearned::rec
+  int seg_n;    // learned index: segments of the nuclide
 };
 static_assert(sizeof(DevMatNuc) == 64, "one 64-byte record per material nuclide");
 
@@ -112,13 +114,18 @@ struct alignas(16) LSeg {
 };
 
 // Learned grid index: piecewise-linear model of i(log E) per nuclide.
+// Nuclide n owns segments [seg_off[n], seg_off[n] + seg_n[n]); seg_n[n] is
+// the base count or a power-of-two multiple of it, as many as keep every
+// recorded error bound within the limit the build enforced.
 struct DevLearned {
-  const LSeg* rec;     // [n_nuc][n_seg] the lookup's copy of seg and err
-  const float* seg;    // [n_nuc][n_seg][2] slope, intercept (i ~= slope*u + icpt)
-  const int* err;      // [n_nuc][n_seg] max |i_true - i_pred| over the segment's grid points
-  const double* u_lo;  // [n_nuc] log(E_first) of each nuclide grid
-  const double* inv_w; // [n_nuc] n_seg / (log(E_last) - log(E_first))
-  int n_seg;
+  const LSeg* rec;     // [all segments] the lookup's copy of seg and err
+  const float* seg;    // [all segments][2] slope, intercept (i ~= slope*f + icpt)
+  const int* err;      // [all segments] max |i_true - i_pred| over the segment's grid points
+  const double* u_lo;  // [n_nuc] log(E_first / E_min) of each nuclide grid
+  const double* inv_w; // [n_nuc] seg_n / (log(E_last / E_min) - u_lo)
+  const int* seg_off;  // [n_nuc]
+  const int* seg_n;    // [n_nuc]
+  int n_seg;           // base segment count
   int enabled;
 };
 

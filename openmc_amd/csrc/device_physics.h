@@ -655,9 +655,10 @@ __device__ __forceinline__ int learned_lower_bound(const DevProblem& P, int i_nu
 {
   const DevLearned& Lm = P.learned;
   double uu = (lx - Lm.u_lo[i_nuc]) * Lm.inv_w[i_nuc];
+  const int ns = Lm.seg_n[i_nuc];
   int s = uu < 0.0 ? 0 : static_cast<int>(uu);
-  s = s >= Lm.n_seg ? Lm.n_seg - 1 : s;
-  const int64_t k = static_cast<int64_t>(i_nuc) * Lm.n_seg + s;
+  s = s >= ns ? ns - 1 : s;
+  const int64_t k = static_cast<int64_t>(Lm.seg_off[i_nuc]) + s;
   const float2 sg = reinterpret_cast<const float2*>(Lm.seg)[k];
   const int err = Lm.err[k];
   float fr = static_cast<float>(uu - static_cast<double>(s));
@@ -826,9 +827,10 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
     for (int k = 0; k < K; ++k) {
       const DevMatNuc* r = mn + (k < kk ? k : 0);
       uu[k] = (logE - uni<UNI>(r->u_lo)) * uni<UNI>(r->inv_w);
+      const int ns = uni<UNI>(r->seg_n);
       int sgi = uu[k] < 0.0 ? 0 : static_cast<int>(uu[k]);
-      seg[k] = sgi >= Lm.n_seg ? Lm.n_seg - 1 : sgi;
-      sg[k] = Lm.rec[static_cast<int64_t>(nuc[k]) * Lm.n_seg + seg[k]];
+      seg[k] = sgi >= ns ? ns - 1 : sgi;
+      sg[k] = Lm.rec[uni<UNI>(r->seg_off) + seg[k]];
     }
     int js[K];
 #pragma unroll

@@ -415,6 +415,7 @@ void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_
   for (const XmlNode* mn : mats.children_named("material")) {
     RawMat r {};
     m.mat_ids.push_back(std::stoi(mn->value("id")));
+    m.mat_names.push_back(mn->attr("name") ? *mn->attr("name") : std::string());
     const XmlNode* dn = mn->child("density");
     if (!dn)
       throw std::runtime_error("material without <density>");
@@ -605,6 +606,7 @@ void read_geometry(HostModel& m, const XmlNode& g)
   for (const XmlNode* cn : g.children_named("cell")) {
     DevCell c {};
     c.id = std::stoi(cn->value("id"));
+    m.cell_names.push_back(cn->attr("name") ? *cn->attr("name") : std::string());
     int uid = cn->has("universe") ? std::stoi(cn->value("universe")) : 0;
     auto it = univ_map.find(uid);
     if (it == univ_map.end()) {
@@ -1035,6 +1037,7 @@ void read_problem_set(HostModel& out, const std::vector<std::string>& dirs)
       out.mat_off.push_back(0);
     for (size_t i = 0; i < m.mat_ids.size(); ++i) {
       out.mat_ids.push_back(m.mat_ids[i]);
+      out.mat_names.push_back(m.mat_names[i]);
       for (int e = m.mat_off[i]; e < m.mat_off[i + 1]; ++e) {
         out.mat_nuc.push_back(nuc_map[m.mat_nuc[e]]);
         out.mat_nuc_local.push_back(m.mat_nuc[e]);
@@ -1072,6 +1075,7 @@ void read_problem_set(HostModel& out, const std::vector<std::string>& dirs)
       c.part = static_cast<int>(k);
       out.cells.push_back(c);
     }
+    out.cell_names.insert(out.cell_names.end(), m.cell_names.begin(), m.cell_names.end());
     out.univ_ids.insert(out.univ_ids.end(), m.univ_ids.begin(), m.univ_ids.end());
     for (DevUniverse u : m.universes) {
       u.cell_begin += ucell0;

@@ -123,6 +123,7 @@ struct HostModel {
   std::unordered_map<std::string, int> nuclide_map;
 
   std::vector<int> mat_ids;
+  std::vector<std::string> mat_names; // the name attribute ("" when absent)
   std::vector<int> mat_off;
   std::vector<int> mat_nuc;
   std::vector<double> mat_dens;
@@ -130,6 +131,7 @@ struct HostModel {
   std::vector<int> surf_ids;
   std::vector<DevSurface> surfaces;
   std::vector<DevCell> cells;
+  std::vector<std::string> cell_names;
   std::vector<int> tokens;
   std::vector<int> univ_ids;
   std::vector<DevUniverse> universes;

@@ -19,6 +19,8 @@
 // which contains the global lower_bound of any E that maps to segment s.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -59,10 +61,14 @@ __device__ __forceinline__ int seg_of(double uu, int n_seg)
   return s >= n_seg ? n_seg - 1 : s;
 }
 
-// uu_i for every grid point of every nuclide: grid (x over points, y = nuclide)
-__global__ void k_lprep(DevProblem P, const double* u_lo, const double* inv_w, double* uu)
+// Every kernel below works on the nuclides list[0, gridDim.y) of one training
+// round: blockIdx.y is the position y in that list, per-round arrays are
+// indexed by y, per-nuclide ones (u_lo, inv_w, grid offsets) by list[y].
+
+// uu_i for every grid point of the listed nuclides: grid (x over points, y)
+__global__ void k_lprep(DevProblem P, const int* list, const double* u_lo, const double* inv_w, double* uu)
 {
-  const int n = blockIdx.y;
+  const int n = list[blockIdx.y];
   const DevNuclide nd = P.nuc[n];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd.n; i += gridDim.x * blockDim.x) {
     double lx = detm::log(P.E[nd.off + i] / P.energy_min);
@@ -71,9 +77,9 @@ __global__ void k_lprep(DevProblem P, const double* u_lo, const double* inv_w, d
 }
 
 // seg_start[n][s] = first point whose segment >= s, s in [0, n_seg]
-__global__ void k_lsegbounds(DevProblem P, const double* uu, int n_seg, int* seg_start)
+__global__ void k_lsegbounds(DevProblem P, const int* list, const double* uu, int n_seg, int* seg_start)
 {
-  const int n = blockIdx.y;
+  const int n = list[blockIdx.y];
   const DevNuclide nd = P.nuc[n];
   const double* u = uu + nd.off;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s <= n_seg; s += gridDim.x * blockDim.x) {
@@ -88,7 +94,7 @@ __global__ void k_lsegbounds(DevProblem P, const double* uu, int n_seg, int* seg
         cnt = step;
       }
     }
-    seg_start[static_cast<int64_t>(n) * (n_seg + 1) + s] = (s == n_seg) ? nd.n : lo;
+    seg_start[static_cast<int64_t>(blockIdx.y) * (n_seg + 1) + s] = (s == n_seg) ? nd.n : lo;
   }
 }
 
@@ -106,15 +112,17 @@ __device__ __forceinline__ float predict(float2 ab, double uu, int s)
 }
 
 // Initial parameters: the chord through the segment's first and last points.
-__global__ void k_linit(DevProblem P, const double* uu, const int* seg_start, int n_seg, float2* params)
+__global__ void k_linit(DevProblem P, const int* list, const double* uu, const int* seg_start, int n_seg,
+  float2* params)
 {
-  const int n = blockIdx.y;
+  const int n = list[blockIdx.y];
+  const int y = blockIdx.y;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg)
     return;
   const DevNuclide nd = P.nuc[n];
   const double* u = uu + nd.off;
-  const int64_t k = static_cast<int64_t>(n) * (n_seg + 1) + s;
+  const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
   int a0 = seg_start[k], a1 = seg_start[k + 1];
   float2 ab;
   if (a1 - a0 >= 2) {
@@ -126,19 +134,20 @@ __global__ void k_linit(DevProblem P, const double* uu, const int* seg_start, in
     ab.x = 0.0f;
     ab.y = static_cast<float>(a0 > 0 ? a0 - 1 : 0);
   }
-  params[static_cast<int64_t>(n) * n_seg + s] = ab;
+  params[static_cast<int64_t>(y) * n_seg + s] = ab;
 }
 
-__global__ void __launch_bounds__(LW) k_lforward(DevProblem P, const double* uu, const int* seg_start, int n_seg,
-  const float2* params, double* loss)
+__global__ void __launch_bounds__(LW) k_lforward(DevProblem P, const int* list, const double* uu,
+  const int* seg_start, int n_seg, const float2* params, double* loss)
 {
-  const int n = blockIdx.y;
+  const int n = list[blockIdx.y];
+  const int y = blockIdx.y;
   const int s = blockIdx.x;
   const DevNuclide nd = P.nuc[n];
   const double* u = uu + nd.off;
-  const int64_t k = static_cast<int64_t>(n) * (n_seg + 1) + s;
+  const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
   const int a0 = seg_start[k], a1 = seg_start[k + 1];
-  const float2 ab = params[static_cast<int64_t>(n) * n_seg + s];
+  const float2 ab = params[static_cast<int64_t>(y) * n_seg + s];
   double acc = 0.0;
   for (int i = a0 + threadIdx.x; i < a1; i += LW) {
     double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
@@ -146,19 +155,20 @@ __global__ void __launch_bounds__(LW) k_lforward(DevProblem P, const double* uu,
   }
   acc = wave_sum(acc);
   if (threadIdx.x == 0)
-    loss[static_cast<int64_t>(n) * n_seg + s] = acc;
+    loss[static_cast<int64_t>(y) * n_seg + s] = acc;
 }
 
-__global__ void __launch_bounds__(LW) k_lbackward(DevProblem P, const double* uu, const int* seg_start, int n_seg,
-  const float2* params, double2* grad)
+__global__ void __launch_bounds__(LW) k_lbackward(DevProblem P, const int* list, const double* uu,
+  const int* seg_start, int n_seg, const float2* params, double2* grad)
 {
-  const int n = blockIdx.y;
+  const int n = list[blockIdx.y];
+  const int y = blockIdx.y;
   const int s = blockIdx.x;
   const DevNuclide nd = P.nuc[n];
   const double* u = uu + nd.off;
-  const int64_t k = static_cast<int64_t>(n) * (n_seg + 1) + s;
+  const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
   const int a0 = seg_start[k], a1 = seg_start[k + 1];
-  const float2 ab = params[static_cast<int64_t>(n) * n_seg + s];
+  const float2 ab = params[static_cast<int64_t>(y) * n_seg + s];
   double ga = 0.0, gb = 0.0;
   for (int i = a0 + threadIdx.x; i < a1; i += LW) {
     double f = static_cast<double>(static_cast<float>(u[i] - static_cast<double>(s)));
@@ -169,22 +179,22 @@ __global__ void __launch_bounds__(LW) k_lbackward(DevProblem P, const double* uu
   ga = wave_sum(ga);
   gb = wave_sum(gb);
   if (threadIdx.x == 0)
-    grad[static_cast<int64_t>(n) * n_seg + s] = double2 {ga, gb};
+    grad[static_cast<int64_t>(y) * n_seg + s] = double2 {ga, gb};
 }
 
 // Adam on the per-point mean gradient
-__global__ void k_ladam(DevProblem P, const int* seg_start, int n_seg, const double2* grad, float2* params,
-  double2* m1, double2* m2, int step, double lr)
+__global__ void k_ladam(const int* seg_start, int n_seg, const double2* grad, float2* params, double2* m1,
+  double2* m2, int step, double lr)
 {
-  const int n = blockIdx.y;
+  const int y = blockIdx.y;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg)
     return;
-  const int64_t k = static_cast<int64_t>(n) * (n_seg + 1) + s;
+  const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
   const int cnt = seg_start[k + 1] - seg_start[k];
   if (cnt < 2)
     return;
-  const int64_t j = static_cast<int64_t>(n) * n_seg + s;
+  const int64_t j = static_cast<int64_t>(y) * n_seg + s;
   const double b1 = 0.9, b2 = 0.999, eps = 1e-12;
   double2 g = grad[j];
   g.x /= cnt;
@@ -204,18 +214,19 @@ __global__ void k_ladam(DevProblem P, const int* seg_start, int n_seg, const dou
 }
 
 // Recentre and record the error bound over points [a0 - 1, a1] of segment s.
-__global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const double* uu, const int* seg_start, int n_seg,
-  float2* params, int* err)
+__global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, const double* uu,
+  const int* seg_start, int n_seg, float2* params, int* err)
 {
-  const int n = blockIdx.y;
+  const int n = list[blockIdx.y];
+  const int y = blockIdx.y;
   const int s = blockIdx.x;
   const DevNuclide nd = P.nuc[n];
   const double* u = uu + nd.off;
-  const int64_t k = static_cast<int64_t>(n) * (n_seg + 1) + s;
+  const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
   int a0 = seg_start[k] - 1, a1 = seg_start[k + 1] + 1;
   a0 = a0 < 0 ? 0 : a0;
   a1 = a1 > nd.n ? nd.n : a1;
-  const int64_t j = static_cast<int64_t>(n) * n_seg + s;
+  const int64_t j = static_cast<int64_t>(y) * n_seg + s;
   float2 ab = params[j];
   double rmin = 1e300, rmax = -1e300;
   for (int i = a0 + threadIdx.x; i < a1; i += LW) {
@@ -251,115 +262,207 @@ void free_learned(Learned& L)
 
 namespace {
 
-struct Work {
+// One training round: the nuclides `list`, each with n_seg segments.
+struct Round {
   std::vector<void*> tmp;
-  double* uu;
-  int* seg_start;
-  double* loss;
-  double2* grad;
+  int* d_list = nullptr;
+  double* d_ulo = nullptr;  // [n_nuc]
+  double* d_invw = nullptr; // [n_nuc]
+  double* uu = nullptr;     // [all grid points]
+  int* seg_start = nullptr; // [list][n_seg + 1]
+  double* loss = nullptr;   // [list][n_seg]
+  double2* grad = nullptr;  // [list][n_seg]
+  int n_list = 0;
+  int n_seg = 0;
+  std::vector<double> u_lo, inv_w; // [n_nuc], set for the listed nuclides
+  ~Round()
+  {
+    for (void* p : tmp)
+      (void)hipFree(p);
+  }
 };
 
-void prepare(const DevProblem& P, const HostModel& m, int n_seg, std::vector<double>& u_lo, std::vector<double>& inv_w,
-  double*& d_ulo, double*& d_invw, Work& w, std::vector<void*>& keep, hipStream_t st)
+// Segment scale of every listed nuclide (segment s covers
+// u in [u_lo + s / inv_w, u_lo + (s + 1) / inv_w)), its grid points' uu and
+// the first point of each segment.
+void prepare(const DevProblem& P, const HostModel& m, const std::vector<int>& list, int n_seg, Round& r,
+  hipStream_t st)
 {
   const int nn = static_cast<int>(m.nuclides.size());
-  u_lo.resize(nn);
-  inv_w.resize(nn);
+  r.n_list = static_cast<int>(list.size());
+  r.n_seg = n_seg;
+  r.u_lo.assign(nn, 0.0);
+  r.inv_w.assign(nn, 0.0);
   int64_t total = 0;
   int maxn = 0;
-  for (int n = 0; n < nn; ++n) {
+  for (int n = 0; n < nn; ++n)
+    total += static_cast<int64_t>(m.nuclides[n].energy.size());
+  for (int n : list) {
     const auto& E = m.nuclides[n].energy;
     double x0 = detm::log(E.front() / m.energy_min);
     double x1 = detm::log(E.back() / m.energy_min);
-    u_lo[n] = x0;
-    inv_w[n] = static_cast<double>(n_seg) / (x1 - x0);
-    total += static_cast<int64_t>(E.size());
+    r.u_lo[n] = x0;
+    r.inv_w[n] = static_cast<double>(n_seg) / (x1 - x0);
     maxn = std::max<int>(maxn, static_cast<int>(E.size()));
   }
-  d_ulo = lalloc<double>(keep, nn);
-  d_invw = lalloc<double>(keep, nn);
-  lcheck(hipMemcpy(d_ulo, u_lo.data(), sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy");
-  lcheck(hipMemcpy(d_invw, inv_w.data(), sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy");
-  w.uu = lalloc<double>(w.tmp, total);
-  w.seg_start = lalloc<int>(w.tmp, static_cast<size_t>(nn) * (n_seg + 1));
-  w.loss = lalloc<double>(w.tmp, static_cast<size_t>(nn) * n_seg);
-  w.grad = lalloc<double2>(w.tmp, static_cast<size_t>(nn) * n_seg);
-  int bx = std::min(1024, (maxn + 255) / 256);
-  hipLaunchKernelGGL(k_lprep, dim3(bx, nn), dim3(256), 0, st, P, d_ulo, d_invw, w.uu);
-  int sx = (n_seg + 1 + 255) / 256;
-  hipLaunchKernelGGL(k_lsegbounds, dim3(sx, nn), dim3(256), 0, st, P, w.uu, n_seg, w.seg_start);
+  r.d_list = lalloc<int>(r.tmp, list.size());
+  r.d_ulo = lalloc<double>(r.tmp, nn);
+  r.d_invw = lalloc<double>(r.tmp, nn);
+  lcheck(hipMemcpy(r.d_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(r.d_ulo, r.u_lo.data(), sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(r.d_invw, r.inv_w.data(), sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy");
+  r.uu = lalloc<double>(r.tmp, total);
+  r.seg_start = lalloc<int>(r.tmp, list.size() * (n_seg + 1));
+  r.loss = lalloc<double>(r.tmp, list.size() * n_seg);
+  r.grad = lalloc<double2>(r.tmp, list.size() * n_seg);
+  const int bx = std::min(1024, (maxn + 255) / 256);
+  hipLaunchKernelGGL(k_lprep, dim3(bx, r.n_list), dim3(256), 0, st, P, r.d_list, r.d_ulo, r.d_invw, r.uu);
+  const int sx = (n_seg + 1 + 255) / 256;
+  hipLaunchKernelGGL(k_lsegbounds, dim3(sx, r.n_list), dim3(256), 0, st, P, r.d_list, r.uu, n_seg, r.seg_start);
   lcheck(hipGetLastError(), "learned prep");
+}
+
+// Per-listed-nuclide sum of the segment losses.
+std::vector<double> round_loss(const DevProblem& P, Round& r, const float2* params, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_lforward, dim3(r.n_seg, r.n_list), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, r.n_seg,
+    params, r.loss);
+  lcheck(hipGetLastError(), "learned forward");
+  std::vector<double> seg(static_cast<size_t>(r.n_list) * r.n_seg);
+  lcheck(hipMemcpyAsync(seg.data(), r.loss, sizeof(double) * seg.size(), hipMemcpyDeviceToHost, st), "hipMemcpy");
+  lcheck(hipStreamSynchronize(st), "learned forward");
+  std::vector<double> out(r.n_list, 0.0);
+  for (int y = 0; y < r.n_list; ++y)
+    for (int s = 0; s < r.n_seg; ++s)
+      out[y] += seg[static_cast<size_t>(y) * r.n_seg + s];
+  return out;
 }
 
 } // namespace
 
-int build_learned_index(Learned& L, const HostModel& m, DevProblem& P, int n_seg, int epochs, hipStream_t st)
+int build_learned_index(Learned& L, const HostModel& m, DevProblem& P, int n_seg, int epochs, hipStream_t st,
+  int max_err_bound)
 {
-  if (n_seg < 1 || n_seg > 1 << 20)
+  if (n_seg < 1 || n_seg > LEARNED_MAX_SEG)
     throw std::runtime_error("n_seg out of range");
+  if (max_err_bound < 0)
+    throw std::runtime_error("the learned index's error bound must be >= 0");
   free_learned(L);
   const int nn = static_cast<int>(m.nuclides.size());
-  std::vector<double> u_lo, inv_w;
-  double *d_ulo = nullptr, *d_invw = nullptr;
-  Work w;
-  prepare(P, m, n_seg, u_lo, inv_w, d_ulo, d_invw, w, L.allocs, st);
-  float2* params = lalloc<float2>(L.allocs, static_cast<size_t>(nn) * n_seg);
-  int* err = lalloc<int>(L.allocs, static_cast<size_t>(nn) * n_seg);
-  double2* m1 = lalloc<double2>(w.tmp, static_cast<size_t>(nn) * n_seg);
-  double2* m2 = lalloc<double2>(w.tmp, static_cast<size_t>(nn) * n_seg);
-  int sx = (n_seg + 255) / 256;
-  std::vector<double> loss(static_cast<size_t>(nn) * n_seg);
-  auto total_loss = [&]() {
-    hipLaunchKernelGGL(k_lforward, dim3(n_seg, nn), dim3(LW), 0, st, P, w.uu, w.seg_start, n_seg, params, w.loss);
-    lcheck(hipGetLastError(), "learned forward");
-    lcheck(hipMemcpyAsync(loss.data(), w.loss, sizeof(double) * loss.size(), hipMemcpyDeviceToHost, st),
-      "hipMemcpy");
-    lcheck(hipStreamSynchronize(st), "learned forward");
-    double t = 0.0;
-    for (double v : loss)
-      t += v;
-    return t;
-  };
-  hipLaunchKernelGGL(k_linit, dim3(sx, nn), dim3(256), 0, st, P, w.uu, w.seg_start, n_seg, params);
-  L.loss_init = total_loss();
-  for (int e = 1; e <= epochs; ++e) {
-    hipLaunchKernelGGL(k_lbackward, dim3(n_seg, nn), dim3(LW), 0, st, P, w.uu, w.seg_start, n_seg, params, w.grad);
-    double lr = 0.5 * std::pow(0.98, e);
-    hipLaunchKernelGGL(k_ladam, dim3(sx, nn), dim3(256), 0, st, P, w.seg_start, n_seg, w.grad, params, m1, m2, e, lr);
+  std::vector<std::vector<float>> seg_n(nn);
+  std::vector<std::vector<int>> err_n(nn);
+  L.h_ulo.assign(nn, 0.0);
+  L.h_invw.assign(nn, 0.0);
+  L.h_segn.assign(nn, 0);
+  L.loss_init = L.loss_trained = L.last_loss = 0.0;
+  L.rounds = 0;
+  std::vector<int> todo(nn);
+  for (int n = 0; n < nn; ++n)
+    todo[n] = n;
+  // Train every nuclide with n_seg segments; a nuclide whose recorded bound
+  // exceeds max_err_bound anywhere is trained again with twice as many.
+  for (int ns = n_seg; !todo.empty(); ns *= 2) {
+    if (ns > LEARNED_MAX_SEG)
+      throw std::runtime_error("learned index: nuclide " + m.nuclides[todo[0]].name + " needs more than " +
+                               std::to_string(LEARNED_MAX_SEG) + " segments to bound its index error by " +
+                               std::to_string(max_err_bound));
+    ++L.rounds;
+    Round r;
+    prepare(P, m, todo, ns, r, st);
+    const size_t np = static_cast<size_t>(r.n_list) * ns;
+    float2* params = lalloc<float2>(r.tmp, np);
+    int* err = lalloc<int>(r.tmp, np);
+    double2* m1 = lalloc<double2>(r.tmp, np);
+    double2* m2 = lalloc<double2>(r.tmp, np);
+    const int sx = (ns + 255) / 256;
+    hipLaunchKernelGGL(k_linit, dim3(sx, r.n_list), dim3(256), 0, st, P, r.d_list, r.uu, r.seg_start, ns, params);
+    const std::vector<double> l0 = round_loss(P, r, params, st);
+    for (int e = 1; e <= epochs; ++e) {
+      hipLaunchKernelGGL(k_lbackward, dim3(ns, r.n_list), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, ns,
+        params, r.grad);
+      double lr = 0.5 * std::pow(0.98, e);
+      hipLaunchKernelGGL(k_ladam, dim3(sx, r.n_list), dim3(256), 0, st, r.seg_start, ns, r.grad, params, m1, m2, e,
+        lr);
+    }
+    const std::vector<double> l1 = round_loss(P, r, params, st);
+    hipLaunchKernelGGL(k_lerror, dim3(ns, r.n_list), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, ns, params,
+      err);
+    lcheck(hipGetLastError(), "learned training");
+    const std::vector<double> l2 = round_loss(P, r, params, st);
+    std::vector<float> h_par(2 * np);
+    std::vector<int> h_err(np);
+    lcheck(hipMemcpy(h_par.data(), params, sizeof(float) * h_par.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    lcheck(hipMemcpy(h_err.data(), err, sizeof(int) * h_err.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    std::vector<int> next;
+    for (int y = 0; y < r.n_list; ++y) {
+      const int n = todo[y];
+      const int* e0 = h_err.data() + static_cast<size_t>(y) * ns;
+      const int worst = *std::max_element(e0, e0 + ns);
+      if (worst > max_err_bound) {
+        next.push_back(n);
+        continue;
+      }
+      seg_n[n].assign(h_par.begin() + 2 * static_cast<size_t>(y) * ns, h_par.begin() + 2 * static_cast<size_t>(y + 1) * ns);
+      err_n[n].assign(e0, e0 + ns);
+      L.h_ulo[n] = r.u_lo[n];
+      L.h_invw[n] = r.inv_w[n];
+      L.h_segn[n] = ns;
+      L.loss_init += l0[y];
+      L.loss_trained += l1[y];
+      L.last_loss += l2[y];
+    }
+    todo.swap(next);
   }
-  L.loss_trained = total_loss();
-  hipLaunchKernelGGL(k_lerror, dim3(n_seg, nn), dim3(LW), 0, st, P, w.uu, w.seg_start, n_seg, params, err);
-  lcheck(hipGetLastError(), "learned training");
-  L.last_loss = total_loss();
-  L.h_seg.resize(static_cast<size_t>(nn) * n_seg * 2);
-  L.h_err.resize(static_cast<size_t>(nn) * n_seg);
-  lcheck(hipMemcpy(L.h_seg.data(), params, sizeof(float) * L.h_seg.size(), hipMemcpyDeviceToHost), "hipMemcpy");
-  lcheck(hipMemcpy(L.h_err.data(), err, sizeof(int) * L.h_err.size(), hipMemcpyDeviceToHost), "hipMemcpy");
-  for (void* p : w.tmp)
-    (void)hipFree(p);
+  // concatenate the nuclides' segments
+  L.h_segoff.assign(nn, 0);
+  int64_t total = 0;
+  for (int n = 0; n < nn; ++n) {
+    if (total > INT32_MAX - L.h_segn[n])
+      throw std::runtime_error("learned index: too many segments in total");
+    L.h_segoff[n] = static_cast<int>(total);
+    total += L.h_segn[n];
+  }
+  L.h_seg.assign(2 * static_cast<size_t>(total), 0.0f);
+  L.h_err.assign(static_cast<size_t>(total), 0);
+  std::vector<LSeg> rec(static_cast<size_t>(total));
+  for (int n = 0; n < nn; ++n) {
+    const size_t o = static_cast<size_t>(L.h_segoff[n]);
+    std::copy(seg_n[n].begin(), seg_n[n].end(), L.h_seg.begin() + 2 * o);
+    std::copy(err_n[n].begin(), err_n[n].end(), L.h_err.begin() + o);
+  }
   L.max_err = 0;
   for (int v : L.h_err)
     L.max_err = std::max(L.max_err, v);
-  L.n_seg = n_seg;
-  L.h_ulo = u_lo;
-  L.h_invw = inv_w;
-  {
+  for (size_t k = 0; k < rec.size(); ++k) {
     // the lookup's packed copy: slope, intercept and error bound in one record
-    std::vector<LSeg> rec(L.h_err.size());
-    for (size_t k = 0; k < rec.size(); ++k) {
-      rec[k].a = L.h_seg[2 * k];
-      rec[k].b = L.h_seg[2 * k + 1];
-      rec[k].err = L.h_err[k];
-      rec[k].pad = 0;
-    }
-    LSeg* d_rec = lalloc<LSeg>(L.allocs, rec.size());
-    lcheck(hipMemcpy(d_rec, rec.data(), sizeof(LSeg) * rec.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    P.learned.rec = d_rec;
+    rec[k].a = L.h_seg[2 * k];
+    rec[k].b = L.h_seg[2 * k + 1];
+    rec[k].err = L.h_err[k];
+    rec[k].pad = 0;
   }
-  P.learned.seg = reinterpret_cast<const float*>(params);
-  P.learned.err = err;
+  LSeg* d_rec = lalloc<LSeg>(L.allocs, rec.size());
+  float* d_seg = lalloc<float>(L.allocs, L.h_seg.size());
+  int* d_err = lalloc<int>(L.allocs, L.h_err.size());
+  double* d_ulo = lalloc<double>(L.allocs, nn);
+  double* d_invw = lalloc<double>(L.allocs, nn);
+  int* d_segoff = lalloc<int>(L.allocs, nn);
+  int* d_segn = lalloc<int>(L.allocs, nn);
+  lcheck(hipMemcpy(d_rec, rec.data(), sizeof(LSeg) * rec.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(d_seg, L.h_seg.data(), sizeof(float) * L.h_seg.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(d_err, L.h_err.data(), sizeof(int) * L.h_err.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(d_ulo, L.h_ulo.data(), sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(d_invw, L.h_invw.data(), sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(d_segoff, L.h_segoff.data(), sizeof(int) * nn, hipMemcpyHostToDevice), "hipMemcpy");
+  lcheck(hipMemcpy(d_segn, L.h_segn.data(), sizeof(int) * nn, hipMemcpyHostToDevice), "hipMemcpy");
+  L.n_seg = n_seg;
+  L.err_bound = max_err_bound;
+  P.learned.rec = d_rec;
+  P.learned.seg = d_seg;
+  P.learned.err = d_err;
   P.learned.u_lo = d_ulo;
   P.learned.inv_w = d_invw;
+  P.learned.seg_off = d_segoff;
+  P.learned.seg_n = d_segn;
   P.learned.n_seg = n_seg;
   P.learned.enabled = 1;
   return L.max_err;
@@ -372,26 +475,19 @@ int learned_loss_and_grad(const DevProblem& P, int i_nuc, int n_seg, const float
   const int nn = static_cast<int>(m.nuclides.size());
   if (i_nuc < 0 || i_nuc >= nn)
     throw std::runtime_error("nuclide index out of bounds");
-  std::vector<double> u_lo, inv_w;
-  double *d_ulo = nullptr, *d_invw = nullptr;
-  Work w;
-  std::vector<void*> keep;
-  prepare(P, m, n_seg, u_lo, inv_w, d_ulo, d_invw, w, keep, st);
-  float2* params = lalloc<float2>(w.tmp, static_cast<size_t>(nn) * n_seg);
-  lcheck(hipMemcpy(params + static_cast<size_t>(i_nuc) * n_seg, params_in, sizeof(float2) * n_seg,
-           hipMemcpyHostToDevice),
-    "hipMemcpy");
-  hipLaunchKernelGGL(k_lforward, dim3(n_seg, nn), dim3(LW), 0, st, P, w.uu, w.seg_start, n_seg, params, w.loss);
-  hipLaunchKernelGGL(k_lbackward, dim3(n_seg, nn), dim3(LW), 0, st, P, w.uu, w.seg_start, n_seg, params, w.grad);
+  if (n_seg < 1 || n_seg > LEARNED_MAX_SEG)
+    throw std::runtime_error("n_seg out of range");
+  Round r;
+  prepare(P, m, std::vector<int> {i_nuc}, n_seg, r, st);
+  float2* params = lalloc<float2>(r.tmp, n_seg);
+  lcheck(hipMemcpy(params, params_in, sizeof(float2) * n_seg, hipMemcpyHostToDevice), "hipMemcpy");
+  hipLaunchKernelGGL(k_lforward, dim3(n_seg, 1), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, n_seg, params,
+    r.loss);
+  hipLaunchKernelGGL(k_lbackward, dim3(n_seg, 1), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, n_seg, params,
+    r.grad);
   lcheck(hipStreamSynchronize(st), "learned loss/grad");
-  lcheck(hipMemcpy(loss, w.loss + static_cast<size_t>(i_nuc) * n_seg, sizeof(double) * n_seg, hipMemcpyDeviceToHost),
-    "hipMemcpy");
-  lcheck(hipMemcpy(grad, w.grad + static_cast<size_t>(i_nuc) * n_seg, sizeof(double2) * n_seg, hipMemcpyDeviceToHost),
-    "hipMemcpy");
-  for (void* p : w.tmp)
-    (void)hipFree(p);
-  for (void* p : keep)
-    (void)hipFree(p);
+  lcheck(hipMemcpy(loss, r.loss, sizeof(double) * n_seg, hipMemcpyDeviceToHost), "hipMemcpy");
+  lcheck(hipMemcpy(grad, r.grad, sizeof(double2) * n_seg, hipMemcpyDeviceToHost), "hipMemcpy");
   return 0;
 }
 
@@ -412,8 +508,10 @@ int omc_learned_loss_grad(int i_nuc, int n_seg, const float* params, double* los
   }
 }
 
-// Training summary: n_seg, max error bound, mean error bound, loss before the
-// first gradient step, after the last one, and after recentring.
+// Training summary: base n_seg, max error bound, mean error bound, loss
+// before the first gradient step, after the last one, after recentring, the
+// number of training rounds, the error bound asked for, and the total,
+// smallest and largest segment counts over the nuclides.
 int omc_learned_stats(double* out6)
 {
   const omc::Learned& L = omc::learned_state();
@@ -432,16 +530,34 @@ int omc_learned_stats(double* out6)
   return 0;
 }
 
-// The trained model of one nuclide: params[n_seg][2] and err[n_seg].
+int omc_learned_stats2(double* out5)
+{
+  const omc::Learned& L = omc::learned_state();
+  if (L.n_seg == 0)
+    return -3;
+  out5[0] = L.rounds;
+  out5[1] = L.err_bound;
+  out5[2] = static_cast<double>(L.h_err.size());
+  out5[3] = L.h_segn.empty() ? 0 : *std::min_element(L.h_segn.begin(), L.h_segn.end());
+  out5[4] = L.h_segn.empty() ? 0 : *std::max_element(L.h_segn.begin(), L.h_segn.end());
+  return 0;
+}
+
+// The trained model of one nuclide: its segment count is returned, and with
+// params and err not null, params[n][2] and err[n] are filled (n = that count).
 int omc_learned_model(int i_nuc, float* params, int* err)
 {
   const omc::Learned& L = omc::learned_state();
-  const int nn = omc::problem_state().n_nuclides;
+  const int nn = static_cast<int>(L.h_segn.size());
   if (L.n_seg == 0 || i_nuc < 0 || i_nuc >= nn)
     return -3;
-  std::memcpy(params, L.h_seg.data() + static_cast<size_t>(i_nuc) * L.n_seg * 2, sizeof(float) * 2 * L.n_seg);
-  std::memcpy(err, L.h_err.data() + static_cast<size_t>(i_nuc) * L.n_seg, sizeof(int) * L.n_seg);
-  return L.n_seg;
+  const int ns = L.h_segn[i_nuc];
+  const size_t o = static_cast<size_t>(L.h_segoff[i_nuc]);
+  if (params)
+    std::memcpy(params, L.h_seg.data() + 2 * o, sizeof(float) * 2 * ns);
+  if (err)
+    std::memcpy(err, L.h_err.data() + o, sizeof(int) * ns);
+  return ns;
 }
 
 } // extern "C"

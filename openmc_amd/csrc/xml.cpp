@@ -113,6 +113,8 @@ struct Parser {
     for (size_t k = 0; k < t.size(); ++k) {
       if (t[k] == '&') {
         size_t e = t.find(';', k);
+        if (e == std::string::npos)
+          throw std::runtime_error("XML parse error: '&' without a closing ';' in \"" + t + "\"");
         std::string ent = t.substr(k + 1, e - k - 1);
         if (ent == "lt")
           out += '<';

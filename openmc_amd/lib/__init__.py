@@ -108,6 +108,53 @@ _bind("openmc_get_seed", [], c_int64, check=False)
 _bind("openmc_set_seed", [c_int64], None, check=False)
 _bind("openmc_get_stride", [], c_uint64, check=False)
 _bind("openmc_set_stride", [c_uint64], None, check=False)
+# model queries (openmc/lib/{tally,filter,mesh,material,cell,nuclide}.py)
+_bind("openmc_tally_get_filters", [c_int32, POINTER(POINTER(c_int32)), POINTER(c_size_t)])
+_bind("openmc_tally_get_scores", [c_int32, POINTER(POINTER(c_int)), POINTER(c_int)])
+_bind("openmc_tally_get_nuclides", [c_int32, POINTER(POINTER(c_int)), POINTER(c_int)])
+_bind("openmc_tally_get_estimator", [c_int32, POINTER(c_int)])
+_bind("openmc_tally_get_type", [c_int32, POINTER(c_int32)])
+_bind("openmc_tally_get_active", [c_int32, POINTER(c_bool)])
+_bind("openmc_tally_get_writable", [c_int32, POINTER(c_bool)])
+_bind("openmc_tally_reset", [c_int32])
+_bind("tally_filters_size", [], c_size_t, check=False)
+_bind("openmc_get_filter_index", [c_int32, POINTER(c_int32)])
+_bind("openmc_filter_get_id", [c_int32, POINTER(c_int32)])
+_bind("openmc_filter_get_type", [c_int32, c_char_p])
+_bind("openmc_filter_get_num_bins", [c_int32, POINTER(c_int)])
+_bind("openmc_mesh_filter_get_mesh", [c_int32, POINTER(c_int32)])
+_bind("openmc_energy_filter_get_bins", [c_int32, POINTER(POINTER(c_double)), POINTER(c_size_t)])
+_bind("n_meshes", [], c_int, check=False)
+_bind("openmc_get_mesh_index", [c_int32, POINTER(c_int32)])
+_bind("openmc_mesh_get_id", [c_int32, POINTER(c_int32)])
+_bind("openmc_mesh_get_type", [c_int32, c_char_p])
+_bind("openmc_mesh_get_n_elements", [c_int32, POINTER(c_size_t)])
+_bind("openmc_regular_mesh_get_dimension", [c_int32, POINTER(POINTER(c_int)), POINTER(c_int)])
+_bind("openmc_regular_mesh_get_params", [c_int32, POINTER(POINTER(c_double)), POINTER(POINTER(c_double)),
+                                         POINTER(POINTER(c_double)), POINTER(c_int)])
+_bind("n_materials", [], c_size_t, check=False)
+_bind("openmc_get_material_index", [c_int32, POINTER(c_int32)])
+_bind("openmc_material_get_id", [c_int32, POINTER(c_int32)])
+_bind("openmc_material_get_name", [c_int32, POINTER(c_char_p)])
+_bind("openmc_material_get_densities", [c_int32, POINTER(POINTER(c_int)), POINTER(POINTER(c_double)),
+                                        POINTER(c_int)])
+_bind("openmc_material_get_density", [c_int32, POINTER(c_double)])
+_bind("cells_size", [], c_int, check=False)
+_bind("openmc_get_cell_index", [c_int32, POINTER(c_int32)])
+_bind("openmc_cell_get_id", [c_int32, POINTER(c_int32)])
+_bind("openmc_cell_get_name", [c_int32, POINTER(c_char_p)])
+_bind("openmc_cell_get_fill", [c_int32, POINTER(c_int), POINTER(POINTER(c_int32)), POINTER(c_int32)])
+_bind("openmc_cell_get_num_instances", [c_int32, POINTER(c_int32)])
+_bind("nuclides_size", [], c_size_t, check=False)
+_bind("openmc_get_nuclide_index", [c_char_p, POINTER(c_int)])
+_bind("openmc_nuclide_name", [c_int, POINTER(c_char_p)])
+_bind("openmc_weight_windows_size", [], c_size_t, check=False)
+_bind("openmc_weight_windows_get_bounds", [c_int32, POINTER(POINTER(c_double)), POINTER(POINTER(c_double)),
+                                           POINTER(c_size_t)])
+_bind("openmc_weight_windows_get_energy_bounds", [c_int32, POINTER(POINTER(c_double)), POINTER(c_size_t)])
+_bind("openmc_weight_windows_get_mesh", [c_int32, POINTER(c_int32)])
+_bind("calc_zn", [c_int, c_double, c_double, POINTER(c_double)], None, check=False)
+_bind("calc_zn_rad", [c_int, c_double, POINTER(c_double)], None, check=False)
 _bind("omc_set_option", [c_char_p, c_double])
 _bind("omc_get_stats", [POINTER(c_double), c_int], c_int, check=False)
 _bind("omc_stat_names", [], c_char_p, check=False)
@@ -139,6 +186,7 @@ _bind("omc_learned_build", [c_int, c_int, POINTER(c_int)])
 _bind("omc_learned_loss_grad", [c_int, c_int, POINTER(c_float), POINTER(c_double), POINTER(c_double)])
 _bind("omc_learned_model", [c_int, POINTER(c_float), POINTER(c_int)], c_int, check=False)
 _bind("omc_learned_stats", [POINTER(c_double)], c_int, check=False)
+_bind("omc_learned_stats2", [POINTER(c_double)], c_int, check=False)
 
 is_initialized = False
 
@@ -164,6 +212,19 @@ class _Settings:
     generations_per_batch = _DLLGlobal(c_int32, "gen_per_batch")
     run_CE = _DLLGlobal(c_bool, "run_CE")
     verbosity = _DLLGlobal(c_int, "verbosity")
+    cmfd_run = _DLLGlobal(c_bool, "cmfd_run")
+    entropy_on = _DLLGlobal(c_bool, "entropy_on")
+    max_lost_particles = _DLLGlobal(c_int32, "max_lost_particles")
+    need_depletion_rx = _DLLGlobal(c_bool, "need_depletion_rx")
+    output_summary = _DLLGlobal(c_bool, "output_summary")
+    rel_max_lost_particles = _DLLGlobal(c_double, "rel_max_lost_particles")
+    restart_run = _DLLGlobal(c_bool, "restart_run")
+    weight_windows_on = _DLLGlobal(c_bool, "weight_windows_on")
+
+    @property
+    def path_statepoint(self):
+        path = c_char_p.in_dll(_dll, "path_statepoint_c").value
+        return path.decode() if path is not None else None
 
     @property
     def run_mode(self):
@@ -283,6 +344,48 @@ class Tally:
         return n.value
 
     @property
+    def filters(self):
+        ptr = POINTER(c_int32)()
+        n = c_size_t()
+        _dll.openmc_tally_get_filters(self._index, ptr, n)
+        return [_filter_from_index(ptr[i]) for i in range(n.value)]
+
+    @property
+    def scores(self):
+        ptr = POINTER(c_int)()
+        n = c_int()
+        _dll.openmc_tally_get_scores(self._index, ptr, n)
+        return [_SCORES.get(ptr[i], ptr[i]) for i in range(n.value)]
+
+    @property
+    def nuclides(self):
+        ptr = POINTER(c_int)()
+        n = c_int()
+        _dll.openmc_tally_get_nuclides(self._index, ptr, n)
+        return ["total" if ptr[i] == -1 else Nuclide(ptr[i]).name for i in range(n.value)]
+
+    @property
+    def estimator(self):
+        e = c_int()
+        _dll.openmc_tally_get_estimator(self._index, e)
+        return {0: "analog", 1: "tracklength", 2: "collision"}[e.value]
+
+    @property
+    def type(self):
+        t = c_int32()
+        _dll.openmc_tally_get_type(self._index, t)
+        return {0: "volume", 1: "mesh-surface", 2: "surface", 3: "pulse-height"}[t.value]
+
+    @property
+    def active(self):
+        a = c_bool()
+        _dll.openmc_tally_get_active(self._index, a)
+        return a.value
+
+    def reset(self):
+        _dll.openmc_tally_reset(self._index)
+
+    @property
     def results(self):
         data = POINTER(c_double)()
         shape = (c_size_t * 3)()
@@ -328,6 +431,224 @@ class _TallyMapping:
 
 
 tallies = _TallyMapping()
+
+_SCORES = {-1: "flux"}
+
+
+class Nuclide:
+    """A loaded nuclide (openmc/lib/nuclide.py)."""
+
+    def __init__(self, index):
+        self._index = int(index)
+
+    @property
+    def name(self):
+        name = c_char_p()
+        _dll.openmc_nuclide_name(self._index, name)
+        return name.value.decode()
+
+
+def nuclides():
+    return {Nuclide(i).name: Nuclide(i) for i in range(_dll.nuclides_size())}
+
+
+class RegularMesh:
+    """A regular mesh of the model (openmc/lib/mesh.py)."""
+
+    def __init__(self, index):
+        self._index = int(index)
+        mid = c_int32()
+        _dll.openmc_mesh_get_id(self._index, mid)
+        self.id = mid.value
+
+    def _params(self):
+        ll, ur, w = POINTER(c_double)(), POINTER(c_double)(), POINTER(c_double)()
+        n = c_int()
+        _dll.openmc_regular_mesh_get_params(self._index, ll, ur, w, n)
+        return [np.ctypeslib.as_array(p, (n.value,)).copy() for p in (ll, ur, w)]
+
+    @property
+    def dimension(self):
+        dims = POINTER(c_int)()
+        n = c_int()
+        _dll.openmc_regular_mesh_get_dimension(self._index, dims, n)
+        return tuple(dims[i] for i in range(n.value))
+
+    @property
+    def lower_left(self):
+        return self._params()[0]
+
+    @property
+    def upper_right(self):
+        return self._params()[1]
+
+    @property
+    def width(self):
+        return self._params()[2]
+
+    @property
+    def n_elements(self):
+        n = c_size_t()
+        _dll.openmc_mesh_get_n_elements(self._index, n)
+        return n.value
+
+
+def meshes():
+    out = {}
+    for i in range(_dll.n_meshes()):
+        m = RegularMesh(i)
+        out[m.id] = m
+    return out
+
+
+class Filter:
+    """A tally filter of the model (openmc/lib/filter.py)."""
+
+    def __init__(self, index):
+        self._index = int(index)
+        fid = c_int32()
+        _dll.openmc_filter_get_id(self._index, fid)
+        self.id = fid.value
+
+    @property
+    def n_bins(self):
+        n = c_int()
+        _dll.openmc_filter_get_num_bins(self._index, n)
+        return n.value
+
+
+class MeshFilter(Filter):
+    filter_type = "mesh"
+
+    @property
+    def mesh(self):
+        idx = c_int32()
+        _dll.openmc_mesh_filter_get_mesh(self._index, idx)
+        return RegularMesh(idx.value)
+
+
+class EnergyFilter(Filter):
+    filter_type = "energy"
+
+    @property
+    def bins(self):
+        ptr = POINTER(c_double)()
+        n = c_size_t()
+        _dll.openmc_energy_filter_get_bins(self._index, ptr, n)
+        e = np.ctypeslib.as_array(ptr, (n.value,)).copy()
+        return np.column_stack([e[:-1], e[1:]])
+
+
+def _filter_from_index(index):
+    buf = create_string_buffer(20)
+    _dll.openmc_filter_get_type(index, buf)
+    return {"mesh": MeshFilter, "energy": EnergyFilter}[buf.value.decode()](index)
+
+
+def filters():
+    out = {}
+    for i in range(_dll.tally_filters_size()):
+        f = _filter_from_index(i)
+        out[f.id] = f
+    return out
+
+
+class Material:
+    """A material of the model (openmc/lib/material.py)."""
+
+    def __init__(self, index):
+        self._index = int(index)
+        mid = c_int32()
+        _dll.openmc_material_get_id(self._index, mid)
+        self.id = mid.value
+
+    @property
+    def name(self):
+        name = c_char_p()
+        _dll.openmc_material_get_name(self._index, name)
+        return name.value.decode()
+
+    @property
+    def nuclides(self):
+        return self._densities()[0]
+
+    @property
+    def densities(self):
+        return self._densities()[1]
+
+    def _densities(self):
+        nuc, dens = POINTER(c_int)(), POINTER(c_double)()
+        n = c_int()
+        _dll.openmc_material_get_densities(self._index, nuc, dens, n)
+        names = [Nuclide(nuc[i]).name for i in range(n.value)]
+        return names, np.ctypeslib.as_array(dens, (n.value,)).copy()
+
+    def get_density(self, units="atom/b-cm"):
+        if units == "atom/b-cm":
+            return float(np.sum(self.densities))
+        rho = c_double()
+        _dll.openmc_material_get_density(self._index, rho)
+        return rho.value
+
+
+def materials():
+    out = {}
+    for i in range(_dll.n_materials()):
+        m = Material(i)
+        out[m.id] = m
+    return out
+
+
+class Cell:
+    """A cell of the model (openmc/lib/cell.py)."""
+
+    def __init__(self, index):
+        self._index = int(index)
+        cid = c_int32()
+        _dll.openmc_cell_get_id(self._index, cid)
+        self.id = cid.value
+
+    @property
+    def name(self):
+        name = c_char_p()
+        _dll.openmc_cell_get_name(self._index, name)
+        return name.value.decode()
+
+    @property
+    def fill(self):
+        """(type, index): type 'material' (index -1 = void), 'universe' or 'lattice'."""
+        t = c_int()
+        ptr = POINTER(c_int32)()
+        n = c_int32()
+        _dll.openmc_cell_get_fill(self._index, t, ptr, n)
+        return {0: "material", 1: "universe", 2: "lattice"}[t.value], ptr[0]
+
+    @property
+    def num_instances(self):
+        n = c_int32()
+        _dll.openmc_cell_get_num_instances(self._index, n)
+        return n.value
+
+
+def cells():
+    out = {}
+    for i in range(_dll.cells_size()):
+        c = Cell(i)
+        out[c.id] = c
+    return out
+
+
+def calc_zn(n, rho, phi):
+    """Zernike moments in the order of openmc.lib.math.calc_zn."""
+    zn = np.zeros(((n + 1) * (n + 2)) // 2)
+    _dll.calc_zn(n, rho, phi, zn.ctypes.data_as(POINTER(c_double)))
+    return zn
+
+
+def calc_zn_rad(n, rho):
+    zn = np.zeros(n // 2 + 1)
+    _dll.calc_zn_rad(n, rho, zn.ctypes.data_as(POINTER(c_double)))
+    return zn
 
 
 def global_tallies():
@@ -580,13 +901,20 @@ def learned_stats():
     if _dll.omc_learned_stats(out) < 0:
         raise AllocationError("learned index has not been built")
     keys = ["n_seg", "max_err", "mean_err", "loss_init", "loss_trained", "loss_final"]
-    return dict(zip(keys, list(out)))
+    st = dict(zip(keys, list(out)))
+    out2 = (c_double * 5)()
+    _dll.omc_learned_stats2(out2)
+    st.update(zip(["rounds", "err_bound", "total_segments", "min_segments", "max_segments"], list(out2)))
+    return st
 
 
-def learned_model(i_nuc, n_seg):
-    p = np.zeros((n_seg, 2), dtype=np.float32)
-    e = np.zeros(n_seg, dtype=np.int32)
-    r = _dll.omc_learned_model(i_nuc, p.ctypes.data_as(POINTER(c_float)), e.ctypes.data_as(POINTER(c_int)))
-    if r < 0:
+def learned_model(i_nuc, n_seg=None):
+    """(params[n][2], err[n]) of one nuclide's model; n is its own segment
+    count (the base count, or a multiple where the error bound needed it)."""
+    n = _dll.omc_learned_model(i_nuc, None, None)
+    if n < 0:
         raise AllocationError("learned index has not been built")
+    p = np.zeros((n, 2), dtype=np.float32)
+    e = np.zeros(n, dtype=np.int32)
+    _dll.omc_learned_model(i_nuc, p.ctypes.data_as(POINTER(c_float)), e.ctypes.data_as(POINTER(c_int)))
     return p, e

@@ -300,6 +300,8 @@ class Geometry:
             c, u = cells[cid]
             ce = ET.SubElement(el, "cell")
             ce.set("id", str(c.id))
+            if c.name:
+                ce.set("name", c.name)
             if c.fill is None:
                 ce.set("material", "void")
             elif isinstance(c.fill, Material):

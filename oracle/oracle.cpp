@@ -633,6 +633,7 @@ struct Particle {
   int64_t id = 0;
   double leakage = 0;
   uint64_t hash = 0;
+  uint64_t dhash = 0; // the discrete part of the history only (no floating-point bits)
   bool lost = false;
 
   Pos& r() { return coord[0].r; }
@@ -654,6 +655,7 @@ struct Sim {
   double leakage_batch = 0, total_weight = 0;
   double leak_sum = 0, leak_sum_sq = 0;
   std::vector<uint64_t> hashes; // last batch, by particle index
+  std::vector<uint64_t> dhashes; // discrete-only hashes of the same histories
   Counters cnt;
   int current_batch = 0;
   bool trace_hash = true;
@@ -764,7 +766,9 @@ void material_calculate_xs(const Problem& P, Particle& p, Counters& cnt, bool ha
   i_grid = i_grid < 0 ? 0 : (i_grid >= P.n_log_bins ? P.n_log_bins - 1 : i_grid);
   if (hash) {
     mix(p.hash, 0x11);
+    mix(p.dhash, 0x11);
     mix(p.hash, static_cast<uint64_t>(p.material));
+    mix(p.dhash, static_cast<uint64_t>(p.material));
   }
   for (size_t i = 0; i < mat.nuclide.size(); ++i) {
     int i_nuclide = mat.nuclide[i];
@@ -775,6 +779,7 @@ void material_calculate_xs(const Problem& P, Particle& p, Counters& cnt, bool ha
     }
     if (hash)
       mix(p.hash, static_cast<uint64_t>(static_cast<uint32_t>(micro.index_grid)));
+      mix(p.dhash, static_cast<uint64_t>(static_cast<uint32_t>(micro.index_grid)));
     double atom_density = mat.atom_density[i] * 1.0;
     p.macro_total += atom_density * micro.total;
     p.macro_absorption += atom_density * micro.absorption;
@@ -1361,9 +1366,11 @@ void initialize_history(Sim& S, Particle& p, int64_t index_source)
   // invalidate_neutron_xs
   p.micro.assign(P.nuclides.size(), MicroXS {0, 0, 0, 0, 0, 0.0});
   p.hash = 0xcbf29ce484222325ULL ^ static_cast<uint64_t>(id);
+  p.dhash = p.hash;
   if (S.trace_hash && P.ww_on) {
     mix(p.hash, as_bits(p.wgt));
     mix(p.hash, static_cast<uint64_t>(p.bank.size()));
+    mix(p.dhash, static_cast<uint64_t>(p.bank.size()));
   }
 }
 
@@ -1422,8 +1429,10 @@ void event_advance(Sim& S, Particle& p, Counters& cnt)
     p.wgt = 0.0;
   if (S.trace_hash) {
     mix(p.hash, 0x22);
+    mix(p.dhash, 0x22);
     mix(p.hash, as_bits(distance));
     mix(p.hash, p.collision_distance > p.boundary.distance ? 2 : 1);
+    mix(p.dhash, p.collision_distance > p.boundary.distance ? 2 : 1);
   }
 }
 
@@ -1500,6 +1509,7 @@ void event_revive(const Sim& S, Particle& p)
   p.n_event = 0;
   if (S.trace_hash) {
     mix(p.hash, 0x66);
+    mix(p.dhash, 0x66);
     mix(p.hash, as_bits(p.wgt));
     mix(p.hash, as_bits(p.E));
   }
@@ -1518,8 +1528,11 @@ void event_cross_surface(Sim& S, Particle& p, Counters& cnt)
   const int* t = p.boundary.lattice_translation;
   if (S.trace_hash) {
     mix(p.hash, 0x33);
+    mix(p.dhash, 0x33);
     mix(p.hash, static_cast<uint64_t>(static_cast<int64_t>(p.surface)));
+    mix(p.dhash, static_cast<uint64_t>(static_cast<int64_t>(p.surface)));
     mix(p.hash, static_cast<uint64_t>((t[0] + 1) | ((t[1] + 1) << 2) | ((t[2] + 1) << 4)));
+    mix(p.dhash, static_cast<uint64_t>((t[0] + 1) | ((t[1] + 1) << 2) | ((t[2] + 1) << 4)));
   }
   if (t[0] != 0 || t[1] != 0 || t[2] != 0) {
     cross_lattice(P, p, p.boundary);
@@ -1532,8 +1545,10 @@ void event_cross_surface(Sim& S, Particle& p, Counters& cnt)
   if (S.trace_hash) {
     mix(p.hash, as_bits(p.wgt));
     mix(p.hash, static_cast<uint64_t>(static_cast<int64_t>(p.alive() ? p.lowest().cell : -1)));
+    mix(p.dhash, static_cast<uint64_t>(static_cast<int64_t>(p.alive() ? p.lowest().cell : -1)));
     if (P.ww_on)
       mix(p.hash, static_cast<uint64_t>(p.bank.size()));
+      mix(p.dhash, static_cast<uint64_t>(p.bank.size()));
   }
 }
 
@@ -1650,8 +1665,11 @@ void event_collide(Sim& S, Particle& p, Counters& cnt)
     apply_weight_windows(P, p);
   if (S.trace_hash) {
     mix(p.hash, 0x44);
+    mix(p.dhash, 0x44);
     mix(p.hash, static_cast<uint64_t>(i_nuclide));
+    mix(p.dhash, static_cast<uint64_t>(i_nuclide));
     mix(p.hash, absorbed ? 101 : 2);
+    mix(p.dhash, absorbed ? 101 : 2);
     mix(p.hash, as_bits(p.E));
     mix(p.hash, as_bits(p.coord[0].u.x));
     mix(p.hash, as_bits(p.coord[0].u.y));
@@ -1659,6 +1677,7 @@ void event_collide(Sim& S, Particle& p, Counters& cnt)
     if (P.ww_on) {
       mix(p.hash, as_bits(p.wgt));
       mix(p.hash, static_cast<uint64_t>(p.bank.size()));
+      mix(p.dhash, static_cast<uint64_t>(p.bank.size()));
     }
   }
   if (p.E < P.energy_cutoff)
@@ -1709,6 +1728,7 @@ void transport_history_based(Sim& S, int64_t n)
         S.cnt.lost++;
       }
       hashes[i_work - 1] = p.hash;
+      S.dhashes[i_work - 1] = p.dhash;
     }
 #pragma omp critical
     {
@@ -1798,6 +1818,7 @@ void transport_event_based(Sim& S, int64_t n_total)
       if (particles[i].lost)
         S.cnt.lost++;
       S.hashes[source_offset + i] = particles[i].hash;
+      S.dhashes[source_offset + i] = particles[i].dhash;
     }
     S.cnt.lookups += c.lookups;
     S.cnt.collisions += c.collisions;
@@ -1823,6 +1844,7 @@ void run_batch(Sim& S, int mode, int64_t n_particles)
   S.total_weight = 0;
   std::fill(S.tally_value.begin(), S.tally_value.end(), 0.0);
   S.hashes.assign(n_particles, 0);
+  S.dhashes.assign(n_particles, 0);
   if (mode == 0)
     transport_history_based(S, n_particles);
   else
@@ -2264,6 +2286,7 @@ int64_t oracle_n_tally_bins(void* h) { return static_cast<Sim*>(h)->P.n_tally_bi
 const double* oracle_results(void* h) { return static_cast<Sim*>(h)->results.data(); }
 const double* oracle_batch_values(void* h) { return static_cast<Sim*>(h)->tally_value.data(); }
 const uint64_t* oracle_hashes(void* h) { return static_cast<Sim*>(h)->hashes.data(); }
+const uint64_t* oracle_dhashes(void* h) { return static_cast<Sim*>(h)->dhashes.data(); }
 int oracle_n_realizations(void* h) { return static_cast<Sim*>(h)->n_realizations; }
 
 void oracle_counters(void* h, int64_t* out)
@@ -2361,6 +2384,141 @@ void oracle_rotate_angle(const double* u, double mu, const double* phi, uint64_t
   out[1] = r.y;
   out[2] = r.z;
 }
+// Elementary pieces of the restatement, for the pins in tests/test_oracle.py
+// (each checked there against what the mathematics fixes).
+
+// Surface::distance (src/surface.cpp:208-218, 354-366, 400-442) of one
+// surface of the given type and coefficients.
+double oracle_surface_distance(int type, const double* c, const double* r, const double* u, int coincident)
+{
+  Surface s;
+  s.id = 1;
+  s.type = type;
+  s.bc = BC_TRANSMIT;
+  for (int k = 0; k < 4; ++k)
+    s.c[k] = c[k];
+  return s.distance({r[0], r[1], r[2]}, {u[0], u[1], u[2]}, coincident != 0);
+}
+
+// RectLattice::distance and get_indices (src/lattice.cpp:252-342) of lattice
+// `lat` of a loaded problem; r is in the lattice's own (local) frame for the
+// distance and in the containing universe's frame for the indices.
+double oracle_lattice_distance(void* h, int lat, const double* r, const double* u, int* trans)
+{
+  const Problem& P = static_cast<Sim*>(h)->P;
+  return P.lattices.at(lat).distance({r[0], r[1], r[2]}, {u[0], u[1], u[2]}, trans);
+}
+
+void oracle_lattice_indices(void* h, int lat, const double* r, const double* u, int* ijk)
+{
+  const Problem& P = static_cast<Sim*>(h)->P;
+  P.lattices.at(lat).get_indices({r[0], r[1], r[2]}, {u[0], u[1], u[2]}, ijk);
+}
+
+// A particle at r going along u: find_cell, then distance_to_boundary.
+// out_i = {found, n_coord, cell of each level (4), lattice index x/y/z of the
+// deepest level, surface (signed index + 1), coord level, translation x/y/z};
+// returns the distance.
+double oracle_locate(void* h, const double* r, const double* u, int* out_i)
+{
+  const Problem& P = static_cast<Sim*>(h)->P;
+  Particle p;
+  p.coord.resize(P.n_coord_levels);
+  p.cell_last.resize(P.n_coord_levels);
+  p.coord[0].r = {r[0], r[1], r[2]};
+  p.coord[0].u = {u[0], u[1], u[2]};
+  bool found = exhaustive_find_cell(P, p);
+  for (int k = 0; k < 15; ++k)
+    out_i[k] = -1;
+  out_i[0] = found ? 1 : 0;
+  if (!found)
+    return -1.0;
+  out_i[1] = p.n_coord;
+  for (int k = 0; k < p.n_coord && k < 4; ++k)
+    out_i[2 + k] = p.coord[k].cell;
+  const LocalCoord& lo = p.coord[p.n_coord - 1];
+  for (int k = 0; k < 3; ++k)
+    out_i[6 + k] = lo.lattice == C_NONE ? -1 : lo.lattice_index[k];
+  Boundary b = distance_to_boundary(P, p);
+  out_i[9] = b.surface;
+  out_i[10] = b.coord_level;
+  for (int k = 0; k < 3; ++k)
+    out_i[11 + k] = b.lattice_translation[k];
+  return b.distance;
+}
+
+// sample_cxs_target_velocity (src/physics.cpp:990-1041): out3 = target velocity
+void oracle_target_velocity(double awr, double E, const double* u, double kT, uint64_t* seed, double* out3)
+{
+  Pos v = sample_cxs_target_velocity(awr, E, {u[0], u[1], u[2]}, kT, seed);
+  out3[0] = v.x;
+  out3[1] = v.y;
+  out3[2] = v.z;
+}
+
+// elastic_scatter (src/physics.cpp:754-818) off one nuclide of atomic weight
+// ratio awr at temperature kT: out4 = {E', u'}
+void oracle_elastic(double awr, double kT, double free_gas_threshold, double E, const double* u, uint64_t* seed,
+  double* out4)
+{
+  Problem P;
+  P.free_gas_threshold = free_gas_threshold;
+  Nuclide nuc;
+  nuc.awr = awr;
+  nuc.kT = kT;
+  P.nuclides.push_back(nuc);
+  Particle p;
+  p.coord.resize(1);
+  p.E = E;
+  p.coord[0].u = {u[0], u[1], u[2]};
+  p.seeds[STREAM_TRACKING] = *seed;
+  p.stream = STREAM_TRACKING;
+  elastic_scatter(P, 0, kT, p);
+  *seed = p.seeds[STREAM_TRACKING];
+  out4[0] = p.E;
+  out4[1] = p.coord[0].u.x;
+  out4[2] = p.coord[0].u.y;
+  out4[3] = p.coord[0].u.z;
+}
+
+// raytrace_mesh over the problem's tally mesh for the segment r0 -> r1 along
+// u: the crossed bins and their track fractions in order; returns how many
+// there are (at most cap are written).
+int oracle_raytrace(void* h, const double* r0, const double* r1, const double* u, int* bins, double* fracs,
+  int cap)
+{
+  const Problem& P = static_cast<Sim*>(h)->P;
+  int n = 0;
+  raytrace_mesh(P.mesh, {r0[0], r0[1], r0[2]}, {r1[0], r1[1], r1[2]}, {u[0], u[1], u[2]},
+    [&](const int* ijk, double w) {
+      if (n < cap) {
+        bins[n] = mesh_bin(P.mesh, ijk);
+        fracs[n] = w;
+      }
+      ++n;
+    });
+  return n;
+}
+
+// Source::satisfies_spatial_constraints (src/source.cpp:226-271)
+int oracle_in_source_domain(void* h, const double* r)
+{
+  return satisfies_spatial_constraints(static_cast<Sim*>(h)->P, {r[0], r[1], r[2]}) ? 1 : 0;
+}
+
+// IndependentSource::sample (src/source.cpp:355-417): out7 = {r, u, E}
+void oracle_sample_source(void* h, uint64_t* seed, double* out7)
+{
+  SourceSite site = sample_source(static_cast<Sim*>(h)->P, seed);
+  out7[0] = site.r.x;
+  out7[1] = site.r.y;
+  out7[2] = site.r.z;
+  out7[3] = site.u.x;
+  out7[4] = site.u.y;
+  out7[5] = site.u.z;
+  out7[6] = site.E;
+}
+
 double oracle_log(double x) { return m_log(x); }
 double oracle_sin(double x) { return m_sin(x); }
 double oracle_cos(double x) { return m_cos(x); }

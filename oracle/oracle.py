@@ -35,6 +35,8 @@ def load_library(glibc_math=False):
     lib.oracle_batch_values.restype = POINTER(c_double)
     lib.oracle_hashes.argtypes = [c_void_p]
     lib.oracle_hashes.restype = POINTER(c_uint64)
+    lib.oracle_dhashes.argtypes = [c_void_p]
+    lib.oracle_dhashes.restype = POINTER(c_uint64)
     lib.oracle_n_realizations.argtypes = [c_void_p]
     lib.oracle_counters.argtypes = [c_void_p, POINTER(c_int64)]
     lib.oracle_leakage.argtypes = [c_void_p, POINTER(c_double)]
@@ -55,11 +57,52 @@ def load_library(glibc_math=False):
     lib.oracle_ww_apply.argtypes = [c_void_p, POINTER(c_double), POINTER(c_uint64), POINTER(c_double),
                                     POINTER(c_double), c_int]
     lib.oracle_ww_apply.restype = c_int
+    lib.oracle_surface_distance.argtypes = [c_int, POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int]
+    lib.oracle_surface_distance.restype = c_double
+    lib.oracle_lattice_distance.argtypes = [c_void_p, c_int, POINTER(c_double), POINTER(c_double), POINTER(c_int)]
+    lib.oracle_lattice_distance.restype = c_double
+    lib.oracle_lattice_indices.argtypes = [c_void_p, c_int, POINTER(c_double), POINTER(c_double), POINTER(c_int)]
+    lib.oracle_locate.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_int)]
+    lib.oracle_locate.restype = c_double
+    lib.oracle_target_velocity.argtypes = [c_double, c_double, POINTER(c_double), c_double, POINTER(c_uint64),
+                                           POINTER(c_double)]
+    lib.oracle_elastic.argtypes = [c_double, c_double, c_double, c_double, POINTER(c_double), POINTER(c_uint64),
+                                   POINTER(c_double)]
+    lib.oracle_raytrace.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                    POINTER(c_int), POINTER(c_double), c_int]
+    lib.oracle_raytrace.restype = c_int
+    lib.oracle_in_source_domain.argtypes = [c_void_p, POINTER(c_double)]
+    lib.oracle_sample_source.argtypes = [c_void_p, POINTER(c_uint64), POINTER(c_double)]
     for fn in ("oracle_log", "oracle_sin", "oracle_cos"):
         getattr(lib, fn).argtypes = [c_double]
         getattr(lib, fn).restype = c_double
     _libs[path] = lib
     return lib
+
+
+def surface_distance(stype, coeffs, r, u, coincident=False):
+    """Surface::distance of one surface (types: 1..3 x/y/z-plane, 4 plane,
+    5 z-cylinder, 6 x-cylinder, 7 y-cylinder, 8 sphere)."""
+    lib = load_library()
+    c = (c_double * 4)(*(list(coeffs) + [0.0] * (4 - len(coeffs))))
+    return lib.oracle_surface_distance(stype, c, (c_double * 3)(*r), (c_double * 3)(*u), 1 if coincident else 0)
+
+
+def elastic(awr, kT, E, u, seed, free_gas_threshold=400.0):
+    """elastic_scatter off a nuclide: (E', u', seed after)."""
+    lib = load_library()
+    s = c_uint64(seed)
+    out = (c_double * 4)()
+    lib.oracle_elastic(awr, kT, free_gas_threshold, E, (c_double * 3)(*u), s, out)
+    return out[0], (out[1], out[2], out[3]), s.value
+
+
+def target_velocity(awr, E, u, kT, seed):
+    lib = load_library()
+    s = c_uint64(seed)
+    out = (c_double * 3)()
+    lib.oracle_target_velocity(awr, E, (c_double * 3)(*u), kT, s, out)
+    return (out[0], out[1], out[2]), s.value
 
 
 class Oracle:
@@ -105,6 +148,12 @@ class Oracle:
     def hashes(self, n):
         return np.ctypeslib.as_array(self.lib.oracle_hashes(self.h), (n,)).copy()
 
+    def discrete_hashes(self, n):
+        """Hashes of the histories' discrete events only (event kinds, cells,
+        surfaces, lattice moves, nuclides, grid indices, absorption), without
+        any floating-point bits."""
+        return np.ctypeslib.as_array(self.lib.oracle_dhashes(self.h), (n,)).copy()
+
     def n_realizations(self):
         return self.lib.oracle_n_realizations(self.h)
 
@@ -140,6 +189,42 @@ class Oracle:
         n = self.lib.oracle_ww_apply(self.h, inp, s, out, bank, 64)
         state = dict(zip(["wgt", "wgt_ww_born", "ww_factor", "n_split", "n_bank"], list(out)))
         return state, list(bank)[:min(n, 64)], s.value
+
+    # elementary pieces, for the oracle's own pins
+    def lattice_distance(self, lat, r, u):
+        t = (c_int * 3)()
+        d = self.lib.oracle_lattice_distance(self.h, lat, (c_double * 3)(*r), (c_double * 3)(*u), t)
+        return d, tuple(t)
+
+    def lattice_indices(self, lat, r, u):
+        ijk = (c_int * 3)()
+        self.lib.oracle_lattice_indices(self.h, lat, (c_double * 3)(*r), (c_double * 3)(*u), ijk)
+        return tuple(ijk)
+
+    def locate(self, r, u):
+        """find_cell + distance_to_boundary at (r, u): (distance, info dict)."""
+        out = (c_int * 15)()
+        d = self.lib.oracle_locate(self.h, (c_double * 3)(*r), (c_double * 3)(*u), out)
+        o = list(out)
+        return d, {"found": o[0] == 1, "n_coord": o[1], "cells": o[2:2 + max(o[1], 0)],
+                   "lattice_index": tuple(o[6:9]), "surface": o[9], "coord_level": o[10],
+                   "translation": tuple(o[11:14])}
+
+    def raytrace(self, r0, r1, u, cap=4096):
+        bins = (c_int * cap)()
+        fr = (c_double * cap)()
+        n = self.lib.oracle_raytrace(self.h, (c_double * 3)(*r0), (c_double * 3)(*r1), (c_double * 3)(*u), bins,
+                                     fr, cap)
+        return list(bins)[:min(n, cap)], list(fr)[:min(n, cap)]
+
+    def in_source_domain(self, r):
+        return self.lib.oracle_in_source_domain(self.h, (c_double * 3)(*r)) == 1
+
+    def sample_source(self, seed):
+        s = c_uint64(seed)
+        out = (c_double * 7)()
+        self.lib.oracle_sample_source(self.h, s, out)
+        return list(out), s.value
 
     def lookup(self, i_nuc, energies):
         E = np.asarray(energies, dtype=np.float64)

@@ -1,0 +1,102 @@
+"""GPU parity on the benchmark-size data.
+
+The 300-nuclide x 200k-point material (the lookup-dominated workload) and the
+full 60-nuclide 17x17 lattice with its 100^3 mesh, against the oracle:
+  * every lookup, plain and learned, index for index and bit for bit, on
+    random energies, exact grid points, midpoints and every log-grid bucket
+    edge of a sample of nuclides;
+  * short transport runs whose histories (hashes) are bit-identical and whose
+    tallies agree within 1e-12 per bin.
+The particle counts are the only thing reduced: the data, geometry, source
+and tally are those of the benchmark.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import assert_tallies_close, chdir, gpu_run, oracle_run, prepare
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from openmc_amd import lib as _lib
+    return _lib
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    print("case: preparing the 300-nuclide x 200k-point problem", flush=True)
+    return prepare(2, n_particles=2000, n_batches=1)
+
+
+def _energies(rng, pts, e_min, e_max, spacing, n_random=20000, n_points=4000):
+    pick = rng.choice(pts.size, size=min(n_points, pts.size), replace=False)
+    mids = 0.5 * (pts[pick[:-1]] + pts[pick[:-1] + 1]) if pts.size > 1 else pts
+    return np.concatenate([np.exp(rng.uniform(np.log(e_min), np.log(e_max), n_random)), pts[pick], mids,
+                           e_min * np.exp(np.arange(8001) * spacing), [pts[0], pts[-1], e_min, e_max]])
+
+
+def test_cfg2_lookups_plain_and_learned_bitwise(lib, cfg2):
+    """Every lookup of 20 of the 300 nuclides (>= 3.4e4 energies each) equals the
+    oracle's index and cross sections, plain and learned, with the learned
+    index's recorded error within 64 everywhere (segments doubled where needed)."""
+    d, model, nucs = cfg2
+    from oracle.oracle import Oracle
+    o = Oracle(os.path.join(d, "oracle_problem.txt"))
+    rng = np.random.default_rng(2)
+    with chdir(d):
+        lib.init(output=False)
+        try:
+            max_err = lib.learned_build(256, 200)
+            st = lib.learned_stats()
+            print("learned index: max_err %d, rounds %d, segments %d..%d, total %d" % (
+                max_err, st["rounds"], st["min_segments"], st["max_segments"], st["total_segments"]), flush=True)
+            assert max_err <= 64 and st["max_err"] <= 64
+            e_min, e_max, spacing = o.energy_grid()
+            for i_nuc in range(0, len(nucs), 15):
+                p, e = lib.learned_model(i_nuc)
+                assert e.max() <= 64
+                E = _energies(rng, nucs[i_nuc].energy, e_min, e_max, spacing)
+                ig_o, xs_o = o.lookup(i_nuc, E)
+                ig_p, xs_p, _ = lib.lookup_batch(i_nuc, E, learned=False)
+                ig_l, xs_l, _ = lib.lookup_batch(i_nuc, E, learned=True)
+                assert np.array_equal(ig_o, ig_p), "plain, nuclide %d" % i_nuc
+                assert np.array_equal(ig_o, ig_l), "learned, nuclide %d" % i_nuc
+                assert np.array_equal(xs_o[:, 1:3].view(np.uint64), xs_p.view(np.uint64))
+                assert np.array_equal(xs_o[:, 1:3].view(np.uint64), xs_l.view(np.uint64))
+        finally:
+            lib.finalize()
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["plain", "learned"])
+def test_cfg2_transport_histories(cfg2, learned):
+    """2,000 histories in the 300-nuclide cube with the 100^3 mesh: hashes
+    bit-identical to the oracle's, tallies within 1e-12 per bin."""
+    print("case: cfg2 transport, 2000 particles, learned=%s" % learned, flush=True)
+    d, model, nucs = cfg2
+    o = oracle_run(d, threads=8)
+    g = gpu_run(d, options={"trace_hash": 1}, learned=(256, 200) if learned else None)
+    h_o = o.hashes(2000)
+    assert np.array_equal(g["hashes"], h_o), "%d of 2000 histories differ" % np.sum(g["hashes"] != h_o)
+    assert_tallies_close(g["results"], o.results())
+    c = o.counters()
+    assert g["stats"]["collisions"] == c["collisions"] and g["stats"]["segments"] == c["segments"]
+
+
+def test_cfg1_full_data_histories():
+    """20,000 histories of the full lattice problem (60 nuclides with 20k-100k
+    point grids, 17x17 pins, reflective sides, Watt source in the fuel, 100^3
+    mesh): hashes bit-identical, tallies within 1e-12 per bin, plain and learned."""
+    print("case: cfg1 full data, 20000 particles", flush=True)
+    n = 20000
+    d, model, nucs = prepare(1, n_particles=n, n_batches=1)
+    o = oracle_run(d, threads=8)
+    h_o = o.hashes(n)
+    for learned in (None, (256, 200)):
+        g = gpu_run(d, options={"trace_hash": 1}, learned=learned)
+        assert np.array_equal(g["hashes"], h_o), "%d of %d histories differ (learned %s)" % (
+            np.sum(g["hashes"] != h_o), n, learned)
+        assert_tallies_close(g["results"], o.results())

@@ -208,6 +208,80 @@ def test_learned_loss_and_gradient_match_cpu(lib):
             lib.finalize()
 
 
+def test_learned_gradient_matches_finite_differences(lib):
+    """The backward kernel's gradient equals central differences of the forward
+    kernel's loss.  The loss of one segment is quadratic in (a, b), so a
+    central difference with a unit step is exact up to the FP32 rounding of
+    the prediction; this pins the backward kernel without restating it."""
+    print("case: learned gradient vs finite differences, cfg0 nuclides", flush=True)
+    d, model, nucs = prepare(0, n_particles=100, n_batches=1)
+    from helpers import chdir
+    n_seg = 32
+    with chdir(d):
+        lib.init(output=False)
+        try:
+            for i_nuc in range(2):
+                rng = np.random.default_rng(100 + i_nuc)
+                n = nucs[i_nuc].energy.size
+                params = np.stack([rng.uniform(0, 600, n_seg), rng.uniform(0, n, n_seg)], 1).astype(np.float32)
+                loss, grad = lib.learned_loss_grad(i_nuc, params)
+                for col in range(2):
+                    up = params.copy()
+                    dn = params.copy()
+                    up[:, col] += 1.0
+                    dn[:, col] -= 1.0
+                    fd = (lib.learned_loss_grad(i_nuc, up)[0] - lib.learned_loss_grad(i_nuc, dn)[0]) / 2.0
+                    scale = np.abs(grad[:, col]).max()
+                    assert np.allclose(grad[:, col], fd, rtol=1e-4, atol=1e-4 * scale), (i_nuc, col)
+                    # the other partial does not match (a swapped column would pass otherwise)
+                    assert not np.allclose(grad[:, 1 - col], fd, rtol=1e-2, atol=1e-3 * scale)
+        finally:
+            lib.finalize()
+
+
+def test_learned_error_bound_enforced_by_doubling(lib):
+    """With a tight bound the build trains again, with twice the segments, the
+    nuclides whose recorded error exceeds it; every recorded error is then
+    within the bound and every lookup still equals the oracle's index.  (The
+    bound must exceed half the largest cluster of points that no segment can
+    split: the 40 points of one resonance span ~1e-8 in log E, so about 20.)"""
+    print("case: learned error bound 32, cfg0 nuclides", flush=True)
+    d, model, nucs = prepare(0, n_particles=100, n_batches=1)
+    from oracle.oracle import Oracle
+    import os
+    o = Oracle(os.path.join(d, "oracle_problem.txt"))
+    from helpers import chdir
+    rng = np.random.default_rng(5)
+    with chdir(d):
+        lib.init(output=False)
+        try:
+            lib.set_option("learned_err_bound", 32)
+            assert lib.learned_build(16, 100) <= 32
+            st = lib.learned_stats()
+            assert st["rounds"] > 1 and st["err_bound"] == 32 and st["max_err"] <= 32
+            assert st["max_segments"] > 16 and st["min_segments"] >= 16
+            sizes = []
+            for i_nuc in range(len(nucs)):
+                p, e = lib.learned_model(i_nuc)
+                assert e.max() <= 32
+                n_s = p.shape[0]
+                assert n_s % 16 == 0 and (n_s // 16) & (n_s // 16 - 1) == 0
+                sizes.append(n_s)
+                pts = nucs[i_nuc].energy
+                e_min, e_max, spacing = o.energy_grid()
+                E = np.concatenate([np.exp(rng.uniform(np.log(e_min), np.log(e_max), 20000)), pts,
+                                    0.5 * (pts[:-1] + pts[1:]), e_min * np.exp(np.arange(8000) * spacing)])
+                ig_o, xs_o = o.lookup(i_nuc, E)
+                ig_l, xs_l, probes = lib.lookup_batch(i_nuc, E, learned=True)
+                assert np.array_equal(ig_o, ig_l)
+                assert np.array_equal(xs_o[:, 1:3].view(np.uint64), xs_l.view(np.uint64))
+            assert sum(sizes) == st["total_segments"]
+            with pytest.raises(lib.OpenMCError, match="learned_err_bound"):
+                lib.set_option("learned_err_bound", 65)
+        finally:
+            lib.finalize()
+
+
 def test_learned_training_reduces_loss_and_bounds_hold(lib):
     d, model, nucs = prepare(0, n_particles=100, n_batches=1)
     from helpers import chdir
