@@ -8,11 +8,17 @@ A step is one batch.  Inputs (nuclear data, geometry, tallies) are resident in
 HBM before timing starts; the timed region is W warm-up batches excluded, then
 exactly K batches, each ending in a stream synchronization.
 
-Prints one JSON line: particles/s (whole job), XS lookups/s, the roofline of
-the dominant kernel from HIP events recorded on the library's stream around
-every kernel launch in the timed region (its measured HBM traffic from the
-rocprofv3 summary in profiles/), the learned-index comparison and a CPU
-baseline (the oracle, history- and event-based, OpenMP over the host cores,
+The timed batches run the library's default path: event cycles replayed
+from hipGraphs, no per-kernel timers.  After them, one more batch runs with a
+HIP event recorded on the library's stream around every kernel launch (direct
+launches); the per-kernel table and the roofline of the dominant kernel come
+from that batch (its measured HBM traffic from the rocprofv3 summary in
+profiles/).
+
+Prints one JSON line: particles/s (whole job), XS lookups/s, the roofline,
+the learned-index comparison on this workload and on the lookup-dominated
+one (300 nuclides x 200k points in one material), and a CPU baseline (the
+oracle, history- and event-based, OpenMP over the host's cores, best of three
 on a bounded sample of the same workload).
 """
 import argparse
@@ -73,7 +79,12 @@ def diff(a, b):
     return {k: b[k] - a[k] for k in b}
 
 
-def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None):
+def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None, timed_steps=1):
+    """W untimed warm-up batches, then exactly K timed batches on the default
+    launch path (event cycles replayed from hipGraphs, no per-kernel timers);
+    then `timed_steps` more batches with a HIP event recorded around every
+    kernel launch for the per-kernel times and the roofline (direct launches,
+    so these are kept out of the headline)."""
     from openmc_amd import lib
     old = os.getcwd()
     os.chdir(d)
@@ -88,7 +99,7 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
             info["learned_build_s"] = time.perf_counter() - t
             info.update({"learned_" + k: v for k, v in lib.learned_stats().items()})
             lib.set_option("learned_index", 1)
-        lib.set_option("event_timing", 1)
+        lib.set_option("event_timing", 0)
         lib.set_option("progress_s", 30)
         lib.simulation_init()
         tw = time.perf_counter()
@@ -103,6 +114,15 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
             log("  batch %d done (%.2f s)" % (k + 1, time.perf_counter() - t0))
         t1 = time.perf_counter()
         s1 = lib.stats()
+        if timed_steps > 0:
+            lib.set_option("event_timing", 1)
+            lib.reset_timers()
+            s1b = lib.stats()
+            tk = time.perf_counter()
+            for k in range(timed_steps):
+                lib.next_batch()
+                log("  event-timed batch %d done (%.2f s)" % (k + 1, time.perf_counter() - tk))
+            s2 = lib.stats()
         lib.simulation_finalize()
         n = lib.settings.particles
         lib.finalize()
@@ -111,6 +131,9 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
     st = diff(s0, s1)
     info.update({"elapsed_s": t1 - t0, "stats": st, "n_particles": n,
                  "config_stats": {k: s1.get(k, 0) for k in CONFIG_STATS}})
+    if timed_steps > 0:
+        info["timed"] = {"stats": diff(s1b, s2), "steps": timed_steps,
+                         "config_stats": {k: s2.get(k, 0) for k in CONFIG_STATS}}
     return info
 
 
@@ -176,11 +199,30 @@ def roofline(info):
                  "kernel": name, "whole_loop_GB_per_s": round(whole, 2)}, per_kernel
 
 
-def cpu_baseline(d, target_s=10.0):
+def host_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS where the host sets
+    it (the host's share of cores), else every core."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(d, target_s=4.0, repeats=3):
     """The oracle on the host cores, history-based and event-based, each on a
-    sample of the first particles of batch 1 sized to about target_s."""
+    sample of the first particles of batch 1 sized to about target_s, timed
+    `repeats` times (best kept)."""
     from oracle.oracle import Oracle
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     path = os.path.join(d, "oracle_problem.txt")
 
     def timed(event_based, n):
@@ -195,7 +237,7 @@ def cpu_baseline(d, target_s=10.0):
     modes = {}
     for name, event_based in (("history", False), ("event", True)):
         # probes of growing size until one takes a tenth of the target, then
-        # one sample sized to the target (every step logged)
+        # a sample sized to the target, timed `repeats` times
         n, el = 64, 0.0
         while True:
             el, c = timed(event_based, n)
@@ -203,23 +245,30 @@ def cpu_baseline(d, target_s=10.0):
             if el >= 0.1 * target_s or n >= 2_000_000:
                 break
             n = int(min(2_000_000, n * min(16.0, max(2.0, 0.1 * target_s / max(el, 1e-3)))))
-        if el < 0.5 * target_s:
-            n = int(max(n, min(2_000_000, n * target_s / max(el, 1e-3))))
+        n = int(max(n, min(2_000_000, n * target_s / max(el, 1e-3))))
+        times = []
+        for r in range(repeats):
             el, c = timed(event_based, n)
-            log("  cpu %s sample: %d histories in %.2f s" % (name, n, el))
-        modes[name] = {"particles_per_s": round(n / el, 2), "particles": n, "seconds": round(el, 2),
-                       "lookups_per_s": round(c["lookups"] / el, 1)}
+            times.append(el)
+            log("  cpu %s sample %d/%d: %d histories in %.2f s" % (name, r + 1, repeats, n, el))
+        best_t = min(times)
+        modes[name] = {"particles_per_s": round(n / best_t, 2), "particles": n, "seconds": round(best_t, 3),
+                       "seconds_all": [round(t, 3) for t in times],
+                       "lookups_per_s": round(c["lookups"] / best_t, 1)}
     best = max(modes, key=lambda k: modes[k]["particles_per_s"])
     return {"value": modes[best]["particles_per_s"], "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": "the oracle (C++ restatement, -O3, OpenMP %d threads) on the first %d histories of "
-                      "batch 1 of the same problem, %s-based (the faster of its two modes); %.1f s"
-                      % (threads, modes[best]["particles"], best, modes[best]["seconds"]),
+            "cpu": cpu_model(),
+            "sample": "the oracle (C++ restatement, -O3, OpenMP %d threads on %s) on the first %d histories of "
+                      "batch 1 of the same problem, %s-based (the faster of its two modes); best of %d runs, "
+                      "%.2f s" % (threads, cpu_model(), modes[best]["particles"], best, repeats,
+                                  modes[best]["seconds"]),
             "modes": modes}
 
 
-def run_problem_set(dirs, steps, warmup, options=None, tune_path=None):
+def run_problem_set(dirs, steps, warmup, options=None, tune_path=None, timed_steps=1):
     """The problems of a set in one shared event loop: W warm-up and K timed
-    batches (every problem advances one batch per step)."""
+    batches on the graph path (every problem advances one batch per step),
+    then `timed_steps` event-timed batches for the kernel table."""
     from openmc_amd import autotune, lib
     lib.init_problems(dirs, output=False)
     info = {}
@@ -231,7 +280,7 @@ def run_problem_set(dirs, steps, warmup, options=None, tune_path=None):
             best, rec = autotune.tune_cached(dirs, tune_path, log=log)
             info["autotune"] = {"options": best, "seconds": round(time.perf_counter() - t, 2),
                                 "trials": rec["trials"], "from_cache": bool(rec.get("cached"))}
-        lib.set_option("event_timing", 1)
+        lib.set_option("event_timing", 0)
         lib.simulation_init()
         for _ in range(warmup):
             lib.next_batch()
@@ -243,6 +292,15 @@ def run_problem_set(dirs, steps, warmup, options=None, tune_path=None):
             log("  batch %d done (%.2f s)" % (k + 1, time.perf_counter() - t0))
         t1 = time.perf_counter()
         s1 = lib.stats()
+        if timed_steps > 0:
+            lib.set_option("event_timing", 1)
+            lib.reset_timers()
+            s1b = lib.stats()
+            for k in range(timed_steps):
+                lib.next_batch()
+            s2 = lib.stats()
+            info["timed"] = {"stats": diff(s1b, s2), "steps": timed_steps,
+                             "config_stats": {k: s2.get(k, 0) for k in CONFIG_STATS}}
         info["options"] = {k: s1[k] for k in ("blocks", "block", "sort_every", "cycles_per_check", "sort_bits",
                                               "xs_group", "history_below", "fused_advance", "recompute_micro")}
         info["config_stats"] = {k: s1.get(k, 0) for k in CONFIG_STATS}
@@ -259,7 +317,7 @@ def cpu_baseline_set(problems, dirs, target_s=10.0):
     problems of a set in turn, until about target_s of CPU work."""
     from oracle.oracle import Oracle
     from oracle.problem import write_oracle_problem
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     done, secs, used = 0, 0.0, 0
     for (model, nucs), d in zip(problems, dirs):
         path = write_oracle_problem(model, nucs, d, data_dir=os.path.join("..", "data"))
@@ -274,6 +332,7 @@ def cpu_baseline_set(problems, dirs, target_s=10.0):
         if secs >= target_s:
             break
     return {"value": round(done / secs, 2), "unit": "particles/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": "the oracle (C++ restatement, -O3, OpenMP %d threads, history-based) on the first "
                       "min(n, 20000) histories of batch 1 of problems 0-%d of the set, one after another; "
                       "%d histories in %.1f s" % (threads, used - 1, done, secs)}
@@ -283,23 +342,25 @@ def bench_problem_set(args):
     """Batched uneven problems: 64 lattice problems from a seed in one event
     loop, against the same problems run one after another."""
     from openmc_amd import configs
-    n_batches = args.warmup + args.steps + 1
+    n_batches = args.warmup + args.steps + args.timed_steps + 1
     log("preparing the problem set")
     d, dirs, problems, nucs = configs.problem_set_dir(n_batches=max(n_batches, 2))
     opts = {k: float(v) for k, v in (o.split("=", 1) for o in args.option)}
     tune_path = None if args.no_autotune else os.path.join(d, "autotune.json")
     log("GPU problem set: %d problems, %d warm-up + %d timed batches" % (len(dirs), args.warmup, args.steps))
-    rs = run_problem_set(dirs, args.steps, args.warmup, options=opts, tune_path=tune_path)
+    rs = run_problem_set(dirs, args.steps, args.warmup, options=opts, tune_path=tune_path,
+                         timed_steps=args.timed_steps)
     st = rs["stats"]
+    tst = rs["timed"]["stats"]
     total = sum(rs["n_particles"])
     value = total * args.steps / rs["elapsed_s"]
-    dom, roof, per_kernel = roofline(dict(rs, stats=st))
+    dom, roof, per_kernel = roofline(rs["timed"])
     chosen = dict(opts)
     chosen.update(rs.get("autotune", {}).get("options", {}))
     log("GPU problems one at a time (1 warm-up + 1 timed batch each)")
     alone = []
     for k, dd in enumerate(dirs):
-        r = run_gpu(dd, 1, 1, options=chosen)
+        r = run_gpu(dd, 1, 1, options=chosen, timed_steps=0)
         alone.append({"particles": r["n_particles"], "ms_per_batch": round(1e3 * r["elapsed_s"], 3),
                       "particles_per_s": round(r["n_particles"] / r["elapsed_s"], 1)})
     seq_s = sum(a["ms_per_batch"] for a in alone) * 1e-3
@@ -319,7 +380,7 @@ def bench_problem_set(args):
         "config": {"workload": "batched-%d-lattice-problems-%.3gpps" % (len(dirs), total), "problems": len(dirs),
                    "particles_per_batch": total, "nuclides": len(nucs),
                    "mode": "event-based, one event loop shared by all problems"},
-        "xs_lookups_per_s": round(kxs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
+        "xs_lookups_per_s": round(kxs_lookups(tst) / tst["t_xs_ms"] * 1e3, 1),
         "sequential": {"particles_per_s": round(total / seq_s, 2), "seconds_per_batch": round(seq_s, 4)},
         "batching_speedup": round(value / (total / seq_s), 3),
         "per_problem": alone,
@@ -407,6 +468,61 @@ def pmc_summary(dirs, events=None, copy_bytes=None):
     return "\n".join(lines)
 
 
+WORKLOADS = {0: "cube-4nuclides-1e5pps-mesh10^3", 1: "lattice-17x17-60nuclides-1e7pps-mesh100^3",
+             2: "cube-1m-300nuclides-200kpoints-1e7pps-mesh100^3"}
+EVENT_KEYS = ("xs_events", "adv_events", "surf_events", "coll_events", "lookups", "collisions", "segments",
+              "search_probes", "cycles", "hist_launches", "hist_xs_events", "hist_lookups", "hist_adv_events",
+              "hist_segments", "hist_surf_events", "hist_coll_events", "scatter_fallthrough")
+
+
+def learned_summary(plain, lr, n, steps):
+    """The learned-index run next to the plain one (same inputs, same launch
+    options): whole-batch speedup on the graph path, lookups/s in the
+    calculate-xs kernel from the event-timed batches."""
+    st, lst = plain["stats"], lr["stats"]
+    out = {"particles_per_s": round(n * steps / lr["elapsed_s"], 2),
+           "speedup_vs_binary": round(plain["elapsed_s"] / lr["elapsed_s"], 4),
+           "probes_per_lookup": round(lst["search_probes"] / max(lst["lookups"], 1), 3),
+           "probes_per_lookup_binary": round(st["search_probes"] / max(st["lookups"], 1), 3),
+           "max_err": lr.get("learned_max_err"), "mean_err": lr.get("learned_mean_err"),
+           "segments": {"base": lr.get("learned_n_seg"), "min": lr.get("learned_min_segments"),
+                        "max": lr.get("learned_max_segments"), "total": lr.get("learned_total_segments")},
+           "build_s": round(lr.get("learned_build_s", 0), 3)}
+    if "timed" in lr and "timed" in plain:
+        lt, pt = lr["timed"]["stats"], plain["timed"]["stats"]
+        out["xs_lookups_per_s"] = round(kxs_lookups(lt) / lt["t_xs_ms"] * 1e3, 1)
+        out["xs_lookups_per_s_binary"] = round(kxs_lookups(pt) / pt["t_xs_ms"] * 1e3, 1)
+        out["k_xs_speedup_vs_binary"] = round(pt["t_xs_ms"] / lt["t_xs_ms"] * kxs_lookups(lt) / kxs_lookups(pt), 4)
+    return out
+
+
+def bench_lookup_workload(args, opts):
+    """The lookup-dominated workload the metric's comparison is about: 300
+    nuclides with 200k-point grids in one material (1 m cube, 1e7 particles
+    per batch), plain and learned, one warm-up and LOOKUP_STEPS timed batches
+    each on the graph path, one event-timed batch each for the kernel split."""
+    from openmc_amd import configs
+    steps = args.lookup_steps
+    log("preparing the lookup-dominated problem (300 nuclides x 200k points)")
+    d, model, nucs = configs.problem_dir(2, n_particles=args.particles, n_batches=1 + steps + 1 + 1)
+    log("GPU plain, lookup-dominated: 1 warm-up + %d timed batches (+1 event-timed)" % steps)
+    plain = run_gpu(d, steps, 1, options=opts, timed_steps=1)
+    log("GPU learned, lookup-dominated: 1 warm-up + %d timed batches (+1 event-timed)" % steps)
+    lr = run_gpu(d, steps, 1, learned=True, options=opts, timed_steps=1)
+    n = plain["n_particles"]
+    st = plain["stats"]
+    tst = plain["timed"]["stats"]
+    dom, roof, per_kernel = roofline(plain["timed"])
+    return {"workload": WORKLOADS[2], "particles_per_batch": n, "nuclides": len(nucs),
+            "grid_points": int(sum(x.n_points for x in nucs)), "steps": steps,
+            "particles_per_s": round(n * steps / plain["elapsed_s"], 2),
+            "xs_lookups_per_s": round(kxs_lookups(tst) / tst["t_xs_ms"] * 1e3, 1),
+            "xs_lookups_per_s_transport": round(xs_lookups(st) / plain["elapsed_s"], 1),
+            "learned": learned_summary(plain, lr, n, steps),
+            "roofline": roof, "kernels": per_kernel,
+            "events": {k: st.get(k, 0) for k in EVENT_KEYS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
@@ -419,6 +535,12 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--timed-steps", type=int, default=1,
+                    help="batches after the timed ones with per-kernel HIP event timers (kernel table, roofline)")
+    ap.add_argument("--no-lookup-workload", action="store_true",
+                    help="skip the 300-nuclide lookup-dominated comparison of plain and learned")
+    ap.add_argument("--lookup-steps", type=int, default=1,
+                    help="timed batches of the lookup-dominated comparison")
     ap.add_argument("--no-autotune", action="store_true",
                     help="with --config 3: keep the default launch options")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
@@ -454,19 +576,21 @@ def main():
         log("copy kernel: %d bytes in %.3f ms (%.0f GB/s read+write)" % (args.copy_calibration, ms,
                                                                           copy_info["GB_per_s"]))
     log("preparing problem (config %d, %d particles/batch)" % (args.config, args.particles))
-    kw = {"n_particles": args.particles, "n_batches": args.warmup + args.steps + 1}
+    kw = {"n_particles": args.particles, "n_batches": args.warmup + args.steps + args.timed_steps + 1}
     d, model, nucs = configs.problem_dir(args.config, **kw)
     from oracle.problem import write_oracle_problem
     if not args.no_cpu_baseline:
         write_oracle_problem(model, nucs, d)
 
-    log("GPU plain binary search: %d warm-up + %d timed batches" % (args.warmup, args.steps))
+    log("GPU plain binary search: %d warm-up + %d timed batches (+%d event-timed)" % (args.warmup, args.steps,
+                                                                                    args.timed_steps))
     opts = {k: float(v) for k, v in (o.split("=", 1) for o in args.option)}
-    plain = run_gpu(d, args.steps, args.warmup, options=opts)
-    dom, roof, per_kernel = roofline(plain)
+    plain = run_gpu(d, args.steps, args.warmup, options=opts, timed_steps=args.timed_steps)
+    dom, roof, per_kernel = roofline(plain["timed"])
     n = plain["n_particles"]
     value = n * args.steps / plain["elapsed_s"]
     st = plain["stats"]
+    tst = plain["timed"]["stats"]
     out = {
         "metric": "particles/s and XS lookups/s on 1 MI355X: learned-index vs binary search",
         "value": round(value, 2),
@@ -480,36 +604,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic nuclides from fixed seeds; random source histories",
-        "config": {"workload": "lattice-17x17-60nuclides-1e7pps-mesh100^3" if args.config == 1 else
-                   "config%d" % args.config, "particles_per_batch": n, "nuclides": len(nucs),
-                   "grid_points": int(sum(x.n_points for x in nucs)), "mode": "event-based"},
-        "xs_lookups_per_s": round(kxs_lookups(st) / st["t_xs_ms"] * 1e3, 1),
-        "xs_lookups_per_s_transport": round(xs_lookups(st) / (st["transport_ms"] * 1e-3), 1),
+        "config": {"workload": WORKLOADS.get(args.config, "config%d" % args.config), "particles_per_batch": n,
+                   "nuclides": len(nucs), "grid_points": int(sum(x.n_points for x in nucs)),
+                   "mode": "event-based, hipGraph-replayed event cycles"},
+        "xs_lookups_per_s": round(kxs_lookups(tst) / tst["t_xs_ms"] * 1e3, 1),
+        "xs_lookups_per_s_transport": round(xs_lookups(st) / plain["elapsed_s"], 1),
         "events_per_particle": round((st["xs_events"] + st["adv_events"] + st["surf_events"] +
                                       st["coll_events"]) / (n * args.steps), 2),
+        "kernel_timing": "per-kernel figures and the roofline from %d further batch(es) with a HIP event "
+                         "around every launch (direct launches); the headline from the %d graph-replayed "
+                         "batches" % (args.timed_steps, args.steps),
         "kernels": per_kernel,
-        "events": {k: st.get(k, 0) for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",
-                                             "collisions", "segments", "search_probes", "cycles", "hist_launches",
-                                             "hist_xs_events", "hist_lookups", "hist_adv_events",
-                                             "hist_segments", "hist_surf_events", "hist_coll_events")},
+        "events": {k: st.get(k, 0) for k in EVENT_KEYS},
         "roofline": roof,
     }
     if copy_info:
         out["copy_calibration"] = copy_info
     if not args.no_learned:
-        log("GPU learned index: %d warm-up + %d timed batches" % (args.warmup, args.steps))
-        lr = run_gpu(d, args.steps, args.warmup, learned=True, options=opts)
-        lst = lr["stats"]
-        out["learned"] = {
-            "particles_per_s": round(n * args.steps / lr["elapsed_s"], 2),
-            "xs_lookups_per_s": round(kxs_lookups(lst) / lst["t_xs_ms"] * 1e3, 1),
-            "speedup_vs_binary": round(plain["elapsed_s"] / lr["elapsed_s"], 4),
-            "probes_per_lookup": round(lst["search_probes"] / max(lst["lookups"], 1), 3),
-            "probes_per_lookup_binary": round(st["search_probes"] / max(st["lookups"], 1), 3),
-            "max_err": lr.get("learned_max_err"), "mean_err": lr.get("learned_mean_err"),
-            "build_s": round(lr.get("learned_build_s", 0), 3)}
+        log("GPU learned index: %d warm-up + %d timed batches (+%d event-timed)" % (args.warmup, args.steps,
+                                                                                  args.timed_steps))
+        lr = run_gpu(d, args.steps, args.warmup, learned=True, options=opts, timed_steps=args.timed_steps)
+        out["learned"] = learned_summary(plain, lr, n, args.steps)
+    if args.config == 1 and not args.no_lookup_workload:
+        out["lookup_dominated"] = bench_lookup_workload(args, opts)
     if not args.no_cpu_baseline:
-        log("CPU baseline (oracle, history-based)")
+        log("CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(d)
     print(json.dumps(out), flush=True)
 

@@ -332,6 +332,11 @@ int omc_lookup_batch(int i_nuc, const double* E, int64_t n, int learned, int* i_
 /* Macroscopic (total, absorption) of material index `mat` at n energies; the
  * GPU time of the kernel in *ms. */
 int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat, double* macro, double* ms);
+/* Reads dir's model as openmc_init does, without a GPU: the model getters
+ * (cells, materials, meshes, filters, tallies, weight windows) answer from
+ * it and transport calls return OPENMC_E_ALLOCATE; openmc_finalize releases
+ * it.  No reference counterpart (model inspection where no GPU is present). */
+int omc_load_model_host(const char* dir);
 /* Host-only model check (no GPU needed): reads dir/model.xml and its data as
  * openmc_init does and runs initialize_data; writes counts, energy grid data
  * and normalized atom densities to out[n_out] and the first n_gi entries of

@@ -39,8 +39,10 @@ def _nuclide_set(prefix, count, seed, grid_lo, grid_hi, n_res=30, pts_per_res=40
                                    n_res=n_res, pts_per_res=pts_per_res) for k in range(count)]
 
 
-def config0(n_particles=100000, n_batches=10, n_grid=10000, mesh=(10, 10, 10), event_based=True):
-    """20 cm cube, one material of 4 nuclides, 1 MeV point source, 10^3 flux mesh."""
+def config0(n_particles=100000, n_batches=10, n_grid=10000, mesh=(10, 10, 10), event_based=True,
+            energy_bins=None):
+    """20 cm cube, one material of 4 nuclides, 1 MeV point source, 10^3 flux mesh
+    (mesh None: a tally with no mesh filter; energy_bins: an energy filter)."""
     om.reset_auto_ids()
     nucs = [data.synthetic_nuclide("N%d" % k, DATA_SEED_BASE + k, n_grid=n_grid) for k in range(4)]
     rng = np.random.default_rng(DATA_SEED_BASE)
@@ -60,12 +62,15 @@ def config0(n_particles=100000, n_batches=10, n_grid=10000, mesh=(10, 10, 10), e
     s.event_based = event_based
     s.source = om.IndependentSource(space=stats.Point((0.0, 0.0, 0.0)), angle=stats.Isotropic(),
                                     energy=stats.Discrete([1.0e6], [1.0]))
-    m = om.RegularMesh()
-    m.dimension = mesh
-    m.lower_left = (-10.0, -10.0, -10.0)
-    m.upper_right = (10.0, 10.0, 10.0)
     t = om.Tally(name="flux")
-    t.filters = [om.MeshFilter(m)]
+    if mesh is not None:
+        m = om.RegularMesh()
+        m.dimension = mesh
+        m.lower_left = (-10.0, -10.0, -10.0)
+        m.upper_right = (10.0, 10.0, 10.0)
+        t.filters = [om.MeshFilter(m)]
+    if energy_bins is not None:
+        t.filters.append(om.EnergyFilter(energy_bins))
     t.scores = ["flux"]
     model = om.Model(geom, om.Materials([mat]), s, om.Tallies([t]))
     return model, nucs

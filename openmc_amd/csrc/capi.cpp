@@ -92,6 +92,7 @@ struct KernelTimer {
 struct Sim {
   HostModel m;
   bool initialized = false;
+  bool host_only = false; // omc_load_model_host: the model without a GPU (queries only)
   bool sim_initialized = false;
   DevProblem P {};
   DevParticles S {};
@@ -164,6 +165,7 @@ struct Sim {
 };
 
 Sim* g = nullptr;
+long long g_generation = 0; // bumped whenever a model is loaded or released
 
 template<class T>
 T* dev_upload(Sim& s, const std::vector<T>& v)
@@ -395,6 +397,7 @@ void upload_problem(Sim& s)
     T.n_energy = t.ebins.empty() ? 0 : static_cast<int>(t.ebins.size()) - 1;
     T.ebins = dev_upload(s, t.ebins);
     T.n_bins = t.n_filter_bins;
+    T.has_mesh = t.has_mesh ? 1 : 0;
     T.value = nullptr; // set by set_tally_copies
     T.n_copies = 1;
     return T;
@@ -1121,9 +1124,14 @@ const HostModel& model_state()
   return g->m;
 }
 
+long long api_generation()
+{
+  return g_generation;
+}
+
 HostModel* api_model()
 {
-  return g && g->initialized ? &g->m : nullptr;
+  return g && (g->initialized || g->host_only) ? &g->m : nullptr;
 }
 
 bool api_simulation_running()
@@ -1178,12 +1186,44 @@ int openmc_init(int argc, char* argv[], const void* intracomm)
     }
     delete g;
     g = new Sim();
+    ++g_generation;
     g->path_input = dir_given ? dir : "";
     const auto t0 = std::chrono::steady_clock::now();
     read_model(g->m, dir);
     g->t_read_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     finish_init(*g);
     g->t_init_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  });
+}
+
+// The model of a directory read as openmc_init reads it, with no GPU: the
+// model getters answer from it, transport calls fail.  No reference
+// counterpart; for inspecting a model (and the tests) where no GPU is present.
+int omc_load_model_host(const char* dir)
+{
+  return guarded([&]() -> int {
+    if (g && g->initialized)
+      openmc_finalize();
+    delete g;
+    g = new Sim();
+    ++g_generation;
+    g->host_only = true;
+    g->path_input = dir ? dir : "";
+    try {
+      read_model(g->m, dir ? dir : ".");
+      initialize_data(g->m);
+    } catch (...) {
+      delete g;
+      g = nullptr;
+      throw;
+    }
+    n_particles = g->m.n_particles;
+    n_inactive = g->m.n_inactive;
+    n_coord_levels = g->m.n_levels;
+    weight_windows_on = g->m.ww.on;
+    current_batch = 0;
+    n_realizations = 0;
     return 0;
   });
 }
@@ -1204,6 +1244,7 @@ int omc_init_problems(int n, const char* const* dirs)
       d.push_back(dirs[k] ? dirs[k] : ".");
     delete g;
     g = new Sim();
+    ++g_generation;
     const auto t0 = std::chrono::steady_clock::now();
     read_problem_set(g->m, d);
     g->t_read_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1216,6 +1257,10 @@ int omc_init_problems(int n, const char* const* dirs)
 
 int openmc_simulation_init(void)
 {
+  if (g && g->host_only) {
+    set_errmsg("the model was read without a GPU (omc_load_model_host); transport needs openmc_init on a GPU");
+    return OPENMC_E_ALLOCATE;
+  }
   if (!g || !g->initialized) {
     set_errmsg("OpenMC has not been initialized.");
     return OPENMC_E_ALLOCATE;
@@ -1364,6 +1409,7 @@ int openmc_finalize(void)
   }
   delete g;
   g = nullptr;
+  ++g_generation;
   current_batch = 0;
   n_realizations = 0;
   return 0;
@@ -1655,7 +1701,7 @@ static const char* k_stat_names =
   "t_coll_ms,t_init_ms,t_hist_ms,n_sort,n_xs,n_adv,n_split,n_surf,n_coll,n_init,n_hist,wall_s,blocks,xs_lanes,"
   "sort,xs_group,block,sort_every,cycles_per_check,sort_bits,sort_blocks,history_below,history_steps,"
   "hist_launches,hist_xs_events,hist_lookups,hist_adv_events,hist_segments,hist_surf_events,hist_coll_events,"
-  "fused_advance,recompute_micro";
+  "fused_advance,recompute_micro,scatter_fallthrough";
 
 const char* omc_stat_names(void)
 {
@@ -1702,6 +1748,7 @@ int omc_get_stats(double* out, int n)
     v.push_back(static_cast<double>(g->counters[c]));
   v.push_back(g->lc.fuse_advance && g->lc.xs_lanes == 1 && (g->lc.xs_group == 4 || g->lc.xs_group == 2) ? 1.0 : 0.0);
   v.push_back(g->S.recompute_micro);
+  v.push_back(static_cast<double>(g->counters[CNT_SCATTER_FALLTHROUGH]));
   for (int k = 0; k < n && k < static_cast<int>(v.size()); ++k)
     out[k] = v[k];
   return static_cast<int>(v.size());

@@ -10,6 +10,9 @@ namespace omc {
 
 // The loaded model, or null before openmc_init / after openmc_finalize.
 HostModel* api_model();
+// Changes whenever a model is loaded or released (tables derived from the
+// model are rebuilt when it changes).
+long long api_generation();
 // Whether openmc_simulation_init has run and openmc_simulation_finalize not yet.
 bool api_simulation_running();
 // Zero one tally's SUM/SUM_SQ (device and host mirror) and restart its

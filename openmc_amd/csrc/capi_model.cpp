@@ -49,6 +49,7 @@ namespace {
 // reference's borrowed pointers do.
 struct ApiTables {
   const HostModel* model = nullptr;
+  long long generation = -1;
   size_t n_tallies = 0;
   // filters: the model's own list, or the concatenation of a problem set's
   std::vector<const HostFilter*> filters;
@@ -117,9 +118,10 @@ int32_t universe_count(const HostModel& m, int u, int target, std::unordered_map
 const HostModel* model_or_null()
 {
   const HostModel* m = api_model();
-  if (m != T.model || (m && T.n_tallies != m->tallies.size())) {
+  if (api_generation() != T.generation || m != T.model || (m && T.n_tallies != m->tallies.size())) {
     T = ApiTables();
     T.model = m;
+    T.generation = api_generation();
     if (!m)
       return nullptr;
     T.n_tallies = m->tallies.size();
@@ -175,6 +177,8 @@ const HostModel* model_or_null()
     }
     T.mat_volume.assign(n_mat, -1.0);
     T.mat_depletable.assign(n_mat, false);
+    for (size_t i = 0; i < n_mat && i < m->mat_depletable.size(); ++i)
+      T.mat_depletable[i] = m->mat_depletable[i];
     for (const DevCell& c : m->cells)
       T.cell_fill.push_back(c.fill_type == FILL_MATERIAL ? c.material : c.fill);
     std::unordered_map<int, std::unordered_map<int, int32_t>> memo;

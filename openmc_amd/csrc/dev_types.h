@@ -156,7 +156,7 @@ struct DevTally {
   double* value;       // [n_copies][n_bins] VALUE accumulator for the current batch
   int64_t n_bins;
   int n_copies;        // 1, or one per XCD (a power of two)
-  int pad_copies;
+  int has_mesh;        // filters [mesh] / [mesh, energy]; else [] or [energy]
 };
 
 // Mesh weight windows for neutrons (src/weight_windows.cpp:56-146, 375-411)
@@ -267,6 +267,9 @@ enum Counter : int {
   CNT_HIST_SEGMENTS,
   CNT_HIST_SURF_EVENTS,
   CNT_HIST_COLL_EVENTS,
+  // scatters whose interpolated elastic cross section did not exceed the
+  // sampled cutoff (src/physics.cpp:690-735 would leave the elastic branch)
+  CNT_SCATTER_FALLTHROUGH,
   CNT_N
 };
 

@@ -1202,7 +1202,7 @@ __device__ __forceinline__ void tally_add(const DevTally& T, int64_t bin, double
   unsafeAtomicAdd(&v[bin], score);
 }
 
-// score_tracklength_tally with [mesh] or [mesh, energy] filters
+// score_tracklength_tally with [mesh], [mesh, energy], [energy] or no filters
 // (src/tallies/tally_scoring.cpp:2416-2541); returns segments scored
 __device__ __forceinline__ int score_tracklength(const DevTally& T, V3 r0, V3 r1, V3 u, double E, double wgt,
   double distance)
@@ -1222,6 +1222,11 @@ __device__ __forceinline__ int score_tracklength(const DevTally& T, V3 r0, V3 r1
       e_bin = lower_bound_idx(eb, 0, nb, E, dummy) - 1;
     }
     n_e = T.n_energy;
+  }
+  if (!T.has_mesh) {
+    // filters [] or [energy]: one bin per energy group, filter weight 1
+    tally_add(T, e_bin, flux * 1.0);
+    return 1;
   }
   int segs = 0;
   double total_distance = v_norm(v_sub(r1, r0));

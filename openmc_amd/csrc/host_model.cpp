@@ -279,12 +279,16 @@ void read_weight_windows(HostModel& m, const XmlNode& s)
 
 void read_settings(HostModel& m, const XmlNode& s)
 {
+  if (s.has("photon_transport") && to_bool(s.value("photon_transport")))
+    throw std::runtime_error("photon transport is outside the GPU path (neutrons only)");
   read_weight_windows(m, s);
   if (s.has("run_mode")) {
     m.run_mode = s.value("run_mode");
     if (m.run_mode != "fixed source")
       throw std::runtime_error("run mode '" + m.run_mode + "' is outside the GPU path (fixed source only)");
   }
+  if (s.has("temperature_default"))
+    m.temperature_default = std::stod(s.value("temperature_default"));
   m.n_particles = std::stoll(s.value("particles"));
   m.n_batches = std::stoi(s.value("batches"));
   // statepoints: the listed batches, else the last one
@@ -324,6 +328,8 @@ void read_settings(HostModel& m, const XmlNode& s)
   const XmlNode& src = *srcs[0];
   if (src.attr("type") && *src.attr("type") != "independent")
     throw std::runtime_error("only independent sources are supported by the GPU path");
+  if (src.attr("particle") && *src.attr("particle") != "neutron")
+    throw std::runtime_error("source particle '" + *src.attr("particle") + "' is outside the GPU path (neutrons only)");
   DevSource& d = m.src;
   std::memset(&d, 0, sizeof(d));
   if (const XmlNode* sp = src.child("space")) {
@@ -416,6 +422,13 @@ void read_materials(HostModel& m, const XmlNode& mats, const std::string& model_
     RawMat r {};
     m.mat_ids.push_back(std::stoi(mn->value("id")));
     m.mat_names.push_back(mn->attr("name") ? *mn->attr("name") : std::string());
+    m.mat_depletable.push_back(mn->attr("depletable") && to_bool(*mn->attr("depletable")));
+    if (mn->child("sab"))
+      throw std::runtime_error("material " + mn->value("id") +
+                               ": thermal scattering data (S(a,b)) are outside the GPU path");
+    if (mn->child("macroscopic") || mn->child("element"))
+      throw std::runtime_error("material " + mn->value("id") +
+                               ": only <nuclide> constituents are supported by the GPU path");
     const XmlNode* dn = mn->child("density");
     if (!dn)
       throw std::runtime_error("material without <density>");
@@ -850,8 +863,8 @@ void read_tallies(HostModel& m, const XmlNode& t)
         ht.filter_ids.push_back(static_cast<int>(fid));
         ht.filter_types.push_back(f.type);
         if (f.type == "mesh") {
-          if (n_mesh++ || !ht.filter_types.empty() && ht.filter_types.size() != 1)
-            throw std::runtime_error("the GPU path supports filters [mesh] or [mesh, energy]");
+          if (n_mesh++ || ht.filter_types.size() != 1)
+            throw std::runtime_error("the GPU path supports filters [], [energy], [mesh] or [mesh, energy]");
           auto mt = meshes.find(f.mesh);
           if (mt == meshes.end())
             throw std::runtime_error("mesh filter refers to unknown mesh");
@@ -870,15 +883,14 @@ void read_tallies(HostModel& m, const XmlNode& t)
             nb *= me.shape[k];
           ht.filter_nbins.push_back(static_cast<int>(nb));
         } else {
-          if (n_energy++ || n_mesh != 1)
-            throw std::runtime_error("the GPU path supports filters [mesh] or [mesh, energy]");
+          // [energy] alone, or after the mesh filter
+          if (n_energy++ || ht.filter_types.size() != static_cast<size_t>(n_mesh) + 1)
+            throw std::runtime_error("the GPU path supports filters [], [energy], [mesh] or [mesh, energy]");
           ht.ebins = f.bins;
           ht.filter_nbins.push_back(static_cast<int>(f.bins.size()) - 1);
         }
       }
     }
-    if (!ht.has_mesh)
-      throw std::runtime_error("the GPU path needs a mesh filter on every tally");
     ht.n_filter_bins = 1;
     for (int nb : ht.filter_nbins)
       ht.n_filter_bins *= nb;
@@ -1038,6 +1050,7 @@ void read_problem_set(HostModel& out, const std::vector<std::string>& dirs)
     for (size_t i = 0; i < m.mat_ids.size(); ++i) {
       out.mat_ids.push_back(m.mat_ids[i]);
       out.mat_names.push_back(m.mat_names[i]);
+      out.mat_depletable.push_back(m.mat_depletable[i]);
       for (int e = m.mat_off[i]; e < m.mat_off[i + 1]; ++e) {
         out.mat_nuc.push_back(nuc_map[m.mat_nuc[e]]);
         out.mat_nuc_local.push_back(m.mat_nuc[e]);

@@ -124,6 +124,7 @@ struct HostModel {
 
   std::vector<int> mat_ids;
   std::vector<std::string> mat_names; // the name attribute ("" when absent)
+  std::vector<bool> mat_depletable;   // the depletable attribute
   std::vector<int> mat_off;
   std::vector<int> mat_nuc;
   std::vector<double> mat_dens;

@@ -12,9 +12,10 @@
 //   k_lforward   loss_s = sum_i (pred(f_i) - i)^2 over the segment's grid points
 //   k_lbackward  d loss_s / d(a_s, b_s) = sum_i 2 r_i (f_i, 1)
 //   k_ladam      Adam step on the mean gradient
-//   k_lerror     recentre b_s on the residual range, then record
-//                err_s = ceil(max_i |pred(f_i) - i|) over the segment's points
-//                and one neighbour on each side, with the FP32 inference formula.
+//   k_lerror     recentre b_s on the residual range, then record err_s, the
+//                largest |pred - lower bound| any query of the segment can
+//                have, from the sub-interval ends (f = 0, each point, f = 1)
+//                with the FP32 inference formula.
 // The lookup then searches E[j - err - 1, j + err + 3) exactly (j = (int)pred),
 // which contains the global lower_bound of any E that maps to segment s.
 #include <hip/hip_runtime.h>
@@ -213,7 +214,16 @@ __global__ void k_ladam(const int* seg_start, int n_seg, const double2* grad, fl
   params[j] = p;
 }
 
-// Recentre and record the error bound over points [a0 - 1, a1] of segment s.
+// Recentre the segment's intercept and record its error bound.
+//
+// A query in segment s has f in [0, 1); with the segment's points a0..a1-1
+// at f_a0 <= ... <= f_{a1-1}, its lower bound lb (first point >= E) is k on
+// the sub-interval (f_{k-1}, f_k] (f_{a0-1} read as 0, f_{a1} as 1).  The FP32
+// prediction is monotone in f, so max |pred - lb| over the segment is reached
+// at sub-interval ends:  pred(0) - a0,  pred(f_i) - i and pred(f_i) - (i + 1)
+// for each point i, and pred(1) - a1.  err is the ceiling of the largest of
+// those after recentring, so every query of the segment has |pred - lb| <= err
+// and the window [j - err - 1, j + err + 3) (j = (int)pred) holds lb.
 __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, const double* uu,
   const int* seg_start, int n_seg, float2* params, int* err)
 {
@@ -223,33 +233,34 @@ __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, co
   const DevNuclide nd = P.nuc[n];
   const double* u = uu + nd.off;
   const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
-  int a0 = seg_start[k] - 1, a1 = seg_start[k + 1] + 1;
-  a0 = a0 < 0 ? 0 : a0;
-  a1 = a1 > nd.n ? nd.n : a1;
+  const int a0 = seg_start[k], a1 = seg_start[k + 1];
   const int64_t j = static_cast<int64_t>(y) * n_seg + s;
   float2 ab = params[j];
-  double rmin = 1e300, rmax = -1e300;
-  for (int i = a0 + threadIdx.x; i < a1; i += LW) {
-    double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
-    rmin = fmin(rmin, r);
-    rmax = fmax(rmax, r);
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    rmin = fmin(rmin, __shfl_xor(rmin, o));
-    rmax = fmax(rmax, __shfl_xor(rmax, o));
-  }
-  if (rmax >= rmin)
-    ab.y = static_cast<float>(static_cast<double>(ab.y) - 0.5 * (rmax + rmin));
-  double amax = 0.0;
-  for (int i = a0 + threadIdx.x; i < a1; i += LW) {
-    double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
-    amax = fmax(amax, fabs(r));
-  }
-  for (int o = 32; o > 0; o >>= 1)
-    amax = fmax(amax, __shfl_xor(amax, o));
-  if (threadIdx.x == 0) {
-    params[j] = ab;
-    err[j] = static_cast<int>(ceil(amax));
+  const double us = static_cast<double>(s);
+  for (int pass = 0; pass < 2; ++pass) {
+    // pass 0: residual range, then recentre; pass 1: the bound
+    double rmin = 1e300, rmax = -1e300;
+    for (int i = a0 + threadIdx.x; i < a1; i += LW) {
+      const double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
+      rmin = fmin(rmin, r - 1.0);
+      rmax = fmax(rmax, r);
+    }
+    if (threadIdx.x == 0) {
+      const double r0 = static_cast<double>(predict(ab, us, s)) - static_cast<double>(a0);
+      const double r1 = static_cast<double>(predict(ab, us + 1.0, s)) - static_cast<double>(a1);
+      rmin = fmin(rmin, fmin(r0, r1));
+      rmax = fmax(rmax, fmax(r0, r1));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      rmin = fmin(rmin, __shfl_xor(rmin, o));
+      rmax = fmax(rmax, __shfl_xor(rmax, o));
+    }
+    if (pass == 0) {
+      ab.y = static_cast<float>(static_cast<double>(ab.y) - 0.5 * (rmax + rmin));
+    } else if (threadIdx.x == 0) {
+      params[j] = ab;
+      err[j] = static_cast<int>(ceil(fmax(fabs(rmin), fabs(rmax))));
+    }
   }
 }
 

@@ -55,6 +55,7 @@ void launch_lookup_batch(const DevProblem& P, int i_nuc, const double* E, int64_
 void launch_macro_batch(const DevProblem& P, int material, const double* E, int64_t n, int learned,
   double* macro, unsigned long long* probes, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n_bytes, hipStream_t st);
+
 void launch_math_batch(int fn, const double* x, const double* y, int64_t n, double* out, hipStream_t st);
 
 } // namespace omc

@@ -764,7 +764,7 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
 
 // Event counts of one lane, summed per wavefront into the counter slots.
 struct EvCount {
-  unsigned long long events = 0, lost = 0, segs = 0, lookups = 0, probes = 0;
+  unsigned long long events = 0, lost = 0, segs = 0, lookups = 0, probes = 0, fallthrough = 0;
 };
 
 // Particle::event_advance for slot p (src/particle.cpp:232-286): distance to
@@ -1268,10 +1268,18 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
       }
     }
     if (!absorbed) {
-      // scatter (src/physics.cpp:674-752): the data hold elastic scattering only
-      double cut = prn(&seed) * (m.total - m.absorption);
-      (void)cut;
+      // scatter (src/physics.cpp:674-752): the data hold elastic scattering
+      // only.  Nuclide::calculate_elastic_xs (src/nuclide.cpp:578-590) at the
+      // sampled nuclide's grid index and interpolation factor decides the
+      // branch; with sigma_t = sigma_el + sigma_c in the data it exceeds the
+      // cutoff except for a cutoff within rounding of sigma_t - sigma_a, a
+      // case the reference sends to its inelastic loop with no inelastic
+      // reactions -- counted here and scattered elastically.
+      const double cut = prn(&seed) * (m.total - m.absorption);
       const DevNuclide nd = P.nuc[i_nuclide];
+      const double elastic = (1.0 - m.f) * P.EL[nd.off + m.i_grid] + m.f * P.EL[nd.off + m.i_grid + 1];
+      if (!(elastic > cut))
+        c.fallthrough++;
       const double awr = nd.awr;
       const double kT = nd.kT;
       // elastic_scatter (src/physics.cpp:754-818)
@@ -1363,6 +1371,7 @@ __global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles 
   wave_count(S, CNT_LOST, c.lost);
   wave_count(S, CNT_LOOKUPS, c.lookups);
   wave_count(S, CNT_SEARCH_PROBES, c.probes);
+  wave_count(S, CNT_SCATTER_FALLTHROUGH, c.fallthrough);
 }
 
 //==============================================================================
@@ -1485,6 +1494,7 @@ __global__ void __launch_bounds__(BLOCK) k_history(DevProblem P, DevParticles S,
   wave_count(S, CNT_HIST_SEGMENTS, ca.segs);
   wave_count(S, CNT_HIST_SURF_EVENTS, cs.events);
   wave_count(S, CNT_HIST_COLL_EVENTS, cc.events);
+  wave_count(S, CNT_SCATTER_FALLTHROUGH, cc.fallthrough);
   for (int o = 32; o > 0; o >>= 1) {
     leak += __shfl_xor(leak, o);
     live += __shfl_xor(live, o);

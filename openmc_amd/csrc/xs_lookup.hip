@@ -7,6 +7,9 @@
 // absorption), i.e. Material::calculate_neutron_xs (src/material.cpp:829-901).
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+#include <string>
+
 #include "device_physics.h"
 #include "transport.h"
 

@@ -170,6 +170,7 @@ _bind("omc_math_batch", [c_int, POINTER(c_double), POINTER(c_double), c_int64, P
 _bind("omc_copy_kernel", [c_int64, c_int, POINTER(c_double)])
 _bind("omc_check_model", [c_char_p, POINTER(c_double), c_int, POINTER(c_int), c_int64], c_int, check=False)
 _bind("omc_init_problems", [c_int, POINTER(c_char_p)])
+_bind("omc_load_model_host", [c_char_p])
 _bind("omc_n_problems", [], c_int, check=False)
 _bind("omc_problem_info", [c_int, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)])
 _bind("omc_problem_global_tallies", [c_int, POINTER(POINTER(c_double))])
@@ -248,6 +249,13 @@ class _Settings:
 
 
 settings = _Settings()
+
+
+def load_model_host(directory="."):
+    """Read a model directory as init() does, without a GPU: the model
+    queries (cells(), materials(), meshes(), filters(), tallies) work,
+    transport does not; finalize() releases it."""
+    _dll.omc_load_model_host(str(directory).encode())
 
 
 def init(args=None, intracomm=None, output=True):

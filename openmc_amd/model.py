@@ -535,7 +535,8 @@ class Tally:
         el.set("id", str(self.id))
         if self.name:
             el.set("name", self.name)
-        ET.SubElement(el, "filters").text = " ".join(str(f.id) for f in self.filters)
+        if self.filters:
+            ET.SubElement(el, "filters").text = " ".join(str(f.id) for f in self.filters)
         ET.SubElement(el, "scores").text = " ".join(self.scores)
         ET.SubElement(el, "estimator").text = self.estimator
         return el

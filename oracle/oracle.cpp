@@ -545,6 +545,7 @@ struct Problem {
 
   // tally
   bool has_tally = false;
+  bool has_mesh = false; // tally filters [mesh] / [mesh, energy]; else [] or [energy]
   Mesh mesh;
   std::vector<double> energy_bins; // empty -> no energy filter
   int64_t n_tally_bins = 0;
@@ -598,6 +599,7 @@ struct Boundary {
 struct Counters {
   int64_t lookups = 0, collisions = 0, events = 0, segments = 0, lost = 0;
   int64_t xs_events = 0, advance_events = 0, surface_events = 0, collision_events = 0;
+  int64_t fallthrough = 0;  // scatters whose elastic cross section did not exceed the cutoff
   double tracklength = 0.0; // sum of wgt * distance over all advances
 };
 
@@ -1117,6 +1119,14 @@ void score_tracklength_tally(Sim& S, Particle& p, double distance, Counters& cnt
                                P.energy_bins.begin()) - 1;
     }
     n_e = static_cast<int>(P.energy_bins.size()) - 1;
+  }
+  if (!P.has_mesh) {
+    // filters [] or [energy]: one bin per energy group, filter weight 1
+    double score = flux * 1.0;
+#pragma omp atomic
+    S.tally_value[e_bin] += score;
+    cnt.segments++;
+    return;
   }
   // filters = [mesh, energy]; strides: energy 1, mesh n_e (src/tallies/tally.cpp:503-516)
   raytrace_mesh(P.mesh, p.r_last, p.r(), p.u(), [&](const int* ijk, double w) {
@@ -1653,10 +1663,12 @@ void event_collide(Sim& S, Particle& p, Counters& cnt)
     if (micro.elastic == CACHE_INVALID)
       calculate_elastic_xs(P, i_nuclide, p);
     double pr = micro.elastic - 0.0;
-    // The data hold no inelastic reactions, so every scatter is elastic
-    // (the reference would otherwise take its empty inelastic loop).
-    (void)pr;
-    (void)cut;
+    // The data hold no inelastic reactions.  sigma_el exceeds the cutoff
+    // except for a cutoff within rounding of sigma_t - sigma_a, where the
+    // reference would take its inelastic loop with no inelastic reaction:
+    // counted, and scattered elastically.
+    if (!(pr > cut))
+      cnt.fallthrough++;
     double kT = P.nuclides[i_nuclide].kT;
     elastic_scatter(P, i_nuclide, kT, p);
   }
@@ -1734,6 +1746,7 @@ void transport_history_based(Sim& S, int64_t n)
     {
       S.cnt.lookups += c.lookups;
       S.cnt.collisions += c.collisions;
+      S.cnt.fallthrough += c.fallthrough;
       S.cnt.segments += c.segments;
       S.cnt.xs_events += c.xs_events;
       S.cnt.advance_events += c.advance_events;
@@ -1822,6 +1835,7 @@ void transport_event_based(Sim& S, int64_t n_total)
     }
     S.cnt.lookups += c.lookups;
     S.cnt.collisions += c.collisions;
+    S.cnt.fallthrough += c.fallthrough;
     S.cnt.segments += c.segments;
     S.cnt.xs_events += c.xs_events;
     S.cnt.advance_events += c.advance_events;
@@ -2180,6 +2194,7 @@ void load_problem(Problem& P, const std::string& path)
       m.ur[j] = t.d();
     for (int j = 0; j < m.n_dim; ++j)
       m.width[j] = (m.ur[j] - m.ll[j]) / static_cast<double>(m.shape[j]);
+    P.has_mesh = m.n_dim > 0;
     int ne = static_cast<int>(t.i());
     for (int j = 0; j < ne; ++j)
       P.energy_bins.push_back(t.d());
@@ -2300,6 +2315,7 @@ void oracle_counters(void* h, int64_t* out)
   out[5] = c.advance_events;
   out[6] = c.surface_events;
   out[7] = c.collision_events;
+  out[8] = c.fallthrough;
 }
 
 void oracle_leakage(void* h, double* out)

@@ -158,10 +158,10 @@ class Oracle:
         return self.lib.oracle_n_realizations(self.h)
 
     def counters(self):
-        out = (c_int64 * 8)()
+        out = (c_int64 * 9)()
         self.lib.oracle_counters(self.h, out)
         keys = ["lookups", "collisions", "segments", "lost", "xs_events", "adv_events", "surf_events",
-                "coll_events"]
+                "coll_events", "scatter_fallthrough"]
         return dict(zip(keys, list(out)))
 
     def leakage(self):

@@ -112,13 +112,16 @@ def write_oracle_problem(model, nuclides, directory, data_dir="data"):
     if tally is None:
         out += ["tally", "0"]
     else:
-        mesh = [f for f in tally.filters if isinstance(f, om.MeshFilter)][0].mesh
+        meshes = [f.mesh for f in tally.filters if isinstance(f, om.MeshFilter)]
         ef = [f for f in tally.filters if isinstance(f, om.EnergyFilter)]
-        dim = list(mesh.dimension)
-        nd = len(dim)
-        shape = dim + [1] * (3 - nd)
-        ll = list(mesh.lower_left) + [0.0] * (3 - nd)
-        ur = list(mesh.upper_right) + [0.0] * (3 - nd)
+        if meshes:
+            dim = list(meshes[0].dimension)
+            nd = len(dim)
+            shape = dim + [1] * (3 - nd)
+            ll = list(meshes[0].lower_left) + [0.0] * (3 - nd)
+            ur = list(meshes[0].upper_right) + [0.0] * (3 - nd)
+        else:  # no mesh filter: n_dim 0
+            nd, shape, ll, ur = 0, [1, 1, 1], [0.0] * 3, [0.0] * 3
         out += ["tally", "1", str(nd)] + [str(v) for v in shape] + [_f(v) for v in ll] + [_f(v) for v in ur]
         if ef:
             out += [str(len(ef[0].values))] + [_f(v) for v in ef[0].values]

@@ -84,6 +84,7 @@ def test_cfg2_transport_histories(cfg2, learned):
     assert_tallies_close(g["results"], o.results())
     c = o.counters()
     assert g["stats"]["collisions"] == c["collisions"] and g["stats"]["segments"] == c["segments"]
+    assert g["stats"]["scatter_fallthrough"] == c["scatter_fallthrough"]
 
 
 def test_cfg1_full_data_histories():

@@ -105,6 +105,27 @@ def test_transport_cfg0_histories_and_tallies(mode):
     assert s["collisions"] == c["collisions"]
     assert s["segments"] == c["segments"]
     assert s["surf_events"] == c["surf_events"]
+    # the elastic cross section is evaluated at every scatter (A12); the
+    # inelastic fallthrough needs a cutoff within rounding of sigma_s
+    assert s["scatter_fallthrough"] == c["scatter_fallthrough"] == 0
+
+
+@pytest.mark.parametrize("filters", ["none", "energy"])
+def test_transport_tally_without_mesh_filter(filters):
+    """Tallies with no filter (one bin) or an energy filter alone, as the
+    reference's fixed_source regression model uses: histories bit-identical,
+    tallies within 1e-12 per bin."""
+    print("case: cfg0 tally filters %s" % filters, flush=True)
+    kw = dict(n_particles=2000, n_batches=2, mesh=None)
+    if filters == "energy":
+        kw["energy_bins"] = [1e-5, 1.0, 1e3, 1e5, 2e7]
+    d, model, nucs = prepare(0, **kw)
+    o = oracle_run(d)
+    g = gpu_run(d, options={"trace_hash": 1})
+    assert g["results"].shape[0] == (1 if filters == "none" else 4)
+    assert np.array_equal(g["hashes"], o.hashes(2000))
+    assert_tallies_close(g["results"], o.results())
+    assert g["stats"]["segments"] == o.counters()["segments"]
 
 
 def test_transport_cfg1_lattice_energy_filter():
@@ -275,6 +296,19 @@ def test_learned_error_bound_enforced_by_doubling(lib):
                 ig_l, xs_l, probes = lib.lookup_batch(i_nuc, E, learned=True)
                 assert np.array_equal(ig_o, ig_l)
                 assert np.array_equal(xs_o[:, 1:3].view(np.uint64), xs_l.view(np.uint64))
+                # the recorded bound holds for every query: |pred - lower bound| <= err
+                # of its segment, with the prediction in FP32 as the lookup makes it
+                Ein = E[(E >= pts[0]) & (E <= pts[-1])]
+                lb = np.searchsorted(pts, Ein, side="left")
+                ol = _oracle_lib()
+                x = np.array([ol.oracle_log(float(v / e_min)) for v in Ein])
+                u_lo = ol.oracle_log(float(pts[0] / e_min))
+                inv_w = n_s / (ol.oracle_log(float(pts[-1] / e_min)) - u_lo)
+                uu = (x - u_lo) * inv_w
+                seg = np.clip(np.where(uu < 0, 0, np.floor(uu)).astype(np.int64), 0, n_s - 1)
+                f = (uu - seg).astype(np.float32)
+                pred = (p[seg, 0] * f).astype(np.float32) + p[seg, 1]
+                assert np.all(np.abs(pred.astype(np.float64) - lb) <= e[seg] + 1e-9)
             assert sum(sizes) == st["total_segments"]
             with pytest.raises(lib.OpenMCError, match="learned_err_bound"):
                 lib.set_option("learned_err_bound", 65)
