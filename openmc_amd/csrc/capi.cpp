@@ -1581,8 +1581,10 @@ int omc_set_option(const char* name, double value)
   else if (n == "learned_index")
     g->use_learned = value != 0;
   else if (n == "learned_err_bound") {
-    if (value < 0 || value > LEARNED_MAX_ERR)
-      throw std::runtime_error("learned_err_bound must be in [0, " + std::to_string(LEARNED_MAX_ERR) + "]");
+    if (value < 0 || value > LEARNED_MAX_ERR) {
+      set_errmsg("learned_err_bound must be in [0, " + std::to_string(LEARNED_MAX_ERR) + "]");
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
     g->learned_err_bound = static_cast<int>(value);
   }
   else if (n == "event_timing")

@@ -115,15 +115,34 @@ int32_t universe_count(const HostModel& m, int u, int target, std::unordered_map
   return count;
 }
 
+void rebuild_tables(const HostModel* m);
+
+// The loaded model with its tables current, or null (also when the tables
+// cannot be built: the message is then set).  Never throws: every C entry
+// point below calls it.
 const HostModel* model_or_null()
 {
   const HostModel* m = api_model();
   if (api_generation() != T.generation || m != T.model || (m && T.n_tallies != m->tallies.size())) {
+    try {
+      rebuild_tables(m);
+    } catch (const std::exception& e) {
+      T = ApiTables();
+      api_set_errmsg(std::string("model tables: ") + e.what());
+      return nullptr;
+    }
+  }
+  return m;
+}
+
+void rebuild_tables(const HostModel* m)
+{
+  {
     T = ApiTables();
     T.model = m;
     T.generation = api_generation();
     if (!m)
-      return nullptr;
+      return;
     T.n_tallies = m->tallies.size();
     // meshes and filters, per problem for a problem set
     auto add_part = [&](int part, const std::vector<std::pair<int, HostMesh>>& meshes,
@@ -207,7 +226,6 @@ const HostModel* model_or_null()
       }
     }
   }
-  return m;
 }
 
 const HostModel* need_model()

@@ -93,6 +93,9 @@ def test_fixed_source_model(lib, tmp_path):
         assert lib.settings.particles == 100
         with pytest.raises(lib.OpenMCError, match="without a GPU"):
             lib.simulation_init()
+        # an option out of range is an error code, not an exception through the C ABI
+        with pytest.raises(lib.InvalidArgumentError, match="learned_err_bound"):
+            lib.set_option("learned_err_bound", 65)
     finally:
         lib.finalize()
 
