@@ -329,6 +329,20 @@ const int* omc_grid_index(int i_nuc);
 /* Batched lookups on the GPU: one nuclide, n energies -> i_grid[n] and
  * (total, absorption)[n]; returns the number of search probes in *probes. */
 int omc_lookup_batch(int i_nuc, const double* E, int64_t n, int learned, int* i_grid, double* xs, uint64_t* probes);
+/* omc_lookup_batch for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  E[n], i_grid[n] and xs[n][2] are DEVICE pointers the
+ * caller owns; the search-probe partial sums use a file-scope device array of
+ * 16385 uint64 per GPU (128 KB, held until the process ends), so n is at most
+ * 4194304 (16384 workgroups of 256); larger n returns OPENMC_E_INVALID_SIZE.
+ * No hipMalloc/hipFree and no allocating library call.  stream: a hipStream_t
+ * (null = the library's stream).  probes not null: the call waits for the
+ * stream and stores the probe total; null: it returns without waiting.  Every
+ * value a call reads from the array was written earlier in the same call, so
+ * nothing needs clearing; calls that overlap on one GPU share the array and
+ * race. */
+int omc_lookup_batch_noalloc(int i_nuc, const double* E, int64_t n, int learned, int* i_grid, double* xs,
+  uint64_t* probes, void* stream);
 /* Macroscopic (total, absorption) of material index `mat` at n energies; the
  * GPU time of the kernel in *ms. */
 int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat, double* macro, double* ms);

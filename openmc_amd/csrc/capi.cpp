@@ -1819,6 +1819,42 @@ int omc_lookup_batch(int i_nuc, const double* E, int64_t n, int learned, int* i_
   });
 }
 
+// omc_lookup_batch without allocation: E, i_grid and xs are device pointers
+// the caller owns; the probe partial sums go to the library's file-scope
+// device array (xs_lookup.hip), so n is at most NOALLOC_MAX_LOOKUPS.  stream
+// null means the library's stream; with probes null the call does not wait.
+int omc_lookup_batch_noalloc(int i_nuc, const double* E, int64_t n, int learned, int* i_grid, double* xs,
+  uint64_t* probes, void* stream)
+{
+  if (!g || !g->initialized)
+    return OPENMC_E_ALLOCATE;
+  if (i_nuc < 0 || i_nuc >= g->P.n_nuclides) {
+    set_errmsg("nuclide index out of bounds");
+    return OPENMC_E_OUT_OF_BOUNDS;
+  }
+  if (learned && !g->P.learned.enabled) {
+    set_errmsg("learned index has not been built");
+    return OPENMC_E_ALLOCATE;
+  }
+  if (n < 0 || n > NOALLOC_MAX_LOOKUPS) {
+    set_errmsg("omc_lookup_batch_noalloc: n = " + std::to_string(n) + " exceeds the scratch limit of " +
+               std::to_string(NOALLOC_MAX_LOOKUPS) + " lookups per call");
+    return OPENMC_E_INVALID_SIZE;
+  }
+  if (n > 0 && (!E || !i_grid || !xs)) {
+    set_errmsg("omc_lookup_batch_noalloc: null device pointer");
+    return OPENMC_E_INVALID_ARGUMENT;
+  }
+  return guarded([&]() -> int {
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g->stream;
+    unsigned long long pr = 0;
+    launch_lookup_noalloc(g->P, i_nuc, E, n, learned, i_grid, xs, probes ? &pr : nullptr, st);
+    if (probes)
+      *probes = pr;
+    return 0;
+  });
+}
+
 int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat, double* macro, double* ms)
 {
   if (!g || !g->initialized)

@@ -56,6 +56,15 @@ void launch_macro_batch(const DevProblem& P, int material, const double* E, int6
   double* macro, unsigned long long* probes, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n_bytes, hipStream_t st);
 
+// The allocation-free variant of launch_lookup_batch (xs_lookup.hip): E,
+// i_grid and xs are device pointers of the caller; the probe partial sums
+// live in the file-scope device array `scratch` (NOALLOC_MAX_BLOCKS + 1
+// values), so at most NOALLOC_MAX_LOOKUPS energies per call.  With
+// probes_host not null the call waits for the stream and returns the total.
+constexpr int NOALLOC_MAX_BLOCKS = 16384;
+constexpr int64_t NOALLOC_MAX_LOOKUPS = static_cast<int64_t>(NOALLOC_MAX_BLOCKS) * 256;
+void launch_lookup_noalloc(const DevProblem& P, int i_nuc, const double* E, int64_t n, int learned, int* i_grid,
+  double* xs, unsigned long long* probes_host, hipStream_t st);
 void launch_math_batch(int fn, const double* x, const double* y, int64_t n, double* out, hipStream_t st);
 
 } // namespace omc

@@ -108,6 +108,115 @@ __global__ void __launch_bounds__(LB_BLOCK) k_copy(const double4* src, double4* 
     dst[i] = src[i];
 }
 
+//==============================================================================
+// The allocation-free lookup batch: device-resident inputs and outputs from
+// the caller, and per-workgroup partial sums of the search probes in a
+// file-scope device array instead of a counter the library allocates.
+//==============================================================================
+
+// scratch[0, NOALLOC_MAX_BLOCKS): one probe partial per workgroup;
+// scratch[NOALLOC_MAX_BLOCKS]: their total.  Every call on a device shares
+// that device's copy.
+__device__ __align__(256) unsigned long long scratch[NOALLOC_MAX_BLOCKS + 1];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+unsigned long long* scratch_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+} // namespace
+
+// One energy per thread (grid = ceil(n / LB_BLOCK) <= NOALLOC_MAX_BLOCKS);
+// workgroup b writes the sum of its threads' probes to partial[b].
+__global__ void __launch_bounds__(LB_BLOCK) k_lookup_partial(DevProblem P, int i_nuc, const double* E, int64_t n,
+  int learned, int* i_grid, double* xs, unsigned long long* partial)
+{
+  __shared__ unsigned long long wsum[LB_BLOCK / 64];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * LB_BLOCK + threadIdx.x;
+  unsigned long long pr = 0;
+  if (i < n) {
+    const double e = E[i];
+    const double lx = detm::log(e / P.energy_min);
+    const int i_log = clamp_log_index(P, lx);
+    int probes = 0;
+    const Micro m = nuclide_xs(P, i_nuc, e, i_log, lx, learned != 0, probes);
+    i_grid[i] = m.i_grid;
+    xs[2 * i] = m.total;
+    xs[2 * i + 1] = m.absorption;
+    pr = static_cast<unsigned long long>(probes);
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    pr += __shfl_xor(pr, o);
+  if (__lane_id() == 0)
+    wsum[threadIdx.x >> 6] = pr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < LB_BLOCK / 64; ++w)
+      t += wsum[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// total = sum of partial[0, n_blocks), in a fixed order: one workgroup,
+// thread t adds partials t, t + LB_BLOCK, ...; then the workgroup's sums in
+// thread order.
+__global__ void __launch_bounds__(LB_BLOCK) k_sum_partials(const unsigned long long* partial, int n_blocks,
+  unsigned long long* total)
+{
+  __shared__ unsigned long long part[LB_BLOCK];
+  unsigned long long t = 0;
+  for (int b = threadIdx.x; b < n_blocks; b += LB_BLOCK)
+    t += partial[b];
+  part[threadIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sum = 0;
+    for (int k = 0; k < LB_BLOCK; ++k)
+      sum += part[k];
+    *total = sum;
+  }
+}
+
+unsigned long long* noalloc_scratch()
+{
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    throw std::runtime_error("no current HIP device for the scratch array");
+  if (!scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<unsigned long long*>(ptr);
+  }
+  return scratch_ptr[dev];
+}
+
+void launch_lookup_noalloc(const DevProblem& P, int i_nuc, const double* E, int64_t n, int learned, int* i_grid,
+  double* xs, unsigned long long* probes_host, hipStream_t st)
+{
+  if (n < 0 || n > NOALLOC_MAX_LOOKUPS)
+    throw std::runtime_error("omc_lookup_batch_noalloc: n = " + std::to_string(n) + " is outside [0, " +
+                             std::to_string(NOALLOC_MAX_LOOKUPS) + "] (" + std::to_string(NOALLOC_MAX_BLOCKS) +
+                             " workgroups of " + std::to_string(LB_BLOCK) + " lookups in the scratch array)");
+  unsigned long long* sp = noalloc_scratch();
+  const int blocks = static_cast<int>((n + LB_BLOCK - 1) / LB_BLOCK);
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_lookup_partial, dim3(blocks), dim3(LB_BLOCK), 0, st, P, i_nuc, E, n, learned, i_grid, xs,
+      sp);
+  // every partial the sum reads was written by this call's lookup kernel
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(LB_BLOCK), 0, st, sp, blocks, sp + NOALLOC_MAX_BLOCKS);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("lookup kernels: ") + hipGetErrorString(e));
+  if (probes_host) {
+    const hipError_t c = hipMemcpyAsync(probes_host, sp + NOALLOC_MAX_BLOCKS, sizeof(unsigned long long),
+      hipMemcpyDeviceToHost, st);
+    const hipError_t w = c == hipSuccess ? hipStreamSynchronize(st) : c;
+    if (w != hipSuccess)
+      throw std::runtime_error(std::string("lookup probes: ") + hipGetErrorString(w));
+  }
+}
+
 void launch_copy(const void* src, void* dst, int64_t n_bytes, hipStream_t st)
 {
   int64_t n = n_bytes / static_cast<int64_t>(sizeof(double4));

@@ -164,6 +164,8 @@ _bind("omc_n_nuclides", [], c_int, check=False)
 _bind("omc_grid_index", [c_int], POINTER(c_int), check=False)
 _bind("omc_lookup_batch", [c_int, POINTER(c_double), c_int64, c_int, POINTER(c_int), POINTER(c_double),
                            POINTER(c_uint64)])
+_bind("omc_lookup_batch_noalloc", [c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p, POINTER(c_uint64),
+                                   c_void_p])
 _bind("omc_macro_batch", [c_int, POINTER(c_double), c_int64, c_int, c_int, POINTER(c_double),
                           POINTER(c_double)])
 _bind("omc_math_batch", [c_int, POINTER(c_double), POINTER(c_double), c_int64, POINTER(c_double)])
