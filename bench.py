@@ -79,7 +79,7 @@ def diff(a, b):
     return {k: b[k] - a[k] for k in b}
 
 
-def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None, timed_steps=1):
+def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None, timed_steps=1, particles=None):
     """W untimed warm-up batches, then exactly K timed batches on the default
     launch path (event cycles replayed from hipGraphs, no per-kernel timers);
     then `timed_steps` more batches with a HIP event recorded around every
@@ -91,6 +91,8 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
     try:
         lib.init(output=False)
         info = {}
+        if particles:
+            lib.settings.particles = int(particles)
         for k, v in (options or {}).items():
             lib.set_option(k, v)
         if learned:
@@ -140,17 +142,19 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
 # HBM traffic per event of each kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over this workload scaled by the copy-kernel calibration
 # (lines "traffic <kernel> <unit> <bytes per unit>").
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_lattice_1e7.txt")
+TRAFFIC_FILES = {1: os.path.join(ROOT, "profiles", "hbm_traffic_lattice_1e7.txt"),
+                 2: os.path.join(ROOT, "profiles", "hbm_traffic_cube300_1e7.txt")}
 UNIT_COUNTER = {"xs_event": "xs_events", "adv_event": "adv_events", "surf_event": "surf_events",
                 "coll_event": "coll_events"}
 
 
-def traffic_per_launch(kernel, st, launches):
+def traffic_per_launch(kernel, st, launches, config=1):
     """Measured HBM bytes per launch of `kernel` in this run: the profiled
     bytes per event times this run's events per launch (None without one)."""
-    if not os.path.exists(TRAFFIC_FILE) or launches <= 0:
+    path = TRAFFIC_FILES.get(config)
+    if not path or not os.path.exists(path) or launches <= 0:
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             parts = line.split()
             if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
@@ -159,7 +163,7 @@ def traffic_per_launch(kernel, st, launches):
     return None
 
 
-def roofline(info):
+def roofline(info, config=1):
     st = info["stats"]
     times = {"k_xs": st["t_xs_ms"], "k_advance": st["t_adv_ms"], "k_surface": st["t_surf_ms"],
              "k_collision": st["t_coll_ms"], "partition_xs": st["t_sort_ms"],
@@ -180,13 +184,13 @@ def roofline(info):
     dom = max(times, key=times.get)
     # average algorithmic bytes per launch over the average launch duration
     achieved = by[dom] / (times[dom] * 1e-3) / 1e9
-    traffic = traffic_per_launch(dom, st, launches[dom])
+    traffic = traffic_per_launch(dom, st, launches[dom], config)
     if fused and dom == "k_xs":
         # the fused kernel's own profile, else the two kernels' profiles added
-        traffic = traffic_per_launch("k_xs_adv", st, launches[dom])
+        traffic = traffic_per_launch("k_xs_adv", st, launches[dom], config)
         if traffic is None:
-            xs = traffic_per_launch("k_xs", st, launches[dom])
-            adv = traffic_per_launch("k_advance", st, launches[dom])
+            xs = traffic_per_launch("k_xs", st, launches[dom], config)
+            adv = traffic_per_launch("k_advance", st, launches[dom], config)
             traffic = None if xs is None or adv is None else round(xs + adv, 1)
     per_kernel = {k: {"ms": times[k], "launches": launches[k], "avg_us": 1e3 * times[k] / max(launches[k], 1),
                       "GB_per_s": by[k] / max(times[k], 1e-9) * 1e-6} for k in times}
@@ -354,7 +358,7 @@ def bench_problem_set(args):
     tst = rs["timed"]["stats"]
     total = sum(rs["n_particles"])
     value = total * args.steps / rs["elapsed_s"]
-    dom, roof, per_kernel = roofline(rs["timed"])
+    dom, roof, per_kernel = roofline(rs["timed"], config=3)
     chosen = dict(opts)
     chosen.update(rs.get("autotune", {}).get("options", {}))
     log("GPU problems one at a time (1 warm-up + 1 timed batch each)")
@@ -496,15 +500,53 @@ def learned_summary(plain, lr, n, steps):
     return out
 
 
+def lookup_kernel_rates(d, n=1 << 20, n_seg=256, epochs=200):
+    """The macroscopic lookup kernel alone on material 0 of the problem in d:
+    n log-uniform energies in random order and sorted, plain and learned
+    (the same interleaved lookup code as the calculate-xs kernel), lookups/s
+    and the check that both give the same bits."""
+    import numpy as np
+    from openmc_amd import lib
+    rng = np.random.default_rng(7)
+    old = os.getcwd()
+    os.chdir(d)
+    try:
+        lib.init(output=False)
+        try:
+            e_min, e_max, _ = lib.energy_grid()
+            E = np.exp(rng.uniform(np.log(e_min), np.log(e_max), n))
+            lib.learned_build(n_seg, epochs)
+            nn = lib.macro_nuclides(0)
+            out = {"energies": n, "nuclides_per_lookup_event": nn}
+            for order, Eo in (("random_order", E), ("sorted", np.sort(E))):
+                res = {}
+                r_plain, ms_p = lib.macro_batch(0, Eo, learned=False, repeat=3)
+                r_learn, ms_l = lib.macro_batch(0, Eo, learned=True, repeat=3)
+                res["plain_lookups_per_s"] = round(n * nn / (ms_p * 1e-3), 1)
+                res["learned_lookups_per_s"] = round(n * nn / (ms_l * 1e-3), 1)
+                res["speedup_vs_binary"] = round(ms_p / ms_l, 4)
+                res["bitwise_equal"] = bool(np.array_equal(r_plain.view(np.uint64), r_learn.view(np.uint64)))
+                out[order] = res
+        finally:
+            lib.finalize()
+    finally:
+        os.chdir(old)
+    return out
+
+
 def bench_lookup_workload(args, opts):
     """The lookup-dominated workload the metric's comparison is about: 300
     nuclides with 200k-point grids in one material (1 m cube, 1e7 particles
-    per batch), plain and learned, one warm-up and LOOKUP_STEPS timed batches
-    each on the graph path, one event-timed batch each for the kernel split."""
+    per batch), plain and learned, one warm-up and --lookup-steps timed
+    batches each on the graph path, one event-timed batch each for the kernel
+    split; then the same two with the calculate-xs queue left in particle
+    order (no energy sort, the reference's own order) at a tenth of the
+    particles; then the lookup kernel alone on random-order and sorted
+    energies."""
     from openmc_amd import configs
     steps = args.lookup_steps
     log("preparing the lookup-dominated problem (300 nuclides x 200k points)")
-    d, model, nucs = configs.problem_dir(2, n_particles=args.particles, n_batches=1 + steps + 1 + 1)
+    d, model, nucs = configs.problem_dir(2, n_particles=args.particles, n_batches=1 + steps + 1 + 1 + 1)
     log("GPU plain, lookup-dominated: 1 warm-up + %d timed batches (+1 event-timed)" % steps)
     plain = run_gpu(d, steps, 1, options=opts, timed_steps=1)
     log("GPU learned, lookup-dominated: 1 warm-up + %d timed batches (+1 event-timed)" % steps)
@@ -512,13 +554,23 @@ def bench_lookup_workload(args, opts):
     n = plain["n_particles"]
     st = plain["stats"]
     tst = plain["timed"]["stats"]
-    dom, roof, per_kernel = roofline(plain["timed"])
+    dom, roof, per_kernel = roofline(plain["timed"], config=2)
+    nu = max(1, n // 10)
+    log("GPU plain and learned with the queue in particle order (no energy sort), %d particles" % nu)
+    pu = run_gpu(d, 1, 1, options=dict(opts, sort=0), timed_steps=1, particles=nu)
+    lu = run_gpu(d, 1, 1, learned=True, options=dict(opts, sort=0), timed_steps=1, particles=nu)
+    unsorted = dict(learned_summary(pu, lu, nu, 1), particles_per_batch=nu,
+                    plain_particles_per_s=round(nu / pu["elapsed_s"], 2))
+    log("lookup kernel alone on the same data (random order and sorted)")
+    kernel = lookup_kernel_rates(d)
     return {"workload": WORKLOADS[2], "particles_per_batch": n, "nuclides": len(nucs),
             "grid_points": int(sum(x.n_points for x in nucs)), "steps": steps,
             "particles_per_s": round(n * steps / plain["elapsed_s"], 2),
             "xs_lookups_per_s": round(kxs_lookups(tst) / tst["t_xs_ms"] * 1e3, 1),
             "xs_lookups_per_s_transport": round(xs_lookups(st) / plain["elapsed_s"], 1),
             "learned": learned_summary(plain, lr, n, steps),
+            "unsorted_queue": unsorted,
+            "lookup_kernel": kernel,
             "roofline": roof, "kernels": per_kernel,
             "events": {k: st.get(k, 0) for k in EVENT_KEYS}}
 
@@ -586,7 +638,7 @@ def main():
                                                                                     args.timed_steps))
     opts = {k: float(v) for k, v in (o.split("=", 1) for o in args.option)}
     plain = run_gpu(d, args.steps, args.warmup, options=opts, timed_steps=args.timed_steps)
-    dom, roof, per_kernel = roofline(plain["timed"])
+    dom, roof, per_kernel = roofline(plain["timed"], config=args.config)
     n = plain["n_particles"]
     value = n * args.steps / plain["elapsed_s"]
     st = plain["stats"]

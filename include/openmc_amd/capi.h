@@ -248,7 +248,7 @@ int openmc_tally_set_scores(int32_t index, int n, const char** scores);
 int openmc_tally_set_nuclides(int32_t index, int n, const char** nuclides);
 int openmc_extend_tallies(int32_t n, int32_t* index_start, int32_t* index_end);
 int openmc_remove_tally(int32_t index);
-/* weight windows: include/openmc/capi.h:186-236 (one set, index 0, id 1) */
+/* weight windows: include/openmc/capi.h:186-236 (the neutron sets in order) */
 size_t openmc_weight_windows_size(void);
 int openmc_get_weight_windows_index(int32_t id, int32_t* idx);
 int openmc_weight_windows_get_id(int32_t index, int32_t* id);

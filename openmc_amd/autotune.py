@@ -35,7 +35,7 @@ from . import lib
 DEFAULT_SPACE = [
     ("blocks", [0, 4096, 8192, 16384, 32768]),
     ("block", [256, 128, 64]),
-    ("sort_bits", [5, 3, 1, 0]),
+    ("sort_bits", [-1, 8, 6, 5, 3, 1]),
     ("sort_every", [1, 2, 4]),
     ("cycles_per_check", [16, 8, 32, 64]),
     ("xs_group", [-1, 2, 4, 8]),
@@ -43,7 +43,7 @@ DEFAULT_SPACE = [
     ("fuse_advance", [-1, 0, 1]),
     ("recompute_micro", [-1, 0, 1]),
 ]
-DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": 5, "sort_every": 1, "cycles_per_check": 16, "xs_group": -1,
+DEFAULTS = {"blocks": 0, "block": 256, "sort_bits": -1, "sort_every": 1, "cycles_per_check": 16, "xs_group": -1,
             "history_below": 65536, "fuse_advance": -1, "recompute_micro": -1}
 
 

@@ -78,8 +78,9 @@ def config0(n_particles=100000, n_batches=10, n_grid=10000, mesh=(10, 10, 10), e
 
 def config1(n_particles=10_000_000, n_batches=20, grid_lo=20000, grid_hi=100000, n_nuclides=60,
             lattice_n=17, mesh=(100, 100, 100), event_based=True, half_height=50.0,
-            energy_bins=None):
-    """17x17 pin lattice, three materials of 20 nuclides, Watt source in the fuel."""
+            energy_bins=None, tally=True):
+    """17x17 pin lattice, three materials of 20 nuclides, Watt source in the fuel
+    (tally False: no tally, for measuring what the mesh tally costs)."""
     om.reset_auto_ids()
     nucs = _nuclide_set("L", n_nuclides, DATA_SEED_BASE + 1, grid_lo, grid_hi)
     per = n_nuclides // 3
@@ -127,7 +128,7 @@ def config1(n_particles=10_000_000, n_batches=20, grid_lo=20000, grid_hi=100000,
     t.filters = [om.MeshFilter(m)]
     if energy_bins is not None:
         t.filters.append(om.EnergyFilter(energy_bins))
-    model = om.Model(geom, om.Materials(mats), s, om.Tallies([t]))
+    model = om.Model(geom, om.Materials(mats), s, om.Tallies([t] if tally else []))
     return model, nucs
 
 
@@ -166,7 +167,7 @@ def config2(n_particles=10_000_000, n_batches=20, n_nuclides=300, n_grid=200000,
 
 def config4(n_particles=10_000_000, n_batches=16, n_grid=200000, n_layers=5, layer=20.0, per_layer=20,
             half_width=50.0, density=0.01, ww_mesh=(100, 50, 50), ww_length=8.0, upper_ratio=5.0,
-            tally_mesh=(200, 200, 200), energy_groups=10, event_based=True):
+            tally_mesh=(200, 200, 200), energy_groups=10, event_based=True, ww_split_sets=False):
     """Deep-penetration slab with mesh weight windows.
 
     Layers of `layer` cm along x (x = 0 and x = n_layers * layer vacuum,
@@ -179,6 +180,10 @@ def config4(n_particles=10_000_000, n_batches=16, n_grid=200000, n_layers=5, lay
     upper = upper_ratio * lower, one energy group; splitting and roulette at
     birth and collisions.  Tally: track-length flux on `tally_mesh` with
     `energy_groups` log-spaced groups over [1e-5 eV, 20 MeV].
+    ww_split_sets: the same windows as two sets, the first valid (lower >= 0)
+    only in the front half of the slab and -1 behind it, the second over the
+    whole slab at 1.5x the bounds: a particle in the back half takes the
+    second set's window (the first set's are invalid there).
     """
     om.reset_auto_ids()
     rng = np.random.default_rng(DATA_SEED_BASE + 44)
@@ -216,7 +221,13 @@ def config4(n_particles=10_000_000, n_batches=16, n_grid=200000, n_layers=5, lay
     xc = (np.arange(ww_mesh[0]) + 0.5) * depth / ww_mesh[0]
     lower = np.broadcast_to((a0 * np.exp(-xc / ww_length))[:, None, None, None],
                             tuple(ww_mesh) + (1,)).copy()
-    s.weight_windows = [om.WeightWindows(wm, lower, lower * upper_ratio)]
+    if ww_split_sets:
+        front = lower.copy()
+        front[ww_mesh[0] // 2:] = -1.0
+        s.weight_windows = [om.WeightWindows(wm, front, np.where(front < 0, -10.0, front * upper_ratio)),
+                            om.WeightWindows(wm, 1.5 * lower, 1.5 * lower * upper_ratio)]
+    else:
+        s.weight_windows = [om.WeightWindows(wm, lower, lower * upper_ratio)]
     tm = om.RegularMesh()
     tm.dimension = tally_mesh
     tm.lower_left = (0.0, -half_width, -half_width)

@@ -133,12 +133,15 @@ struct Sim {
   bool use_graph = true;
   int cycles_per_check = 16;
   int xs_lanes_opt = 0; // 0 = from the largest material
+  size_t lds_per_block = 65536; // the device's LDS per workgroup (set at init)
   bool sort = true;
   int sort_blocks_opt = 0; // 0 = one workgroup per CU
   int blocks_opt = 0;      // 0 = from the capacity
   int xs_group_opt = -1; // -1: 2 when collisions repeat the lookups, else 4
   int bank_cap = 32; // secondary bank sites per particle (weight windows)
-  int sort_bits = 5; // mantissa bits of E in the calculate-xs sort key (32 buckets per binade)
+  int sort_bits = -1; // mantissa bits of E in the calculate-xs sort key (-1: 5, or 8 for materials of more
+                      // than 64 nuclides, whose 300-lookup events repay a finer order)
+  int sort_bits_used = 5;
   int block_opt = 256;  // threads per workgroup of the event kernels
   int sort_every = 1;   // full (material, energy) sort every this many cycles
   int fuse_advance = -1; // advance in the calculate-xs launch (-1: when collisions repeat the lookups)
@@ -436,22 +439,29 @@ void upload_problem(Sim& s)
     d.on = 1;
     d.ckpt_collision = w.ckpt_collision ? 1 : 0;
     d.ckpt_surface = w.ckpt_surface ? 1 : 0;
-    d.max_split = w.max_split;
-    d.n_dim = w.mesh.n_dim;
-    for (int k = 0; k < 3; ++k) {
-      d.shape[k] = w.mesh.shape[k];
-      d.ll[k] = w.mesh.ll[k];
-      d.width[k] = w.mesh.width[k];
-    }
-    d.n_e = static_cast<int>(w.ebounds.size()) - 1;
-    d.n_mesh = w.mesh.n_bins();
-    d.ebounds = dev_upload(s, w.ebounds);
-    d.lower = dev_upload(s, w.lower);
-    d.upper = dev_upload(s, w.upper);
-    d.survival_ratio = w.survival_ratio;
-    d.max_lb_ratio = w.max_lb_ratio;
-    d.weight_cutoff = w.weight_cutoff;
     d.max_history_splits = w.max_history_splits;
+    std::vector<DevWWSet> sets;
+    for (const HostWWSet& ws : w.sets) {
+      DevWWSet e {};
+      e.n_dim = ws.mesh.n_dim;
+      for (int k = 0; k < 3; ++k) {
+        e.shape[k] = ws.mesh.shape[k];
+        e.ll[k] = ws.mesh.ll[k];
+        e.width[k] = ws.mesh.width[k];
+      }
+      e.n_e = static_cast<int>(ws.ebounds.size()) - 1;
+      e.max_split = ws.max_split;
+      e.n_mesh = ws.mesh.n_bins();
+      e.ebounds = dev_upload(s, ws.ebounds);
+      e.lower = dev_upload(s, ws.lower);
+      e.upper = dev_upload(s, ws.upper);
+      e.survival_ratio = ws.survival_ratio;
+      e.max_lb_ratio = ws.max_lb_ratio;
+      e.weight_cutoff = ws.weight_cutoff;
+      sets.push_back(e);
+    }
+    d.n_sets = static_cast<int>(sets.size());
+    d.sets = dev_upload(s, sets);
   }
 
   P.n_particles = m.n_particles;
@@ -540,23 +550,32 @@ void alloc_particles(Sim& s, int64_t cap)
   S.counters = dev_alloc<unsigned long long>(L, static_cast<size_t>(CNT_N) * CNT_SLOTS);
   s.h_counter_slots.assign(static_cast<size_t>(CNT_N) * CNT_SLOTS, 0);
   s.h_leak_slots.assign(std::max<size_t>(CNT_SLOTS, leak_slots(s)), 0.0);
-  // calculate-xs keys (material, energy bucket): the bucket is the sign,
-  // exponent and top sort_bits mantissa bits of E, offset by those of
-  // energy_min; 64 binades of 2^sort_bits buckets each (32 per binade by
-  // default: the lanes of a wavefront then read neighbouring grid points),
-  // fewer when the materials times buckets exceed the LDS histogram
-  const int n_mat = s.P.n_materials + 1;
-  int shift = 52 - s.sort_bits, eb = 64 << s.sort_bits;
-  while (n_mat * eb > SORT_MAX_BINS && eb > 1) {
-    ++shift;
-    eb >>= 1;
-  }
-  uint64_t emin_bits;
+  // calculate-xs keys: 0 for void, else 1 + material * eb + energy bucket;
+  // the bucket is the exponent and top sort_bits mantissa bits of E, offset
+  // by those of energy_min, so eb = (binades of [energy_min, energy_max])
+  // x 2^sort_bits; fewer bits when the materials times buckets exceed the
+  // LDS histogram
+  const int n_mat = std::max(1, s.P.n_materials);
+  uint64_t emin_bits, emax_bits;
   std::memcpy(&emin_bits, &s.m.energy_min, 8);
+  std::memcpy(&emax_bits, &s.m.energy_max, 8);
+  const int binades = std::max(1, static_cast<int>((emax_bits >> 52) - (emin_bits >> 52)) + 1);
+  int bits = s.sort_bits >= 0 ? s.sort_bits : (s.P.max_mat_nuc > RECOMPUTE_MAX_NUC ? 8 : 5);
+  while (bits > 0 && 1 + static_cast<int64_t>(n_mat) * (binades << bits) > SORT_MAX_BINS)
+    --bits;
+  int shift = 52 - bits;
+  int64_t eb = static_cast<int64_t>(binades) << bits;
+  while (1 + n_mat * eb > SORT_MAX_BINS && eb > 1) {
+    // more materials than the histogram holds at one bucket per binade:
+    // buckets of several binades
+    ++shift;
+    eb = (eb + 1) / 2;
+  }
   S.sort_shift = shift;
-  S.sort_eb = eb;
-  S.sort_base = static_cast<int>(emin_bits >> shift) - 1;
-  S.sort_nb = std::min(n_mat * eb, SORT_MAX_BINS);
+  S.sort_eb = static_cast<int>(eb);
+  S.sort_base = static_cast<int>(emin_bits >> shift);
+  S.sort_nb = static_cast<int>(std::min<int64_t>(1 + n_mat * eb, SORT_MAX_BINS));
+  s.sort_bits_used = shift <= 52 ? 52 - shift : 0;
   // partition scratch, shared by the calculate-xs partition (sort_nb keys)
   // and the advance split (two keys)
   const int rows = std::max(S.sort_nb, 2);
@@ -1076,6 +1095,9 @@ void finish_init(Sim& s)
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
     throw std::runtime_error("no GPU visible: the MI355X transport path needs a HIP device");
   hip_check(hipSetDevice(0), "hipSetDevice");
+  hipDeviceProp_t prop {};
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.sharedMemPerBlock > 0)
+    s.lds_per_block = prop.sharedMemPerBlock;
   hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipHostMalloc(reinterpret_cast<void**>(&s.h_qcount), sizeof(int) * QC_N, hipHostMallocDefault),
     "hipHostMalloc");
@@ -1599,6 +1621,15 @@ int omc_set_option(const char* name, double value)
       set_errmsg("xs_lanes must be 0 or a power of two in [1, 64]");
       return OPENMC_E_INVALID_ARGUMENT;
     }
+    // several lanes per particle stage the material's micro cross sections
+    // in LDS: (256 / lanes) particles x nuclides x 20 bytes per workgroup
+    const size_t need = v > 1 ? static_cast<size_t>(256 / v) * std::max(1, g->P.max_mat_nuc) * 20 : 0;
+    if (need > g->lds_per_block) {
+      set_errmsg("xs_lanes " + std::to_string(v) + " needs " + std::to_string(need) + " bytes of LDS per workgroup "
+                 "for materials of " + std::to_string(g->P.max_mat_nuc) + " nuclides; the device has " +
+                 std::to_string(g->lds_per_block));
+      return OPENMC_E_INVALID_ARGUMENT;
+    }
     g->xs_lanes_opt = v;
     update_launch_config(*g);
   } else if (n == "sort") {
@@ -1610,8 +1641,8 @@ int omc_set_option(const char* name, double value)
       set_errmsg("bank_cap must be in [1, 4096]");
       return OPENMC_E_INVALID_ARGUMENT;
     }
-    if (n == "sort_bits" && (v < 0 || v > 8)) {
-      set_errmsg("sort_bits must be in [0, 8]");
+    if (n == "sort_bits" && (v < -1 || v > 8)) {
+      set_errmsg("sort_bits must be in [0, 8] (-1: by the material size)");
       return OPENMC_E_INVALID_ARGUMENT;
     }
     (n == "bank_cap" ? g->bank_cap : g->sort_bits) = v;
@@ -1741,7 +1772,7 @@ int omc_get_stats(double* out, int n)
   v.push_back(g->lc.block);
   v.push_back(g->lc.sort_every);
   v.push_back(g->cycles_per_check);
-  v.push_back(g->sort_bits);
+  v.push_back(g->sort_bits_used);
   v.push_back(g->lc.sort_blocks);
   v.push_back(g->history_below);
   v.push_back(g->history_steps);
@@ -1873,14 +1904,15 @@ int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat
     double* dM = dev_alloc<double>(tmp, 2 * n);
     unsigned long long* dP = dev_alloc<unsigned long long>(tmp, 1);
     hip_check(hipMemcpy(dE, E, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
-    launch_macro_batch(g->P, mat, dE, n, learned, dM, dP, g->stream); // warm-up
+    const int group = g->xs_group_opt > 0 ? g->xs_group_opt : 4;
+    launch_macro_batch(g->P, mat, dE, n, learned, dM, dP, g->stream, group); // warm-up
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     int reps = std::max(1, repeat);
     hip_check(hipEventRecord(e0, g->stream), "hipEventRecord");
     for (int r = 0; r < reps; ++r)
-      launch_macro_batch(g->P, mat, dE, n, learned, dM, nullptr, g->stream);
+      launch_macro_batch(g->P, mat, dE, n, learned, dM, nullptr, g->stream, group);
     hip_check(hipEventRecord(e1, g->stream), "hipEventRecord");
     hip_check(hipEventSynchronize(e1), "macro kernel");
     float t = 0;

@@ -68,7 +68,7 @@ struct ApiTables {
   std::vector<std::string> nuclide_names;
   std::vector<std::vector<int>> mesh_dims;
   std::vector<std::vector<double>> mesh_ll, mesh_ur, mesh_width;
-  int ww_mesh_index = -1;
+  std::vector<int> ww_mesh_index; // per weight-window set
 };
 
 ApiTables T;
@@ -214,16 +214,14 @@ void rebuild_tables(const HostModel* m)
     for (size_t k = 0; k < m->nuclides.size(); ++k)
       if (T.nuclide_names[k].empty())
         T.nuclide_names[k] = m->nuclides[k].name;
-    if (m->ww.on) {
-      // the weight-window mesh is the settings mesh with the same bounds
-      for (size_t i = 0; i < T.meshes.size(); ++i) {
-        const HostMesh& me = *T.meshes[i].second;
-        if (std::equal(me.shape, me.shape + 3, m->ww.mesh.shape) && std::equal(me.ll, me.ll + 3, m->ww.mesh.ll) &&
-            std::equal(me.ur, me.ur + 3, m->ww.mesh.ur)) {
-          T.ww_mesh_index = static_cast<int>(i);
+    for (const HostWWSet& ws : m->ww.sets) {
+      int idx = -1;
+      for (size_t i = 0; i < T.meshes.size(); ++i)
+        if (T.meshes[i].first == ws.mesh_id) {
+          idx = static_cast<int>(i);
           break;
         }
-      }
+      T.ww_mesh_index.push_back(idx);
     }
   }
 }
@@ -283,9 +281,14 @@ int check_mesh(int32_t index)
 int check_ww(int32_t index)
 {
   const HostModel* m = model_or_null();
-  if (!m || !m->ww.on || index != 0)
+  if (!m || bad_index(index, m->ww.sets.size()))
     return fail(OPENMC_E_OUT_OF_BOUNDS, "Index in weight windows vector is out of bounds.");
   return 0;
+}
+
+const HostWWSet& ww_set(int32_t index)
+{
+  return model_or_null()->ww.sets[static_cast<size_t>(index)];
 }
 
 double material_density_gpcc(const HostModel& m, int index)
@@ -1352,21 +1355,25 @@ int openmc_remove_tally(int32_t index)
 }
 
 // ---------------------------------------------------------------------------
-// Weight windows (src/weight_windows.cpp:1000-1400): one set, index 0, when
-// the model has <weight_windows>
+// Weight windows (src/weight_windows.cpp:1000-1400): the model's neutron
+// sets in definition order
 // ---------------------------------------------------------------------------
 size_t openmc_weight_windows_size(void)
 {
   const HostModel* m = model_or_null();
-  return m && m->ww.on ? 1 : 0;
+  return m ? m->ww.sets.size() : 0;
 }
 
 int openmc_get_weight_windows_index(int32_t id, int32_t* idx)
 {
   const HostModel* m = model_or_null();
-  if (m && m->ww.on && id == 1) {
-    *idx = 0;
-    return 0;
+  if (m) {
+    for (size_t i = 0; i < m->ww.sets.size(); ++i) {
+      if (m->ww.sets[i].id == id) {
+        *idx = static_cast<int32_t>(i);
+        return 0;
+      }
+    }
   }
   return fail(OPENMC_E_INVALID_ID, "No weight windows exist with ID=" + std::to_string(id) + ".");
 }
@@ -1375,7 +1382,7 @@ int openmc_weight_windows_get_id(int32_t index, int32_t* id)
 {
   if (int err = check_ww(index))
     return err;
-  *id = 1;
+  *id = ww_set(index).id;
   return 0;
 }
 
@@ -1383,7 +1390,7 @@ int openmc_weight_windows_get_mesh(int32_t index, int32_t* mesh_idx)
 {
   if (int err = check_ww(index))
     return err;
-  *mesh_idx = T.ww_mesh_index;
+  *mesh_idx = T.ww_mesh_index[static_cast<size_t>(index)];
   return 0;
 }
 
@@ -1391,9 +1398,8 @@ int openmc_weight_windows_get_energy_bounds(int32_t index, const double** e_boun
 {
   if (int err = check_ww(index))
     return err;
-  const HostModel* m = model_or_null();
-  *e_bounds = m->ww.ebounds.data();
-  *e_bounds_size = m->ww.ebounds.size();
+  *e_bounds = ww_set(index).ebounds.data();
+  *e_bounds_size = ww_set(index).ebounds.size();
   return 0;
 }
 
@@ -1410,10 +1416,9 @@ int openmc_weight_windows_get_bounds(int32_t index, const double** lower_bounds,
 {
   if (int err = check_ww(index))
     return err;
-  const HostModel* m = model_or_null();
-  *lower_bounds = m->ww.lower.data();
-  *upper_bounds = m->ww.upper.data();
-  *size = m->ww.lower.size();
+  *lower_bounds = ww_set(index).lower.data();
+  *upper_bounds = ww_set(index).upper.data();
+  *size = ww_set(index).lower.size();
   return 0;
 }
 
@@ -1421,7 +1426,7 @@ int openmc_weight_windows_get_survival_ratio(int32_t index, double* ratio)
 {
   if (int err = check_ww(index))
     return err;
-  *ratio = model_or_null()->ww.survival_ratio;
+  *ratio = ww_set(index).survival_ratio;
   return 0;
 }
 
@@ -1429,7 +1434,7 @@ int openmc_weight_windows_get_max_lower_bound_ratio(int32_t index, double* lb_ra
 {
   if (int err = check_ww(index))
     return err;
-  *lb_ratio = model_or_null()->ww.max_lb_ratio;
+  *lb_ratio = ww_set(index).max_lb_ratio;
   return 0;
 }
 
@@ -1437,7 +1442,7 @@ int openmc_weight_windows_get_weight_cutoff(int32_t index, double* cutoff)
 {
   if (int err = check_ww(index))
     return err;
-  *cutoff = model_or_null()->ww.weight_cutoff;
+  *cutoff = ww_set(index).weight_cutoff;
   return 0;
 }
 
@@ -1445,7 +1450,7 @@ int openmc_weight_windows_get_max_split(int32_t index, int* max_split)
 {
   if (int err = check_ww(index))
     return err;
-  *max_split = model_or_null()->ww.max_split;
+  *max_split = ww_set(index).max_split;
   return 0;
 }
 

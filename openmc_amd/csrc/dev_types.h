@@ -159,18 +159,14 @@ struct DevTally {
   int has_mesh;        // filters [mesh] / [mesh, energy]; else [] or [energy]
 };
 
-// Mesh weight windows for neutrons (src/weight_windows.cpp:56-146, 375-411)
-struct DevWW {
-  int on;
-  int ckpt_collision;
-  int ckpt_surface;
-  int max_split;
+// One set of mesh weight windows (WeightWindows, src/weight_windows.cpp:375-411)
+struct DevWWSet {
   int n_dim;
   int shape[3];
   double ll[3];
   double width[3];
   int n_e; // energy groups (bounds n_e + 1)
-  int pad;
+  int max_split;
   int64_t n_mesh;
   const double* ebounds;
   const double* lower; // [n_e][n_mesh]
@@ -178,7 +174,17 @@ struct DevWW {
   double survival_ratio;
   double max_lb_ratio;
   double weight_cutoff;
+};
+
+// Mesh weight windows for neutrons (src/weight_windows.cpp:56-146): a
+// particle takes the window of the first set that has a valid one for it.
+struct DevWW {
+  int on;
+  int ckpt_collision;
+  int ckpt_surface;
+  int n_sets;
   int64_t max_history_splits;
+  const DevWWSet* sets;
 };
 
 // One problem of a problem set: several independent models whose particles
@@ -357,7 +363,7 @@ struct DevParticles {
   double* total_weight; // [CNT_SLOTS] per-batch source weight partial sums
   PHot* hot;      // [cap]
   PLevel* lev;    // [cap][MAX_LEVELS - 1]
-  double* micro_t; // [cap][max_mat_nuc]: per-nuclide totals for nuclide sampling
+  double* micro_t; // [cap][max_mat_nuc]: running sums of density x micro total, for nuclide sampling
   int max_nn;
   int* lost;
   uint64_t* hash;
@@ -376,7 +382,7 @@ struct DevParticles {
   // key totals then key starts
   int* sort_count;
   int* sort_total;
-  int sort_nb;    // calculate-xs keys: (material + 1) * sort_eb + energy bucket
+  int sort_nb;    // calculate-xs keys: 0 (void), 1 + material * sort_eb + energy bucket
   int sort_shift; // energy bucket = (bits(E) >> sort_shift) - sort_base
   int sort_base;
   int sort_eb;    // energy buckets per material

@@ -1325,8 +1325,10 @@ struct WeightWindow {
   }
 };
 
-// WeightWindows::get_weight_window for a neutron at r with energy E
-__device__ __forceinline__ WeightWindow get_weight_window(const DevWW& W, V3 r, double E)
+// WeightWindows::get_weight_window of one set for a neutron at r with energy E
+// (src/weight_windows.cpp:375-411); invalid (lower -1) outside its mesh or
+// energy range
+__device__ __forceinline__ WeightWindow set_weight_window(const DevWWSet& W, V3 r, double E)
 {
   WeightWindow ww;
   ww.lower = -1.0;
@@ -1355,6 +1357,25 @@ __device__ __forceinline__ WeightWindow get_weight_window(const DevWW& W, V3 r, 
   ww.max_lb_ratio = W.max_lb_ratio;
   ww.max_split = W.max_split;
   ww.cutoff = W.weight_cutoff;
+  return ww;
+}
+
+// The first set's valid window, else the last set's (invalid) answer
+// (apply_weight_windows, src/weight_windows.cpp:70-75)
+__device__ __forceinline__ WeightWindow get_weight_window(const DevWW& W, V3 r, double E)
+{
+  WeightWindow ww;
+  ww.lower = -1.0;
+  ww.upper = 1.0;
+  ww.max_lb_ratio = 1.0;
+  ww.survival = 0.5;
+  ww.cutoff = 1.0e-38;
+  ww.max_split = 10;
+  for (int k = 0; k < W.n_sets; ++k) {
+    ww = set_weight_window(W.sets[k], r, E);
+    if (ww.valid())
+      break;
+  }
   return ww;
 }
 

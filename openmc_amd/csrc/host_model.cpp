@@ -219,52 +219,55 @@ void read_weight_windows(HostModel& m, const XmlNode& s)
     meshes[id] = parse_regular_mesh(*mn);
     add_mesh(m, id, meshes[id]);
   }
-  const XmlNode* sel = nullptr;
-  for (const XmlNode* n : wws) {
-    if (n->value("particle_type") != "neutron")
+  w.sets.clear();
+  for (const XmlNode* sel : wws) {
+    // photon windows never apply: the path transports neutrons only
+    if ((sel->has("particle_type") ? sel->value("particle_type") : std::string("neutron")) != "neutron")
       continue;
-    if (sel)
-      throw std::runtime_error("the GPU path supports one set of neutron weight windows");
-    sel = n;
-  }
-  w.on = sel != nullptr;
-  if (sel) {
-    auto mt = meshes.find(std::stoi(sel->value("mesh")));
+    HostWWSet ws;
+    ws.id = std::stoi(sel->value("id"));
+    ws.mesh_id = std::stoi(sel->value("mesh"));
+    auto mt = meshes.find(ws.mesh_id);
     if (mt == meshes.end())
       throw std::runtime_error("weight windows refer to an unknown mesh");
-    w.mesh = mt->second;
+    ws.mesh = mt->second;
     if (sel->has("energy_bounds"))
-      w.ebounds = to_doubles(sel->value("energy_bounds"));
+      ws.ebounds = to_doubles(sel->value("energy_bounds"));
     else
-      w.ebounds = {0.0, INFTY};
-    if (w.ebounds.size() < 2)
+      ws.ebounds = {0.0, INFTY};
+    if (ws.ebounds.size() < 2)
       throw std::runtime_error("weight windows need at least two energy bounds");
     if (sel->has("survival_ratio")) {
-      w.survival_ratio = std::stod(sel->value("survival_ratio"));
-      if (w.survival_ratio <= 1)
+      ws.survival_ratio = std::stod(sel->value("survival_ratio"));
+      if (ws.survival_ratio <= 1)
         throw std::runtime_error("Survival to lower weight window ratio must bigger than 1");
     }
     if (sel->has("max_lower_bound_ratio")) {
-      w.max_lb_ratio = std::stod(sel->value("max_lower_bound_ratio"));
-      if (w.max_lb_ratio < 1.0)
+      ws.max_lb_ratio = std::stod(sel->value("max_lower_bound_ratio"));
+      if (ws.max_lb_ratio < 1.0)
         throw std::runtime_error("Maximum lower bound ratio must be larger than 1");
     }
     if (sel->has("max_split")) {
-      w.max_split = static_cast<int>(std::stod(sel->value("max_split")));
-      if (w.max_split <= 1)
+      ws.max_split = static_cast<int>(std::stod(sel->value("max_split")));
+      if (ws.max_split <= 1)
         throw std::runtime_error("max split must be larger than 1");
     }
     if (sel->has("weight_cutoff")) {
-      w.weight_cutoff = std::stod(sel->value("weight_cutoff"));
-      if (w.weight_cutoff <= 0 || w.weight_cutoff > 1)
+      ws.weight_cutoff = std::stod(sel->value("weight_cutoff"));
+      if (ws.weight_cutoff <= 0 || ws.weight_cutoff > 1)
         throw std::runtime_error("weight_cutoff must be in (0, 1]");
     }
-    w.lower = to_doubles(sel->value("lower_ww_bounds"));
-    w.upper = to_doubles(sel->value("upper_ww_bounds"));
-    const size_t want = static_cast<size_t>(w.mesh.n_bins()) * (w.ebounds.size() - 1);
-    if (w.lower.size() != want || w.upper.size() != want)
+    ws.lower = to_doubles(sel->value("lower_ww_bounds"));
+    ws.upper = to_doubles(sel->value("upper_ww_bounds"));
+    const size_t want = static_cast<size_t>(ws.mesh.n_bins()) * (ws.ebounds.size() - 1);
+    if (ws.lower.size() != want || ws.upper.size() != want)
       throw std::runtime_error("weight window bounds do not match the mesh and energy bounds");
+    for (const HostWWSet& o : w.sets)
+      if (o.id == ws.id)
+        throw std::runtime_error("Two or more weight windows use the same unique ID: " + std::to_string(ws.id));
+    w.sets.push_back(std::move(ws));
   }
+  w.on = !w.sets.empty();
   if (s.has("weight_windows_on"))
     w.on = w.on && to_bool(s.value("weight_windows_on"));
   if (const XmlNode* ck = s.child("weight_window_checkpoints")) {

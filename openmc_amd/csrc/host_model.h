@@ -62,10 +62,11 @@ struct HostFilter {
   int n_bins = 0;
 };
 
-// Mesh weight windows for neutrons (src/weight_windows.cpp:165-229 reads the
-// element; src/settings.cpp:1203-1260 the switches)
-struct HostWW {
-  bool on = false;
+// One set of mesh weight windows for neutrons (src/weight_windows.cpp:165-229
+// reads the element)
+struct HostWWSet {
+  int id = 0;
+  int mesh_id = 0;
   HostMesh mesh;
   std::vector<double> ebounds;      // energy bounds (default [0, inf))
   std::vector<double> lower, upper; // [n_energy][n_mesh_bins]
@@ -73,6 +74,14 @@ struct HostWW {
   double max_lb_ratio = 1.0;
   int max_split = 10;
   double weight_cutoff = 1.0e-38;
+};
+
+// Weight windows: every neutron set in definition order (a particle uses the
+// first set that has a valid window for it, src/weight_windows.cpp:70-75) and
+// the switches (src/settings.cpp:1203-1260)
+struct HostWW {
+  bool on = false;
+  std::vector<HostWWSet> sets;
   bool ckpt_collision = true;
   bool ckpt_surface = false;
   int64_t max_history_splits = 10000000;

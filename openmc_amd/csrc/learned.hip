@@ -214,16 +214,35 @@ __global__ void k_ladam(const int* seg_start, int n_seg, const double2* grad, fl
   params[j] = p;
 }
 
-// Recentre the segment's intercept and record its error bound.
-//
-// A query in segment s has f in [0, 1); with the segment's points a0..a1-1
-// at f_a0 <= ... <= f_{a1-1}, its lower bound lb (first point >= E) is k on
-// the sub-interval (f_{k-1}, f_k] (f_{a0-1} read as 0, f_{a1} as 1).  The FP32
-// prediction is monotone in f, so max |pred - lb| over the segment is reached
-// at sub-interval ends:  pred(0) - a0,  pred(f_i) - i and pred(f_i) - (i + 1)
-// for each point i, and pred(1) - a1.  err is the ceiling of the largest of
-// those after recentring, so every query of the segment has |pred - lb| <= err
-// and the window [j - err - 1, j + err + 3) (j = (int)pred) holds lb.
+// The largest |pred - lower bound| a query of segment s (f in [0, 1)) can
+// see.  Its points a0..a1-1 sit at f_a0 <= ... <= f_{a1-1}; the lower bound
+// lb (first point >= E) is k on the sub-interval (f_{k-1}, f_k] (f_{a0-1}
+// read as 0, f_{a1} as 1).  The FP32 prediction is monotone in f, so the
+// extremes are at sub-interval ends:  pred(0) - a0,  pred(f_i) - i and
+// pred(f_i) - (i + 1) for each point i, and pred(1) - a1.  Every lane of the
+// wavefront returns the same value.
+__device__ double seg_bound(float2 ab, const double* u, int a0, int a1, int s)
+{
+  const double us = static_cast<double>(s);
+  double amax = 0.0;
+  for (int i = a0 + static_cast<int>(threadIdx.x); i < a1; i += LW) {
+    const double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
+    amax = fmax(amax, fmax(fabs(r), fabs(r - 1.0)));
+  }
+  const double r0 = static_cast<double>(predict(ab, us, s)) - static_cast<double>(a0);
+  const double r1 = static_cast<double>(predict(ab, us + 1.0, s)) - static_cast<double>(a1);
+  amax = fmax(amax, fmax(fabs(r0), fabs(r1)));
+  for (int o = 32; o > 0; o >>= 1)
+    amax = fmax(amax, __shfl_xor(amax, o));
+  return amax;
+}
+
+// Recentre the segment's intercept on its points, record its error bound,
+// and keep instead a flat or end-to-end line where that bounds the error
+// more tightly.  A cluster of points that no segment can split (the points
+// of one narrow resonance) makes the trained slope steep, and a steep line
+// is far off at the segment's ends; the flat line through the middle of the
+// segment's index range is within half that range everywhere.
 __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, const double* uu,
   const int* seg_start, int n_seg, float2* params, int* err)
 {
@@ -236,31 +255,39 @@ __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, co
   const int a0 = seg_start[k], a1 = seg_start[k + 1];
   const int64_t j = static_cast<int64_t>(y) * n_seg + s;
   float2 ab = params[j];
-  const double us = static_cast<double>(s);
-  for (int pass = 0; pass < 2; ++pass) {
-    // pass 0: residual range, then recentre; pass 1: the bound
+  // recentre on the points' residual range, pred(f_i) - i, so that a query
+  // between points i and i + 1 predicts inside [i, i + 1) and the lookup's
+  // speculative bracket hits
+  {
     double rmin = 1e300, rmax = -1e300;
     for (int i = a0 + threadIdx.x; i < a1; i += LW) {
       const double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
-      rmin = fmin(rmin, r - 1.0);
+      rmin = fmin(rmin, r);
       rmax = fmax(rmax, r);
-    }
-    if (threadIdx.x == 0) {
-      const double r0 = static_cast<double>(predict(ab, us, s)) - static_cast<double>(a0);
-      const double r1 = static_cast<double>(predict(ab, us + 1.0, s)) - static_cast<double>(a1);
-      rmin = fmin(rmin, fmin(r0, r1));
-      rmax = fmax(rmax, fmax(r0, r1));
     }
     for (int o = 32; o > 0; o >>= 1) {
       rmin = fmin(rmin, __shfl_xor(rmin, o));
       rmax = fmax(rmax, __shfl_xor(rmax, o));
     }
-    if (pass == 0) {
+    if (rmax >= rmin)
       ab.y = static_cast<float>(static_cast<double>(ab.y) - 0.5 * (rmax + rmin));
-    } else if (threadIdx.x == 0) {
-      params[j] = ab;
-      err[j] = static_cast<int>(ceil(fmax(fabs(rmin), fabs(rmax))));
-    }
+  }
+  double best = seg_bound(ab, u, a0, a1, s);
+  const float2 flat {0.0f, static_cast<float>(0.5 * (static_cast<double>(a0) + static_cast<double>(a1)))};
+  const float2 ends {static_cast<float>(a1 - a0), static_cast<float>(a0)};
+  const double b_flat = seg_bound(flat, u, a0, a1, s);
+  const double b_ends = seg_bound(ends, u, a0, a1, s);
+  if (b_flat < best) {
+    best = b_flat;
+    ab = flat;
+  }
+  if (b_ends < best) {
+    best = b_ends;
+    ab = ends;
+  }
+  if (threadIdx.x == 0) {
+    params[j] = ab;
+    err[j] = static_cast<int>(ceil(best));
   }
 }
 

@@ -52,8 +52,11 @@ void launch_accumulate(double* value, double* results, int64_t n_bins, int n_cop
 // Standalone batched lookups (xs_lookup.hip): E[n] -> i_grid[n], xs[n][2]
 void launch_lookup_batch(const DevProblem& P, int i_nuc, const double* E, int64_t n, int learned, int* i_grid,
   double* xs, unsigned long long* probes, hipStream_t st);
+// Macroscopic (total, absorption) of one material at n energies with the
+// calculate-xs lookup code, `group` (1, 2, 4 or 8) nuclide lookups
+// interleaved per lane.
 void launch_macro_batch(const DevProblem& P, int material, const double* E, int64_t n, int learned,
-  double* macro, unsigned long long* probes, hipStream_t st);
+  double* macro, unsigned long long* probes, hipStream_t st, int group = 4);
 void launch_copy(const void* src, void* dst, int64_t n_bytes, hipStream_t st);
 
 // The allocation-free variant of launch_lookup_batch (xs_lookup.hip): E,

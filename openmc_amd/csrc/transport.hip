@@ -39,18 +39,20 @@ __device__ __forceinline__ int lane_id()
 }
 
 // Key of a particle about to calculate cross sections, for the counting sort
-// of the calculate-xs queue: (material + 1, energy bucket), the bucket being
-// the sign, exponent and leading mantissa bits of E offset by those of the
-// grid minimum.  The kernel that appends the particle stores it beside the slot.
+// of the calculate-xs queue: 0 for a particle in void, else
+// 1 + material * sort_eb + energy bucket, the bucket being the exponent and
+// leading mantissa bits of E offset by those of the grid minimum (sort_eb
+// buckets span the binades of the data's energy range).  The kernel that
+// appends the particle stores it beside the slot.
 __device__ __forceinline__ int sort_key(const DevParticles& S, int material, double E)
 {
-  const int mat = material + 1; // void -> 0
+  if (material < 0)
+    return 0;
   const uint64_t bits = static_cast<uint64_t>(__double_as_longlong(E));
   int eb = static_cast<int>(bits >> S.sort_shift) - S.sort_base;
   eb = eb < 0 ? 0 : (eb >= S.sort_eb ? S.sort_eb - 1 : eb);
-  int key = mat * S.sort_eb + eb;
-  key = key < S.sort_nb ? key : S.sort_nb - 1;
-  return key < 0 ? 0 : key;
+  int key = 1 + material * S.sort_eb + eb;
+  return key < S.sort_nb ? key : S.sort_nb - 1;
 }
 
 __device__ __forceinline__ int wave_slot()
@@ -401,7 +403,7 @@ struct Part {
   int* n_out;
   int* bin_tot;
   int nb;
-  int shift; // keys are used >> shift (the material part of a calculate-xs key)
+  int div; // 0: keys as they are; else the material part of a calculate-xs key, 1 + (key - 1) / div
 };
 
 // A partition of n entries uses nw(n) = clamp(ceil(n / 2048), 1, max_blocks)
@@ -419,7 +421,7 @@ __device__ __forceinline__ int sort_workgroups(int n, int max_blocks)
 __device__ __forceinline__ int part_key(const Part& A, int i)
 {
   const int k = A.key[i];
-  return k < 0 ? -1 : (A.nb == 1 ? 0 : k >> A.shift);
+  return k < 0 ? -1 : (A.nb == 1 ? 0 : (A.div == 0 || k == 0 ? k : 1 + (k - 1) / A.div));
 }
 
 // Per-workgroup key counts of its chunk into row b of the count matrix
@@ -600,13 +602,10 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
       i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
       const int j0 = P.mat_off[material];
       nn = P.mat_off[material + 1] - j0;
-      double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
       int probes = 0;
       for (int jj = sub; jj < nn; jj += LPP) {
         const int j = j0 + jj;
         Micro m = nuclide_xs(P, P.mat_nuc[j], E, i_log, lx, learned, probes);
-        if (!S.recompute_micro)
-          micro[jj] = m.total;
         const double dens = P.mat_dens[j] * 1.0;
         prod[2 * jj] = dens * m.total;
         prod[2 * jj + 1] = dens * m.absorption;
@@ -619,9 +618,12 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
     if (need && sub == 0) {
       double xs_t = 0.0;
       double xs_a = 0.0;
+      double* row = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
       for (int jj = 0; jj < nn; ++jj) {
         xs_t += prod[2 * jj];
         xs_a += prod[2 * jj + 1];
+        if (!S.recompute_micro)
+          row[jj] = xs_t; // running sums (see store_row)
       }
       S.hot[p].xs_t = xs_t;
       S.hot[p].xs_a = xs_a;
@@ -641,22 +643,25 @@ __global__ void __launch_bounds__(BLOCK) k_xs(DevProblem P, DevParticles S, cons
   wave_count(S, CNT_SEARCH_PROBES, n_probes);
 }
 
-// The kk (<= G) nuclide totals of a group into the particle's micro_t row,
-// two per 16-byte store: the group starts at a multiple of G in a row padded
-// to a multiple of four values, so the pairs are 16-byte aligned.
+// kk (<= G) values of a group into the particle's micro_t row, two per
+// 16-byte store: the group starts at a multiple of G in a row padded to a
+// multiple of four values, so the pairs are 16-byte aligned.  The row holds
+// the running sums of atom density x micro total in material order (the
+// macroscopic total's partial sums), which the collision searches for the
+// sampled nuclide.
 template<int G>
-__device__ __forceinline__ void store_micro(double* dst, const Micro* m, int kk)
+__device__ __forceinline__ void store_row(double* dst, const double* v, int kk)
 {
   if (G == 1) {
-    dst[0] = m[0].total;
+    dst[0] = v[0];
     return;
   }
 #pragma unroll
   for (int k = 0; k < G; k += 2) {
     if (k + 1 < kk)
-      *reinterpret_cast<double2*>(dst + k) = double2 {m[k].total, m[k + 1].total};
+      *reinterpret_cast<double2*>(dst + k) = double2 {v[k], v[k + 1]};
     else if (k < kk)
-      dst[k] = m[k].total;
+      dst[k] = v[k];
   }
 }
 
@@ -685,19 +690,20 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
     const int kk = nn - jj < G ? nn - jj : G;
     Micro m[G];
     nuclide_xs_group<G, LEARNED>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
-    if (!S.recompute_micro)
-      if (!S.recompute_micro)
-        store_micro<G>(micro + jj, m, kk);
+    double run[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
         const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
+        run[k] = xs_t;
         if (trace)
           hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
       }
     }
+    if (!S.recompute_micro)
+      store_row<G>(micro + jj, run, kk);
   }
   // xs_t and xs_a are adjacent and 16-byte aligned in the hot record: one store
   *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
@@ -736,18 +742,20 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
     const int kk = nn - jj < G ? nn - jj : G;
     Micro m[G];
     nuclide_xs_wave<G>(P, P.matnuc + j0 + jj, kk, E, i_log, il_lo, il_hi, act, probes, m);
-    if (act)
-      store_micro<G>(micro + jj, m, kk);
+    double run[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
         const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
+        run[k] = xs_t;
         if (trace)
           hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
       }
     }
+    if (act && !S.recompute_micro)
+      store_row<G>(micro + jj, run, kk);
   }
   if (act) {
     *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
@@ -1226,27 +1234,24 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
       }
     }
   } else {
-    // the row is read two values per 16-byte load (rows are padded to a
-    // multiple of four values)
-    const double2* micro2 = reinterpret_cast<const double2*>(S.micro_t + static_cast<int64_t>(p) * S.max_nn);
-    for (int j = j0; j < j1; j += 2) {
-      const double2 mm = micro2[(j - j0) >> 1];
-      const double dens = P.mat_dens[j] * 1.0;
-      prob += dens * mm.x;
-      if (prob >= cutoff) {
-        i_nuclide = P.mat_nuc[j];
-        j_nuclide = j;
-        break;
+    // the row holds the running sums of density x micro total in material
+    // order, the very values the reference's loop accumulates
+    // (src/physics.cpp:494-518): the sampled nuclide is the first whose sum
+    // reaches the cutoff, found by a binary search (the sums never decrease)
+    const double* row = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
+    int lo = 0, cnt = j1 - j0;
+    while (cnt > 0) {
+      const int step = cnt >> 1;
+      if (row[lo + step] < cutoff) {
+        lo += step + 1;
+        cnt -= step + 1;
+      } else {
+        cnt = step;
       }
-      if (j + 1 < j1) {
-        const double dens1 = P.mat_dens[j + 1] * 1.0;
-        prob += dens1 * mm.y;
-        if (prob >= cutoff) {
-          i_nuclide = P.mat_nuc[j + 1];
-          j_nuclide = j + 1;
-          break;
-        }
-      }
+    }
+    if (lo < j1 - j0) {
+      i_nuclide = P.mat_nuc[j0 + lo];
+      j_nuclide = j0 + lo;
     }
     // micro cross sections of the sampled nuclide at this energy
     if (i_nuclide >= 0)
@@ -1419,18 +1424,20 @@ __device__ __forceinline__ void xs_one(const DevProblem& P, const DevParticles& 
     const int kk = nn - jj < HG ? nn - jj : HG;
     Micro m[HG];
     nuclide_xs_group<HG, LEARNED, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, probes, m);
-    if (!S.recompute_micro)
-      store_micro<HG>(micro + jj, m, kk);
+    double run[HG];
 #pragma unroll
     for (int k = 0; k < HG; ++k) {
       if (k < kk) {
         const double dens = P.matnuc[j0 + jj + k].dens * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
+        run[k] = xs_t;
         if (trace)
           hmix(hh, static_cast<uint64_t>(static_cast<uint32_t>(m[k].i_grid)));
       }
     }
+    if (!S.recompute_micro)
+      store_row<HG>(micro + jj, run, kk);
   }
   *reinterpret_cast<double2*>(&h->xs_t) = double2 {xs_t, xs_a};
   if (trace)
@@ -1651,9 +1658,8 @@ void kernel_nl(int which, const DevProblem& P, const DevParticles& S, const Laun
     // staged survivors -> q_xs, sorted by (material, energy bucket) key, by
     // material only on the cycles between full sorts, or only compacted
     const bool full = lc.sort_every <= 1 || cycle % lc.sort_every == 0;
-    const int eb_shift = __builtin_ctz(static_cast<unsigned>(S.sort_eb));
     Part A {S.stage_p, S.stage_key, S.qcount + QC_STAGE, S.q_xs, S.qcount + QC_XS, nullptr,
-      !lc.sort ? 1 : (full ? S.sort_nb : S.sort_nb >> eb_shift), full ? 0 : eb_shift};
+      !lc.sort ? 1 : (full ? S.sort_nb : 1 + (S.sort_nb - 1) / S.sort_eb), full ? 0 : S.sort_eb};
     launch_partition(S, A, lc, st);
     break;
   }

@@ -45,28 +45,44 @@ __global__ void __launch_bounds__(LB_BLOCK) k_lookup_batch(DevProblem P, int i_n
     atomicAdd(probes_out, probes_sum);
 }
 
+// Material::calculate_neutron_xs over an array of energies with the
+// calculate-xs kernel's lookup code: every wavefront works on the one
+// material, so the nuclide records are wavefront-uniform, and each lane
+// interleaves G nuclide lookups (nuclide_xs_group); sums in material order.
+template<int G, bool LEARNED>
 __global__ void __launch_bounds__(LB_BLOCK) k_macro_batch(DevProblem P, int material, const double* E, int64_t n,
-  int learned, double* macro, unsigned long long* probes_out)
+  double* macro, unsigned long long* probes_out)
 {
   unsigned long long probes_sum = 0;
-  const int j0 = P.mat_off[material];
-  const int j1 = P.mat_off[material + 1];
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    double e = E[i];
-    double lx = detm::log(e / P.energy_min);
-    int i_log = clamp_log_index(P, lx);
+  const int j0 = wave_uniform(P.mat_off[material]);
+  const int nn = wave_uniform(P.mat_off[material + 1]) - j0;
+  for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; i0 < n;
+       i0 += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    // lanes past the end repeat the last energy and store nothing, so the
+    // whole wavefront runs the group lookups together
+    const double e = E[i < n ? i : n - 1];
+    const double lx = detm::log(e / P.energy_min);
+    const int i_log = clamp_log_index(P, lx);
     int probes = 0;
     double xs_t = 0.0, xs_a = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      Micro m = nuclide_xs(P, P.mat_nuc[j], e, i_log, lx, learned != 0, probes);
-      const double dens = P.mat_dens[j] * 1.0;
-      xs_t += dens * m.total;
-      xs_a += dens * m.absorption;
+    for (int jj = 0; jj < nn; jj += G) {
+      const int kk = nn - jj < G ? nn - jj : G;
+      Micro m[G];
+      nuclide_xs_group<G, LEARNED>(P, P.matnuc + j0 + jj, kk, e, i_log, lx, probes, m);
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        if (k < kk) {
+          const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
+          xs_t += dens * m[k].total;
+          xs_a += dens * m[k].absorption;
+        }
+      }
     }
-    macro[2 * i] = xs_t;
-    macro[2 * i + 1] = xs_a;
-    probes_sum += probes;
+    if (i < n) {
+      *reinterpret_cast<double2*>(macro + 2 * i) = double2 {xs_t, xs_a};
+      probes_sum += probes;
+    }
   }
   for (int o = 32; o > 0; o >>= 1)
     probes_sum += __shfl_xor(probes_sum, o);
@@ -240,11 +256,25 @@ void launch_lookup_batch(const DevProblem& P, int i_nuc, const double* E, int64_
 }
 
 void launch_macro_batch(const DevProblem& P, int material, const double* E, int64_t n, int learned, double* macro,
-  unsigned long long* probes, hipStream_t st)
+  unsigned long long* probes, hipStream_t st, int group)
 {
   int64_t b = (n + LB_BLOCK - 1) / LB_BLOCK;
   int blocks = static_cast<int>(b > 65536 ? 65536 : (b < 1 ? 1 : b));
-  hipLaunchKernelGGL(k_macro_batch, dim3(blocks), dim3(LB_BLOCK), 0, st, P, material, E, n, learned, macro, probes);
+#define OMC_MACRO(G, L)                                                                                               \
+  hipLaunchKernelGGL((k_macro_batch<G, L>), dim3(blocks), dim3(LB_BLOCK), 0, st, P, material, E, n, macro, probes)
+  if (n <= 0)
+    return;
+  const bool l = learned != 0;
+  if (group == 1) {
+    if (l) OMC_MACRO(1, true); else OMC_MACRO(1, false);
+  } else if (group == 2) {
+    if (l) OMC_MACRO(2, true); else OMC_MACRO(2, false);
+  } else if (group == 8) {
+    if (l) OMC_MACRO(8, true); else OMC_MACRO(8, false);
+  } else {
+    if (l) OMC_MACRO(4, true); else OMC_MACRO(4, false);
+  }
+#undef OMC_MACRO
 }
 
 } // namespace omc

@@ -710,6 +710,15 @@ def lookup_batch(i_nuc, energies, learned=False):
     return ig, xs, probes.value
 
 
+def macro_nuclides(material):
+    """Number of nuclides of a material (by index)."""
+    nuc = POINTER(c_int)()
+    dens = POINTER(c_double)()
+    n = c_int()
+    _dll.openmc_material_get_densities(material, nuc, dens, n)
+    return n.value
+
+
 def macro_batch(material, energies, learned=False, repeat=1):
     E = np.ascontiguousarray(energies, dtype=np.float64)
     out = np.zeros((E.size, 2), dtype=np.float64)

@@ -550,16 +550,20 @@ struct Problem {
   std::vector<double> energy_bins; // empty -> no energy filter
   int64_t n_tally_bins = 0;
 
-  // mesh weight windows for neutrons (src/weight_windows.cpp), one set
+  // mesh weight windows for neutrons (src/weight_windows.cpp): the sets in
+  // definition order
   bool ww_on = false;
-  Mesh ww_mesh;
-  std::vector<double> ww_ebounds;
-  std::vector<double> ww_lower, ww_upper; // [n_energy][n_mesh_bins]
-  int64_t ww_n_mesh = 0;
-  double ww_survival_ratio = 3.0;
-  double ww_max_lb_ratio = 1.0;
-  int ww_max_split = 10;
-  double ww_weight_cutoff = 1.0e-38;
+  struct WWSet {
+    Mesh mesh;
+    std::vector<double> ebounds;
+    std::vector<double> lower, upper; // [n_energy][n_mesh_bins]
+    int64_t n_mesh = 0;
+    double survival_ratio = 3.0;
+    double max_lb_ratio = 1.0;
+    int max_split = 10;
+    double weight_cutoff = 1.0e-38;
+  };
+  std::vector<WWSet> ww_sets;
   bool ww_ckpt_collision = true; // src/settings.cpp:86-87
   bool ww_ckpt_surface = false;
   int64_t max_history_splits = 10000000; // src/settings.cpp:121
@@ -1174,25 +1178,38 @@ inline int lower_bound_index(const std::vector<double>& v, double x)
   return static_cast<int>(std::lower_bound(v.begin(), v.end(), x) - v.begin()) - 1;
 }
 
-WeightWindow get_weight_window(const Problem& P, Particle& p)
+// WeightWindows::get_weight_window (src/weight_windows.cpp:375-411) of one set
+WeightWindow set_weight_window(const Problem::WWSet& W, Particle& p)
 {
   WeightWindow ww;
   int ijk[3] = {0, 0, 0};
   bool in_mesh;
-  mesh_indices(P.ww_mesh, p.r(), ijk, in_mesh);
+  mesh_indices(W.mesh, p.r(), ijk, in_mesh);
   if (!in_mesh)
     return ww;
   const double E = p.E;
-  if (E < P.ww_ebounds.front() || E > P.ww_ebounds.back())
+  if (E < W.ebounds.front() || E > W.ebounds.back())
     return ww;
-  const int e_bin = lower_bound_index(P.ww_ebounds, E);
-  const int64_t k = static_cast<int64_t>(e_bin) * P.ww_n_mesh + mesh_bin(P.ww_mesh, ijk);
-  ww.lower = P.ww_lower[k];
-  ww.upper = P.ww_upper[k];
-  ww.survival = ww.lower * P.ww_survival_ratio;
-  ww.max_lb_ratio = P.ww_max_lb_ratio;
-  ww.max_split = P.ww_max_split;
-  ww.cutoff = P.ww_weight_cutoff;
+  const int e_bin = lower_bound_index(W.ebounds, E);
+  const int64_t k = static_cast<int64_t>(e_bin) * W.n_mesh + mesh_bin(W.mesh, ijk);
+  ww.lower = W.lower[k];
+  ww.upper = W.upper[k];
+  ww.survival = ww.lower * W.survival_ratio;
+  ww.max_lb_ratio = W.max_lb_ratio;
+  ww.max_split = W.max_split;
+  ww.cutoff = W.weight_cutoff;
+  return ww;
+}
+
+// apply_weight_windows' search over the sets (src/weight_windows.cpp:70-75)
+WeightWindow get_weight_window(const Problem& P, Particle& p)
+{
+  WeightWindow ww;
+  for (const Problem::WWSet& W : P.ww_sets) {
+    ww = set_weight_window(W, p);
+    if (ww.valid())
+      break;
+  }
   return ww;
 }
 
@@ -2204,9 +2221,11 @@ void load_problem(Problem& P, const std::string& path)
     P.n_tally_bins = nb * (P.energy_bins.empty() ? 1 : static_cast<int64_t>(P.energy_bins.size()) - 1);
   }
   t.expect("ww");
-  P.ww_on = t.i() != 0;
-  if (P.ww_on) {
-    Mesh& m = P.ww_mesh;
+  const int n_sets = static_cast<int>(t.i());
+  P.ww_on = n_sets > 0;
+  for (int k = 0; k < n_sets; ++k) {
+    Problem::WWSet W;
+    Mesh& m = W.mesh;
     m.n_dim = static_cast<int>(t.i());
     for (int j = 0; j < 3; ++j)
       m.shape[j] = static_cast<int>(t.i());
@@ -2216,25 +2235,28 @@ void load_problem(Problem& P, const std::string& path)
       m.ur[j] = t.d();
     for (int j = 0; j < m.n_dim; ++j)
       m.width[j] = (m.ur[j] - m.ll[j]) / static_cast<double>(m.shape[j]);
-    P.ww_n_mesh = 1;
+    W.n_mesh = 1;
     for (int j = 0; j < m.n_dim; ++j)
-      P.ww_n_mesh *= m.shape[j];
+      W.n_mesh *= m.shape[j];
     const int ne = static_cast<int>(t.i());
     for (int j = 0; j < ne; ++j)
-      P.ww_ebounds.push_back(t.d());
+      W.ebounds.push_back(t.d());
     const int64_t nbnd = t.i();
-    if (ne < 2 || nbnd != P.ww_n_mesh * (ne - 1))
+    if (ne < 2 || nbnd != W.n_mesh * (ne - 1))
       throw std::runtime_error("weight window bounds do not match the mesh and energy bounds");
-    P.ww_lower.resize(nbnd);
-    P.ww_upper.resize(nbnd);
+    W.lower.resize(nbnd);
+    W.upper.resize(nbnd);
     for (int64_t j = 0; j < nbnd; ++j)
-      P.ww_lower[j] = t.d();
+      W.lower[j] = t.d();
     for (int64_t j = 0; j < nbnd; ++j)
-      P.ww_upper[j] = t.d();
-    P.ww_survival_ratio = t.d();
-    P.ww_max_lb_ratio = t.d();
-    P.ww_max_split = static_cast<int>(t.i());
-    P.ww_weight_cutoff = t.d();
+      W.upper[j] = t.d();
+    W.survival_ratio = t.d();
+    W.max_lb_ratio = t.d();
+    W.max_split = static_cast<int>(t.i());
+    W.weight_cutoff = t.d();
+    P.ww_sets.push_back(std::move(W));
+  }
+  if (P.ww_on) {
     P.ww_ckpt_collision = t.i() != 0;
     P.ww_ckpt_surface = t.i() != 0;
     P.max_history_splits = t.i();

@@ -136,28 +136,31 @@ def write_oracle_problem(model, nuclides, directory, data_dir="data"):
 
 
 def _weight_windows_tokens(s):
-    """ww 0 | ww 1 <mesh: nd shape[3] ll[3] ur[3]> <n_e+1 bounds> <n lower> <n upper>
-    survival max_lb_ratio max_split cutoff ckpt_collision ckpt_surface max_history_splits"""
-    wws = list(s.weight_windows or [])
+    """ww 0 | ww N <set>*N ckpt_collision ckpt_surface max_history_splits, a set being
+    <mesh: nd shape[3] ll[3] ur[3]> <n_e+1 bounds> <n lower> <n upper> survival max_lb_ratio
+    max_split cutoff"""
+    wws = [w for w in (s.weight_windows or []) if getattr(w, "particle_type", "neutron") == "neutron"]
     on = s.weight_windows_on if s.weight_windows_on is not None else len(wws) > 0
     if not wws or not on:
         return ["ww", "0"]
-    ww = wws[0]
-    m = ww.mesh
-    dim = list(m.dimension)
-    nd = len(dim)
-    shape = dim + [1] * (3 - nd)
-    ll = list(m.lower_left) + [0.0] * (3 - nd)
-    ur = list(m.upper_right) + [0.0] * (3 - nd)
-    eb = list(ww.energy_bounds) if ww.energy_bounds is not None else [0.0, float("inf")]
-    lo, hi = ww.flat_bounds()
+    out = ["ww", str(len(wws))]
+    for ww in wws:
+        m = ww.mesh
+        dim = list(m.dimension)
+        nd = len(dim)
+        shape = dim + [1] * (3 - nd)
+        ll = list(m.lower_left) + [0.0] * (3 - nd)
+        ur = list(m.upper_right) + [0.0] * (3 - nd)
+        eb = list(ww.energy_bounds) if ww.energy_bounds is not None else [0.0, float("inf")]
+        lo, hi = ww.flat_bounds()
+        out += [str(nd)] + [str(v) for v in shape] + [_f(v) for v in ll] + [_f(v) for v in ur]
+        out += [str(len(eb))] + [_f(v) for v in eb]
+        out += [str(lo.size)] + [_f(v) for v in lo] + [_f(v) for v in hi]
+        out += [_f(ww.survival_ratio),
+                _f(ww.max_lower_bound_ratio if ww.max_lower_bound_ratio is not None else 1.0),
+                str(int(ww.max_split)), _f(ww.weight_cutoff)]
     ck = s.weight_window_checkpoints or {}
-    out = ["ww", "1", str(nd)] + [str(v) for v in shape] + [_f(v) for v in ll] + [_f(v) for v in ur]
-    out += [str(len(eb))] + [_f(v) for v in eb]
-    out += [str(lo.size)] + [_f(v) for v in lo] + [_f(v) for v in hi]
-    out += [_f(ww.survival_ratio), _f(ww.max_lower_bound_ratio if ww.max_lower_bound_ratio is not None else 1.0),
-            str(int(ww.max_split)), _f(ww.weight_cutoff),
-            "1" if ck.get("collision", True) else "0", "1" if ck.get("surface", False) else "0",
+    out += ["1" if ck.get("collision", True) else "0", "1" if ck.get("surface", False) else "0",
             str(int(s.max_history_splits if s.max_history_splits is not None else 10_000_000))]
     return out
 

@@ -347,13 +347,17 @@ def test_weight_windows_bank_overflow_is_an_error():
         gpu_run(d, options={"bank_cap": 2})
 
 
-@pytest.mark.parametrize("bank_cap,tail,recompute", [(64, 65536, -1), (64, 0, -1), (64, 65536, 0)],
-                         ids=["history-tail", "event-only", "micro-rows"])
-def test_transport_weight_windows_histories(bank_cap, tail, recompute):
+@pytest.mark.parametrize("bank_cap,tail,recompute,sets", [(64, 65536, -1, 1), (64, 0, -1, 1), (64, 65536, 0, 1),
+                                                          (64, 65536, -1, 2)],
+                         ids=["history-tail", "event-only", "micro-rows", "two-sets"])
+def test_transport_weight_windows_histories(bank_cap, tail, recompute, sets):
     """Splitting, roulette and LIFO revival from the per-particle bank give the
     oracle's histories bit for bit (3-layer slab, 12-bin windows), with the
-    history-based tail and with event kernels only."""
-    d, model, nucs = prepare(4, **WW_SLAB)
+    history-based tail and with event kernels only; with two window sets a
+    particle takes the first set with a valid window at its position."""
+    print("case: weight windows bank %d tail %d recompute %d sets %d" % (bank_cap, tail, recompute, sets),
+          flush=True)
+    d, model, nucs = prepare(4, ww_split_sets=(sets == 2), **WW_SLAB)
     o = oracle_run(d, event_based=True)
     g = gpu_run(d, options={"trace_hash": 1, "bank_cap": bank_cap, "history_below": tail,
                             "recompute_micro": recompute})

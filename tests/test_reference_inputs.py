@@ -103,7 +103,7 @@ def test_fixed_source_model(lib, tmp_path):
 def test_weightwindows_model(lib, tmp_path):
     """weightwindows: as written it asks for photon transport and a particle
     filter (refused, named); without them the neutron windows, their mesh
-    and the mesh x energy tally read back exactly."""
+    and the mesh x energy tally read back exactly, both window sets in order."""
     txt = open(os.path.join(FIX, "weightwindows_model.xml")).read()
     xs, nucs = _library(str(tmp_path), _nuclide_names(txt))
     (tmp_path / "model.xml").write_text(txt)
@@ -114,24 +114,29 @@ def test_weightwindows_model(lib, tmp_path):
     t2 = re.sub(r'<filter id="3" type="particle">.*?</filter>', "", t2, flags=re.S)
     t2 = t2.replace("<filters>1 2 3</filters>", "<filters>1 2</filters>")
     (tmp_path / "model.xml").write_text(t2)
-    # two neutron window sets: the path takes one
-    with pytest.raises(lib.OpenMCError, match="one set of neutron weight windows"):
-        _load(lib, str(tmp_path), xs)
+    # both neutron window sets are read, in order (a particle takes the first
+    # set with a valid window at its position and energy)
     ww1 = re.search(r'(<weight_windows id="1">.*?</weight_windows>)', t2, flags=re.S).group(1)
-    t3 = re.sub(r'<weight_windows id="2">.*?</weight_windows>', "", t2, flags=re.S)
-    (tmp_path / "model.xml").write_text(t3)
+    ww2 = re.search(r'(<weight_windows id="2">.*?</weight_windows>)', t2, flags=re.S).group(1)
     _load(lib, str(tmp_path), xs)
     try:
         assert lib.settings.weight_windows_on is True
         from ctypes import POINTER, byref, c_double, c_int, c_int32, c_size_t
-        lo, hi = POINTER(c_double)(), POINTER(c_double)()
-        n = c_size_t()
-        lib._dll.openmc_weight_windows_get_bounds(0, lo, hi, n)
-        want_lo = np.array(re.search(r"<lower_ww_bounds>(.*?)</lower_ww_bounds>", ww1, re.S).group(1).split(), float)
-        want_hi = np.array(re.search(r"<upper_ww_bounds>(.*?)</upper_ww_bounds>", ww1, re.S).group(1).split(), float)
-        assert n.value == 5 * 6 * 7 * 2 == want_lo.size
-        assert np.array_equal(np.ctypeslib.as_array(lo, (n.value,)), want_lo)
-        assert np.array_equal(np.ctypeslib.as_array(hi, (n.value,)), want_hi)
+        assert lib._dll.openmc_weight_windows_size() == 2
+        for k, (wid, ww) in enumerate(((1, ww1), (2, ww2))):
+            idx = c_int32()
+            lib._dll.openmc_get_weight_windows_index(wid, byref(idx))
+            assert idx.value == k
+            lo, hi = POINTER(c_double)(), POINTER(c_double)()
+            n = c_size_t()
+            lib._dll.openmc_weight_windows_get_bounds(k, lo, hi, n)
+            want_lo = np.array(re.search(r"<lower_ww_bounds>(.*?)</lower_ww_bounds>", ww, re.S).group(1).split(),
+                               float)
+            want_hi = np.array(re.search(r"<upper_ww_bounds>(.*?)</upper_ww_bounds>", ww, re.S).group(1).split(),
+                               float)
+            assert n.value == 5 * 6 * 7 * 2 == want_lo.size
+            assert np.array_equal(np.ctypeslib.as_array(lo, (n.value,)), want_lo)
+            assert np.array_equal(np.ctypeslib.as_array(hi, (n.value,)), want_hi)
         eb, ne = POINTER(c_double)(), c_size_t()
         lib._dll.openmc_weight_windows_get_energy_bounds(0, eb, ne)
         assert list(np.ctypeslib.as_array(eb, (ne.value,))) == [0.0, 0.5, 2.0e7]
