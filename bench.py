@@ -104,6 +104,7 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
         lib.set_option("event_timing", 0)
         lib.set_option("progress_s", 30)
         lib.simulation_init()
+        s_first = lib.stats()
         tw = time.perf_counter()
         for k in range(warmup):
             lib.next_batch()
@@ -125,13 +126,14 @@ def run_gpu(d, steps, warmup, learned=False, n_seg=256, epochs=200, options=None
                 lib.next_batch()
                 log("  event-timed batch %d done (%.2f s)" % (k + 1, time.perf_counter() - tk))
             s2 = lib.stats()
+        s_last = lib.stats()
         lib.simulation_finalize()
         n = lib.settings.particles
         lib.finalize()
     finally:
         os.chdir(old)
     st = diff(s0, s1)
-    info.update({"elapsed_s": t1 - t0, "stats": st, "n_particles": n,
+    info.update({"elapsed_s": t1 - t0, "stats": st, "n_particles": n, "stats_all": diff(s_first, s_last),
                  "config_stats": {k: s1.get(k, 0) for k in CONFIG_STATS}})
     if timed_steps > 0:
         info["timed"] = {"stats": diff(s1b, s2), "steps": timed_steps,
@@ -606,7 +608,8 @@ def main():
             with open(args.events) as f:
                 for line in f:
                     if line.startswith("{"):
-                        events = json.loads(line).get("events")
+                        rec = json.loads(line)
+                        events = rec.get("events_all_batches") or rec.get("events")
         print(pmc_summary(args.pmc_summary, events, args.copy_calibration or None))
         return
     if args.gpus != 1:
@@ -668,6 +671,9 @@ def main():
                          "batches" % (args.timed_steps, args.steps),
         "kernels": per_kernel,
         "events": {k: st.get(k, 0) for k in EVENT_KEYS},
+        # every batch of the plain run (warm-up, timed, event-timed): what a
+        # profiler wrapped around a --no-learned run sees
+        "events_all_batches": {k: plain["stats_all"].get(k, 0) for k in EVENT_KEYS},
         "roofline": roof,
     }
     if copy_info:

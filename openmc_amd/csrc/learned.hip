@@ -12,8 +12,10 @@
 //   k_lforward   loss_s = sum_i (pred(f_i) - i)^2 over the segment's grid points
 //   k_lbackward  d loss_s / d(a_s, b_s) = sum_i 2 r_i (f_i, 1)
 //   k_ladam      Adam step on the mean gradient
-//   k_lerror     recentre b_s on the residual range, then record err_s, the
-//                largest |pred - lower bound| any query of the segment can
+//   k_lerror     pick the segment's line among the trained one (recentred)
+//                and lines through its sample points, by the expected search
+//                cost of the lookup's speculative bracket, and record err_s,
+//                the largest |pred - lower bound| any query of the segment can
 //                have, from the sub-interval ends (f = 0, each point, f = 1)
 //                with the FP32 inference formula.
 // The lookup then searches E[j - err - 1, j + err + 3) exactly (j = (int)pred),
@@ -237,14 +239,55 @@ __device__ double seg_bound(float2 ab, const double* u, int a0, int a1, int s)
   return amax;
 }
 
-// Recentre the segment's intercept on its points, record its error bound,
-// and keep instead a flat or end-to-end line where that bounds the error
-// more tightly.  A cluster of points that no segment can split (the points
-// of one narrow resonance) makes the trained slope steep, and a steep line
-// is far off at the segment's ends; the flat line through the middle of the
-// segment's index range is within half that range everywhere.
+// The fraction of segment s's queries (taken uniform in f over [0, 1)) for
+// which the lookup's speculative bracket hits with line ab: a query between
+// points i - 1 and i (lower bound i) hits when pred(f) lies in [i - 1, i).
+// The prediction is taken in FP64 here; the value only ranks candidate lines,
+// the error bound is computed exactly by seg_bound.  Every lane of the
+// wavefront returns the same value.
+__device__ double seg_hits(float2 ab, const double* u, int a0, int a1, int s)
+{
+  const double a = static_cast<double>(ab.x), b = static_cast<double>(ab.y);
+  const double us = static_cast<double>(s);
+  double h = 0.0;
+  for (int i = a0 + static_cast<int>(threadIdx.x); i <= a1; i += LW) {
+    double lo = (i == a0) ? 0.0 : fmin(fmax(u[i - 1] - us, 0.0), 1.0);
+    double hi = (i == a1) ? 1.0 : fmin(fmax(u[i] - us, 0.0), 1.0);
+    const double di = static_cast<double>(i);
+    if (a > 0.0) {
+      lo = fmax(lo, (di - 1.0 - b) / a);
+      hi = fmin(hi, (di - b) / a);
+    } else if (a < 0.0) {
+      lo = fmax(lo, (di - b) / a);
+      hi = fmin(hi, (di - 1.0 - b) / a);
+    } else if (!(b >= di - 1.0 && b < di)) {
+      continue;
+    }
+    h += fmax(hi - lo, 0.0);
+  }
+  return wave_sum(h);
+}
+
+constexpr int LSAMPLES = 16; // sample points per segment for candidate lines
+
+// Choose the segment's line and record its error bound.  Candidates:
+//  - the trained line, recentred on its points' residual range
+//    pred(f_i) - i (so that a query between points i and i + 1 predicts
+//    inside [i, i + 1) and the speculative bracket hits);
+//  - the trained slope through each of LSAMPLES + 1 sample points, and the
+//    chord through each pair of neighbouring sample points: where a cluster
+//    of points that no segment can split (the points of one narrow
+//    resonance) sits in the segment, these follow the grid on one side of it
+//    exactly instead of missing it by half the cluster everywhere;
+//  - the flat line through the middle of the segment's index range and the
+//    end-to-end line, whose bounds are the smallest where the cluster is
+//    large.
+// Among the candidates whose bound is within cap, the one with the smallest
+// expected search cost (miss fraction x the probes of its window) is kept;
+// with none within cap, the one with the smallest bound (the build then
+// doubles the nuclide's segments).
 __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, const double* uu,
-  const int* seg_start, int n_seg, float2* params, int* err)
+  const int* seg_start, int n_seg, int cap, float2* params, int* err)
 {
   const int n = list[blockIdx.y];
   const int y = blockIdx.y;
@@ -254,14 +297,12 @@ __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, co
   const int64_t k = static_cast<int64_t>(y) * (n_seg + 1) + s;
   const int a0 = seg_start[k], a1 = seg_start[k + 1];
   const int64_t j = static_cast<int64_t>(y) * n_seg + s;
-  float2 ab = params[j];
-  // recentre on the points' residual range, pred(f_i) - i, so that a query
-  // between points i and i + 1 predicts inside [i, i + 1) and the lookup's
-  // speculative bracket hits
+  const double us = static_cast<double>(s);
+  float2 trained = params[j];
   {
     double rmin = 1e300, rmax = -1e300;
     for (int i = a0 + threadIdx.x; i < a1; i += LW) {
-      const double r = static_cast<double>(predict(ab, u[i], s)) - static_cast<double>(i);
+      const double r = static_cast<double>(predict(trained, u[i], s)) - static_cast<double>(i);
       rmin = fmin(rmin, r);
       rmax = fmax(rmax, r);
     }
@@ -270,24 +311,59 @@ __global__ void __launch_bounds__(LW) k_lerror(DevProblem P, const int* list, co
       rmax = fmax(rmax, __shfl_xor(rmax, o));
     }
     if (rmax >= rmin)
-      ab.y = static_cast<float>(static_cast<double>(ab.y) - 0.5 * (rmax + rmin));
+      trained.y = static_cast<float>(static_cast<double>(trained.y) - 0.5 * (rmax + rmin));
   }
-  double best = seg_bound(ab, u, a0, a1, s);
-  const float2 flat {0.0f, static_cast<float>(0.5 * (static_cast<double>(a0) + static_cast<double>(a1)))};
-  const float2 ends {static_cast<float>(a1 - a0), static_cast<float>(a0)};
-  const double b_flat = seg_bound(flat, u, a0, a1, s);
-  const double b_ends = seg_bound(ends, u, a0, a1, s);
-  if (b_flat < best) {
-    best = b_flat;
-    ab = flat;
-  }
-  if (b_ends < best) {
-    best = b_ends;
-    ab = ends;
+  const int n_pts = a1 - a0;
+  const int n_cand = 3 + (n_pts >= 2 ? 2 * LSAMPLES + 1 : 0);
+  float2 best_ab = trained;
+  double best_bound = 1e300, best_cost = 1e300;
+  bool best_ok = false;
+  for (int c = 0; c < n_cand; ++c) {
+    float2 ab;
+    if (c == 0) {
+      ab = trained;
+    } else if (c == 1) {
+      ab = float2 {0.0f, static_cast<float>(0.5 * (static_cast<double>(a0) + static_cast<double>(a1)))};
+    } else if (c == 2) {
+      ab = float2 {static_cast<float>(a1 - a0), static_cast<float>(a0)};
+    } else if (c < 3 + LSAMPLES + 1) {
+      // trained slope through sample point t
+      const int t = c - 3;
+      const int p = a0 + static_cast<int>((static_cast<int64_t>(n_pts - 1) * t) / LSAMPLES);
+      const double fp = u[p] - us;
+      ab = float2 {trained.x,
+                   static_cast<float>(static_cast<double>(p) - static_cast<double>(trained.x) * fp)};
+    } else {
+      // chord through sample points t and t + 1
+      const int t = c - (3 + LSAMPLES + 1);
+      const int p = a0 + static_cast<int>((static_cast<int64_t>(n_pts - 1) * t) / LSAMPLES);
+      const int q = a0 + static_cast<int>((static_cast<int64_t>(n_pts - 1) * (t + 1)) / LSAMPLES);
+      const double fp = u[p] - us, fq = u[q] - us;
+      if (!(q > p && fq > fp))
+        continue;
+      const double slope = static_cast<double>(q - p) / (fq - fp);
+      ab = float2 {static_cast<float>(slope), static_cast<float>(static_cast<double>(p) - slope * fp)};
+    }
+    const double bound = seg_bound(ab, u, a0, a1, s);
+    const bool ok = ceil(bound) <= static_cast<double>(cap);
+    const double cost = ok ? (1.0 - seg_hits(ab, u, a0, a1, s)) * (log2(2.0 * ceil(bound) + 4.0) + 2.0) : 0.0;
+    bool take;
+    if (ok != best_ok)
+      take = ok;
+    else if (ok)
+      take = cost < best_cost || (cost == best_cost && bound < best_bound);
+    else
+      take = bound < best_bound;
+    if (take) {
+      best_ok = ok;
+      best_cost = cost;
+      best_bound = bound;
+      best_ab = ab;
+    }
   }
   if (threadIdx.x == 0) {
-    params[j] = ab;
-    err[j] = static_cast<int>(ceil(best));
+    params[j] = best_ab;
+    err[j] = static_cast<int>(ceil(best_bound));
   }
 }
 
@@ -423,8 +499,8 @@ int build_learned_index(Learned& L, const HostModel& m, DevProblem& P, int n_seg
         lr);
     }
     const std::vector<double> l1 = round_loss(P, r, params, st);
-    hipLaunchKernelGGL(k_lerror, dim3(ns, r.n_list), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, ns, params,
-      err);
+    hipLaunchKernelGGL(k_lerror, dim3(ns, r.n_list), dim3(LW), 0, st, P, r.d_list, r.uu, r.seg_start, ns,
+      max_err_bound, params, err);
     lcheck(hipGetLastError(), "learned training");
     const std::vector<double> l2 = round_loss(P, r, params, st);
     std::vector<float> h_par(2 * np);
