@@ -475,7 +475,8 @@ def pmc_summary(dirs, events=None, copy_bytes=None):
 
 
 WORKLOADS = {0: "cube-4nuclides-1e5pps-mesh10^3", 1: "lattice-17x17-60nuclides-1e7pps-mesh100^3",
-             2: "cube-1m-300nuclides-200kpoints-1e7pps-mesh100^3"}
+             2: "cube-1m-300nuclides-200kpoints-1e7pps-mesh100^3",
+             4: "slab-5x20cm-20nuclides-per-layer-weight-windows-50x50x100-mesh200^3x10groups"}
 EVENT_KEYS = ("xs_events", "adv_events", "surf_events", "coll_events", "lookups", "collisions", "segments",
               "search_probes", "cycles", "hist_launches", "hist_xs_events", "hist_lookups", "hist_adv_events",
               "hist_segments", "hist_surf_events", "hist_coll_events", "scatter_fallthrough")
@@ -641,11 +642,15 @@ def main():
                                                                                     args.timed_steps))
     opts = {k: float(v) for k, v in (o.split("=", 1) for o in args.option)}
     plain = run_gpu(d, args.steps, args.warmup, options=opts, timed_steps=args.timed_steps)
-    dom, roof, per_kernel = roofline(plain["timed"], config=args.config)
+    if "timed" in plain:
+        dom, roof, per_kernel = roofline(plain["timed"], config=args.config)
+    else:
+        # --timed-steps 0: no event-timed batch, so no per-kernel times
+        dom, roof, per_kernel = None, None, None
     n = plain["n_particles"]
     value = n * args.steps / plain["elapsed_s"]
     st = plain["stats"]
-    tst = plain["timed"]["stats"]
+    tst = plain["timed"]["stats"] if "timed" in plain else None
     out = {
         "metric": "particles/s and XS lookups/s on 1 MI355X: learned-index vs binary search",
         "value": round(value, 2),
@@ -662,7 +667,7 @@ def main():
         "config": {"workload": WORKLOADS.get(args.config, "config%d" % args.config), "particles_per_batch": n,
                    "nuclides": len(nucs), "grid_points": int(sum(x.n_points for x in nucs)),
                    "mode": "event-based, hipGraph-replayed event cycles"},
-        "xs_lookups_per_s": round(kxs_lookups(tst) / tst["t_xs_ms"] * 1e3, 1),
+        "xs_lookups_per_s": round(kxs_lookups(tst) / tst["t_xs_ms"] * 1e3, 1) if tst else None,
         "xs_lookups_per_s_transport": round(xs_lookups(st) / plain["elapsed_s"], 1),
         "events_per_particle": round((st["xs_events"] + st["adv_events"] + st["surf_events"] +
                                       st["coll_events"]) / (n * args.steps), 2),

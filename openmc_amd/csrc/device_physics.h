@@ -646,10 +646,12 @@ __device__ __forceinline__ int lower_bound_idx(const double* E, int first, int c
 
 // Learned bounded search.  The nuclide's piecewise-linear model maps
 // x = log(E/E_min) to a predicted interval index j; the recorded bound err of
-// the segment (max |i - pred(x_i)| over the grid points the segment can see,
-// evaluated with this same FP32 arithmetic) guarantees that the global
-// lower_bound of E lies in [j - err - 1, j + err + 3), which is then searched
-// exactly.  Returns the global lower_bound index (first i with E[i] >= E).
+// the segment (the largest |pred - lower bound| a query of the segment can
+// see, evaluated with this same FP32 arithmetic) guarantees that the global
+// lower_bound of E lies in [j - err - 1, j + err + 3), which is searched
+// exactly.  The predicted interval is tried first: when E[j] < E < E[j + 1]
+// holds strictly, the lower bound is j + 1 and no search runs.  Returns the
+// global lower_bound index (first i with E[i] >= E).
 __device__ __forceinline__ int learned_lower_bound(const DevProblem& P, int i_nuc, const double* Eg, int n,
   double E, double lx, int& probes)
 {
@@ -664,6 +666,9 @@ __device__ __forceinline__ int learned_lower_bound(const DevProblem& P, int i_nu
   float fr = static_cast<float>(uu - static_cast<double>(s));
   float pred = sg.x * fr + sg.y;
   int j = static_cast<int>(pred);
+  const int js = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
+  if (Eg[js] < E && E < Eg[js + 1])
+    return js + 1;
   int lo = j - err - 1;
   int hi = j + err + 3;
   lo = lo < 0 ? 0 : (lo > n ? n : lo);
@@ -770,15 +775,29 @@ __device__ __forceinline__ double wave_uniform(double x)
   return __longlong_as_double(wave_uniform(static_cast<int64_t>(__double_as_longlong(x))));
 }
 
-// The value itself where the lanes need not agree (UNI false: one particle
-// per lane, any material), the wavefront's copy where they do.
+// A read-only value at an address that is the same in every active lane of
+// the wavefront (the records of the wavefront's material): read through the
+// scalar data cache into scalar registers (a constant-address-space load from
+// the first active lane's address), so the vector-memory pipeline carries
+// only the per-lane gathers.  Only for data that no kernel writes (the
+// nuclear data and material records, uploaded at set-up).
+template<class T>
+__device__ __forceinline__ T uniform_load(const T* p)
+{
+  const auto a = wave_uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(p)));
+  return *reinterpret_cast<const __attribute__((address_space(4))) T*>(static_cast<uintptr_t>(a));
+}
+
+// The value at p: each lane's own where the lanes need not agree (UNI false:
+// one particle per lane, any material), the wavefront's scalar copy where
+// they do.
 template<bool UNI, class T>
-__device__ __forceinline__ T uni(T x)
+__device__ __forceinline__ T uld(const T* p)
 {
   if constexpr (UNI)
-    return wave_uniform(x);
+    return uniform_load(p);
   else
-    return x;
+    return *p;
 }
 
 // UNI (default) requires the wavefront-uniform material above; with UNI
@@ -795,11 +814,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const DevMatNuc* r = mn + (k < kk ? k : 0);
-    nuc[k] = uni<UNI>(r->nuc);
-    off[k] = uni<UNI>(r->off);
-    n[k] = uni<UNI>(r->n);
-    e_first[k] = uni<UNI>(r->e_first);
-    e_last[k] = uni<UNI>(r->e_last);
+    nuc[k] = uld<UNI>(&r->nuc);
+    off[k] = uld<UNI>(&r->off);
+    n[k] = uld<UNI>(&r->n);
+    e_first[k] = uld<UNI>(&r->e_first);
+    e_last[k] = uld<UNI>(&r->e_last);
     Eg[k] = P.E + off[k];
   }
   // final bracket values; with the learned index they are first read
@@ -826,11 +845,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const DevMatNuc* r = mn + (k < kk ? k : 0);
-      uu[k] = (logE - uni<UNI>(r->u_lo)) * uni<UNI>(r->inv_w);
-      const int ns = uni<UNI>(r->seg_n);
+      uu[k] = (logE - uld<UNI>(&r->u_lo)) * uld<UNI>(&r->inv_w);
+      const int ns = uld<UNI>(&r->seg_n);
       int sgi = uu[k] < 0.0 ? 0 : static_cast<int>(uu[k]);
       seg[k] = sgi >= ns ? ns - 1 : sgi;
-      sg[k] = Lm.rec[uni<UNI>(r->seg_off) + seg[k]];
+      sg[k] = Lm.rec[uld<UNI>(&r->seg_off) + seg[k]];
     }
     int js[K];
 #pragma unroll
@@ -1003,11 +1022,11 @@ __device__ __forceinline__ void nuclide_xs_wave(const DevProblem& P, const DevMa
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const DevMatNuc* r = mn + (k < kk ? k : 0);
-    const int nuc = wave_uniform(r->nuc);
-    off[k] = wave_uniform(r->off);
-    n[k] = wave_uniform(r->n);
-    const double e_first = wave_uniform(r->e_first);
-    const double e_last = wave_uniform(r->e_last);
+    const int nuc = uniform_load(&r->nuc);
+    off[k] = uniform_load(&r->off);
+    n[k] = uniform_load(&r->n);
+    const double e_first = uniform_load(&r->e_first);
+    const double e_last = uniform_load(&r->e_last);
     Eg[k] = P.E + off[k];
     const int* gi = P.grid_index + static_cast<int64_t>(nuc) * (P.n_log_bins + 1);
     lo_w[k] = wave_uniform(gi[il_lo]);

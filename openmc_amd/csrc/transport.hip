@@ -675,8 +675,8 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
   const double lx = detm::log(E / P.energy_min);
   int i_log = static_cast<int>(lx / P.log_spacing);
   i_log = i_log < 0 ? 0 : (i_log >= P.n_log_bins ? P.n_log_bins - 1 : i_log);
-  const int j0 = wave_uniform(P.mat_off[material]);
-  const int nn = wave_uniform(P.mat_off[material + 1]) - j0;
+  const int j0 = uniform_load(P.mat_off + material);
+  const int nn = uniform_load(P.mat_off + material + 1) - j0;
   double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
   uint64_t hh = 0;
   if (trace) {
@@ -694,7 +694,7 @@ __device__ __forceinline__ int particle_xs(const DevProblem& P, const DevParticl
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
-        const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
+        const double dens = uniform_load(&P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
         run[k] = xs_t;
@@ -727,8 +727,8 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
   // the log-grid range of the participating lanes
   const int il_lo = wave_min_int(act ? i_log : P.n_log_bins);
   const int il_hi = wave_max_int(act ? i_log : -1);
-  const int j0 = wave_uniform(P.mat_off[material]);
-  const int nn = wave_uniform(P.mat_off[material + 1]) - j0;
+  const int j0 = uniform_load(P.mat_off + material);
+  const int nn = uniform_load(P.mat_off + material + 1) - j0;
   double* micro = S.micro_t + static_cast<int64_t>(p) * S.max_nn;
   uint64_t hh = 0;
   if (trace && act) {
@@ -746,7 +746,7 @@ __device__ __forceinline__ int particle_xs_wave(const DevProblem& P, const DevPa
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < kk) {
-        const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
+        const double dens = uniform_load(&P.matnuc[j0 + jj + k].dens) * 1.0;
         xs_t += dens * m[k].total;
         xs_a += dens * m[k].absorption;
         run[k] = xs_t;
@@ -1207,31 +1207,45 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
   if constexpr (RECOMP) {
     // the micro totals again, CG nuclides at a time (their searches
     // interleaved), until the sum passes the cutoff: the same lookups as
-    // calculate-xs, so the same values
+    // calculate-xs, so the same values.  The lanes of one material at a time
+    // (one pass for almost every wavefront of the sorted collision queue), so
+    // that the material's nuclide records are the same in every active lane
+    // and are read through the scalar cache.
     constexpr int CG = 2; // measured: 2 beats 1 and 4 (4 costs 195 VGPRs)
     const bool learned = S.use_learned != 0;
-    for (int jj = 0; jj < j1 - j0 && i_nuclide < 0; jj += CG) {
-      const int kk = j1 - j0 - jj < CG ? j1 - j0 - jj : CG;
-      Micro mg[CG];
-      int pr = 0;
-      if (learned)
-        nuclide_xs_group<CG, true, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, pr, mg);
-      else
-        nuclide_xs_group<CG, false, false>(P, P.matnuc + j0 + jj, kk, E, i_log, lx, pr, mg);
-      probes += pr;
+    unsigned long long todo = __ballot(1);
+    while (todo != 0ull) {
+      const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      const int mat = __builtin_amdgcn_readfirstlane(__shfl(material, leader));
+      const bool mine = material == mat;
+      if (mine) {
+        const int u0 = uniform_load(P.mat_off + mat);
+        const int un = uniform_load(P.mat_off + mat + 1) - u0;
+        for (int jj = 0; jj < un && i_nuclide < 0; jj += CG) {
+          const int kk = un - jj < CG ? un - jj : CG;
+          Micro mg[CG];
+          int pr = 0;
+          if (learned)
+            nuclide_xs_group<CG, true>(P, P.matnuc + u0 + jj, kk, E, i_log, lx, pr, mg);
+          else
+            nuclide_xs_group<CG, false>(P, P.matnuc + u0 + jj, kk, E, i_log, lx, pr, mg);
+          probes += pr;
 #pragma unroll
-      for (int k = 0; k < CG; ++k) {
-        if (k < kk && i_nuclide < 0) {
-          const int j = j0 + jj + k;
-          const double dens = P.mat_dens[j] * 1.0;
-          prob += dens * mg[k].total;
-          if (prob >= cutoff) {
-            i_nuclide = P.mat_nuc[j];
-            j_nuclide = j;
-            m = mg[k];
+          for (int k = 0; k < CG; ++k) {
+            if (k < kk && i_nuclide < 0) {
+              const int j = u0 + jj + k;
+              const double dens = uniform_load(&P.matnuc[j].dens) * 1.0;
+              prob += dens * mg[k].total;
+              if (prob >= cutoff) {
+                i_nuclide = uniform_load(&P.matnuc[j].nuc);
+                j_nuclide = j;
+                m = mg[k];
+              }
+            }
           }
         }
       }
+      todo &= ~__ballot(mine);
     }
   } else {
     // the row holds the running sums of density x micro total in material

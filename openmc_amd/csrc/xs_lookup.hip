@@ -54,8 +54,8 @@ __global__ void __launch_bounds__(LB_BLOCK) k_macro_batch(DevProblem P, int mate
   double* macro, unsigned long long* probes_out)
 {
   unsigned long long probes_sum = 0;
-  const int j0 = wave_uniform(P.mat_off[material]);
-  const int nn = wave_uniform(P.mat_off[material + 1]) - j0;
+  const int j0 = uniform_load(P.mat_off + material);
+  const int nn = uniform_load(P.mat_off + material + 1) - j0;
   for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; i0 < n;
        i0 += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t i = i0 + threadIdx.x;
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(LB_BLOCK) k_macro_batch(DevProblem P, int mate
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         if (k < kk) {
-          const double dens = wave_uniform(P.matnuc[j0 + jj + k].dens) * 1.0;
+          const double dens = uniform_load(&P.matnuc[j0 + jj + k].dens) * 1.0;
           xs_t += dens * m[k].total;
           xs_a += dens * m[k].absorption;
         }
