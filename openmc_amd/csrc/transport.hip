@@ -949,10 +949,11 @@ __global__ void __launch_bounds__(BLOCK) k_xs_lane(DevProblem P, DevParticles S,
   xs_lane_body<NL, G, LEARNED, 0, WAVE>(P, S, q);
 }
 
-// The fused kernel is held to three wavefronts per SIMD (168 VGPRs), the
-// occupancy of the calculate-xs kernel alone.
+// The fused kernel is held to four wavefronts per SIMD (128 VGPRs; two of
+// its values then live in scratch): measured faster than three waves with
+// no scratch, its loads wait on the caches and more waves hide more of it.
 template<int NL, bool LEARNED, int ADV, int G>
-__global__ void __launch_bounds__(BLOCK, 3) k_xs_adv(DevProblem P, DevParticles S, const int* q)
+__global__ void __launch_bounds__(BLOCK, 4) k_xs_adv(DevProblem P, DevParticles S, const int* q)
 {
   xs_lane_body<NL, G, LEARNED, ADV, false>(P, S, q);
 }
@@ -1372,8 +1373,10 @@ __device__ __forceinline__ bool collision_one(const DevProblem& P, const DevPart
   return wgt != 0.0;
 }
 
+// Held to four wavefronts per SIMD (128 VGPRs), which it fits without
+// scratch; measured faster than the three its registers allowed otherwise.
 template<int NL, bool WW, bool RECOMP>
-__global__ void __launch_bounds__(BLOCK) k_collision(DevProblem P, DevParticles S)
+__global__ void __launch_bounds__(BLOCK, 4) k_collision(DevProblem P, DevParticles S)
 {
   const int n = S.qcount[QC_COLL];
   const int n_surf = S.qcount[QC_SURF]; // collisions follow the crossings in q_sc
