@@ -271,10 +271,12 @@ def cpu_baseline(d, target_s=4.0, repeats=3):
             "modes": modes}
 
 
-def run_problem_set(dirs, steps, warmup, options=None, tune_path=None, timed_steps=1):
+def run_problem_set(dirs, steps, warmup, options=None, tune_path=None, timed_steps=1, learned=False, n_seg=256,
+                    epochs=200):
     """The problems of a set in one shared event loop: W warm-up and K timed
     batches on the graph path (every problem advances one batch per step),
-    then `timed_steps` event-timed batches for the kernel table."""
+    then `timed_steps` event-timed batches for the kernel table; with
+    `learned`, on the learned index."""
     from openmc_amd import autotune, lib
     lib.init_problems(dirs, output=False)
     info = {}
@@ -286,6 +288,12 @@ def run_problem_set(dirs, steps, warmup, options=None, tune_path=None, timed_ste
             best, rec = autotune.tune_cached(dirs, tune_path, log=log)
             info["autotune"] = {"options": best, "seconds": round(time.perf_counter() - t, 2),
                                 "trials": rec["trials"], "from_cache": bool(rec.get("cached"))}
+        if learned:
+            t = time.perf_counter()
+            info["learned_max_err"] = lib.learned_build(n_seg, epochs)
+            info["learned_build_s"] = time.perf_counter() - t
+            info.update({"learned_" + k: v for k, v in lib.learned_stats().items()})
+            lib.set_option("learned_index", 1)
         lib.set_option("event_timing", 0)
         lib.simulation_init()
         for _ in range(warmup):
@@ -370,6 +378,8 @@ def bench_problem_set(args):
         alone.append({"particles": r["n_particles"], "ms_per_batch": round(1e3 * r["elapsed_s"], 3),
                       "particles_per_s": round(r["n_particles"] / r["elapsed_s"], 1)})
     seq_s = sum(a["ms_per_batch"] for a in alone) * 1e-3
+    log("GPU problem set on the learned index (the same launch options)")
+    rl = run_problem_set(dirs, args.steps, args.warmup, options=chosen, timed_steps=args.timed_steps, learned=True)
     out = {
         "metric": "particles/s and XS lookups/s on 1 MI355X: learned-index vs binary search",
         "value": round(value, 2),
@@ -390,6 +400,7 @@ def bench_problem_set(args):
         "sequential": {"particles_per_s": round(total / seq_s, 2), "seconds_per_batch": round(seq_s, 4)},
         "batching_speedup": round(value / (total / seq_s), 3),
         "per_problem": alone,
+        "learned": learned_summary(rs, rl, total, args.steps),
         "launch_options": rs["options"],
         "kernels": per_kernel,
         "events": {k: st.get(k, 0) for k in ("xs_events", "adv_events", "surf_events", "coll_events", "lookups",

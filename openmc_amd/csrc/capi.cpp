@@ -137,7 +137,9 @@ struct Sim {
   bool sort = true;
   int sort_blocks_opt = 0; // 0 = one workgroup per CU
   int blocks_opt = 0;      // 0 = from the capacity
-  int xs_group_opt = -1; // -1: 2 when collisions repeat the lookups, else 4
+  int xs_group_opt = -1; // -1: 2 (a lane's interleaved lookups: with 2 the calculate-xs
+                       // kernels keep four waves per SIMD; measured faster than 4
+                       // with either collision mode)
   int bank_cap = 32; // secondary bank sites per particle (weight windows)
   int sort_bits = -1; // mantissa bits of E in the calculate-xs sort key (-1: 5, or 8 for materials of more
                       // than 64 nuclides, whose 300-lookup events repay a finer order)
@@ -287,8 +289,7 @@ void upload_problem(Sim& s)
 
   std::vector<DevNuclide> nucs;
   s.nuc_off.clear();
-  std::vector<double> E, EL;
-  std::vector<double2> XS;
+  std::vector<double> E, EL, B;
   int64_t off = 0;
   for (const auto& n : m.nuclides) {
     DevNuclide d {};
@@ -304,16 +305,19 @@ void upload_problem(Sim& s)
   }
   E.reserve(off);
   EL.reserve(off);
-  XS.reserve(off);
+  B.reserve(3 * static_cast<size_t>(off));
   for (const auto& n : m.nuclides) {
     E.insert(E.end(), n.energy.begin(), n.energy.end());
     EL.insert(EL.end(), n.elastic.begin(), n.elastic.end());
-    for (size_t k = 0; k < n.energy.size(); ++k)
-      XS.push_back(double2 {n.total[k], n.absorption[k]});
+    for (size_t k = 0; k < n.energy.size(); ++k) {
+      B.push_back(n.energy[k]);
+      B.push_back(n.total[k]);
+      B.push_back(n.absorption[k]);
+    }
   }
   P.nuc = dev_upload(s, nucs);
   P.E = dev_upload(s, E);
-  P.XS = dev_upload(s, XS);
+  P.B = dev_upload(s, B);
   P.EL = dev_upload(s, EL);
   P.grid_index = dev_upload(s, m.grid_index);
   P.n_nuclides = static_cast<int>(m.nuclides.size());
@@ -492,7 +496,7 @@ void update_launch_config(Sim& s)
   s.lc.sort_blocks = std::min(s.lc.blocks, s.sort_blocks_opt > 0 ? s.sort_blocks_opt : 256);
   // auto: 2 nuclides a lane when calculate-xs writes no rows (1% faster than
   // 4 on the lattice problem, fused or not), else 4
-  s.lc.xs_group = s.xs_group_opt > 0 ? s.xs_group_opt : (recompute_on(s) ? 2 : 4);
+  s.lc.xs_group = s.xs_group_opt > 0 ? s.xs_group_opt : 2;
   s.lc.block = s.block_opt;
   s.lc.sort_every = s.sort_every;
   // auto: fused when the collisions repeat the lookups (then calculate-xs
@@ -1654,7 +1658,7 @@ int omc_set_option(const char* name, double value)
   } else if (n == "xs_group") {
     int v = static_cast<int>(value);
     if (v != -1 && v != 1 && v != 2 && v != 4 && v != 8) {
-      set_errmsg("xs_group must be 1, 2, 4 or 8 (or -1: by the collision mode)");
+      set_errmsg("xs_group must be 1, 2, 4 or 8 (or -1: the default, 2)");
       return OPENMC_E_INVALID_ARGUMENT;
     }
     g->xs_group_opt = v;
@@ -1904,7 +1908,7 @@ int omc_macro_batch(int mat, const double* E, int64_t n, int learned, int repeat
     double* dM = dev_alloc<double>(tmp, 2 * n);
     unsigned long long* dP = dev_alloc<unsigned long long>(tmp, 1);
     hip_check(hipMemcpy(dE, E, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
-    const int group = g->xs_group_opt > 0 ? g->xs_group_opt : 4;
+    const int group = g->xs_group_opt > 0 ? g->xs_group_opt : 2;
     launch_macro_batch(g->P, mat, dE, n, learned, dM, dP, g->stream, group); // warm-up
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");

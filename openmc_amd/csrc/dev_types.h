@@ -205,8 +205,9 @@ struct DevProblem {
   DevWW ww;
   // cross sections
   const DevNuclide* nuc;
-  const double* E;   // concatenated energy grids
-  const double2* XS; // {total, absorption} per grid point
+  const double* E;   // concatenated energy grids (the searches read these)
+  const double* B;   // {E, total, absorption} per grid point: the bracket of
+                     // interval i is 48 contiguous bytes from B + 3 i
   const double* EL;  // elastic per grid point
   const int* grid_index; // [n_nuc][n_log_bins + 1]
   int n_nuclides;

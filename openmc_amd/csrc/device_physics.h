@@ -683,6 +683,27 @@ struct Micro {
   double absorption;
 };
 
+// One grid point's {E, total, absorption} record, and the bracket of
+// interval g (points g and g + 1, global indices): 48 contiguous bytes, three
+// 16-byte loads touching one or two lines, where separate energy and cross
+// section arrays touch two or more.
+struct PointRec {
+  double e, t, a;
+};
+struct Bracket {
+  double e0, t0, a0, e1, t1, a1;
+};
+__device__ __forceinline__ PointRec load_point(const DevProblem& P, int64_t g)
+{
+  const double* b = P.B + 3 * g;
+  return PointRec {b[0], b[1], b[2]};
+}
+__device__ __forceinline__ Bracket load_bracket(const DevProblem& P, int64_t g)
+{
+  const double* b = P.B + 3 * g;
+  return Bracket {b[0], b[1], b[2], b[3], b[4], b[5]};
+}
+
 // Nuclide::calculate_xs, NEAREST temperature with one temperature
 __device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, double E, int i_log, double logE,
   bool learned, int& probes)
@@ -723,16 +744,17 @@ __device__ __forceinline__ Micro nuclide_xs(const DevProblem& P, int i_nuc, doub
   // which the parity tests check index for index)
   i_grid = i_grid < 0 ? 0 : (i_grid > n - 2 ? n - 2 : i_grid);
   // check for rare case where two energy points are the same
-  if (i_grid + 2 < n && Eg[i_grid] == Eg[i_grid + 1])
+  Bracket b = load_bracket(P, nd.off + i_grid);
+  if (i_grid + 2 < n && b.e0 == b.e1) {
     ++i_grid;
-  double f = (E - Eg[i_grid]) / (Eg[i_grid + 1] - Eg[i_grid]);
-  const double2 x0 = P.XS[nd.off + i_grid];
-  const double2 x1 = P.XS[nd.off + i_grid + 1];
+    b = load_bracket(P, nd.off + i_grid);
+  }
+  double f = (E - b.e0) / (b.e1 - b.e0);
   Micro m;
   m.i_grid = i_grid;
   m.f = f;
-  m.total = (1.0 - f) * x0.x + f * x1.x;
-  m.absorption = (1.0 - f) * x0.y + f * x1.y;
+  m.total = (1.0 - f) * b.t0 + f * b.t1;
+  m.absorption = (1.0 - f) * b.a0 + f * b.a1;
   return m;
 }
 
@@ -868,10 +890,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
     // speculative brackets at the predicted interval js
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      e0[k] = Eg[k][js[k]];
-      e1[k] = Eg[k][js[k] + 1];
-      x0[k] = P.XS[off[k] + js[k]];
-      x1[k] = P.XS[off[k] + js[k] + 1];
+      const Bracket b = load_bracket(P, off[k] + js[k]);
+      e0[k] = b.e0;
+      e1[k] = b.e1;
+      x0[k] = double2 {b.t0, b.a0};
+      x1[k] = double2 {b.t1, b.a1};
     }
     // E[js] < E < E[js + 1] makes the lower bound js + 1 with E not a grid
     // point (so neither the bucket-start case nor the duplicate bump applies):
@@ -944,10 +967,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
     int i = ig[k];
     i = i < 0 ? 0 : (i > n[k] - 2 ? n[k] - 2 : i);
     ig[k] = i;
-    e0[k] = Eg[k][i];
-    e1[k] = Eg[k][i + 1];
-    x0[k] = P.XS[off[k] + i];
-    x1[k] = P.XS[off[k] + i + 1];
+    const Bracket b = load_bracket(P, off[k] + i);
+    e0[k] = b.e0;
+    e1[k] = b.e1;
+    x0[k] = double2 {b.t0, b.a0};
+    x1[k] = double2 {b.t1, b.a1};
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -956,10 +980,11 @@ __device__ __forceinline__ void nuclide_xs_group(const DevProblem& P, const DevM
     int i = ig[k];
     if (!hit[k] && i + 2 < n[k] && e0[k] == e1[k]) {
       ++i;
+      const PointRec q = load_point(P, off[k] + i + 1);
       e0[k] = e1[k];
-      e1[k] = Eg[k][i + 1];
+      e1[k] = q.e;
       x0[k] = x1[k];
-      x1[k] = P.XS[off[k] + i + 1];
+      x1[k] = double2 {q.t, q.a};
     }
     const double f = (E - e0[k]) / (e1[k] - e0[k]);
     out[k].i_grid = i;
@@ -1119,13 +1144,13 @@ __device__ __forceinline__ void nuclide_xs_wave(const DevProblem& P, const DevMa
   double2 x0[K], x1[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int i = ig[k];
+    const Bracket b = load_bracket(P, off[k] + ig[k]);
     if (!have[k]) {
-      e0[k] = Eg[k][i];
-      e1[k] = Eg[k][i + 1];
+      e0[k] = b.e0;
+      e1[k] = b.e1;
     }
-    x0[k] = P.XS[off[k] + i];
-    x1[k] = P.XS[off[k] + i + 1];
+    x0[k] = double2 {b.t0, b.a0};
+    x1[k] = double2 {b.t1, b.a1};
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -1135,10 +1160,11 @@ __device__ __forceinline__ void nuclide_xs_wave(const DevProblem& P, const DevMa
     // duplicate grid point: the upper interval (src/nuclide.cpp:740-742)
     if (i + 2 < n[k] && e0[k] == e1[k]) {
       ++i;
+      const PointRec q = load_point(P, off[k] + i + 1);
       e0[k] = e1[k];
-      e1[k] = Eg[k][i + 1];
+      e1[k] = q.e;
       x0[k] = x1[k];
-      x1[k] = P.XS[off[k] + i + 1];
+      x1[k] = double2 {q.t, q.a};
     }
     const double f = (E - e0[k]) / (e1[k] - e0[k]);
     out[k].i_grid = i;
