@@ -67,12 +67,15 @@ def test_problem_set_rejects_mismatched_settings(small_set, tmp_path):
         lib.check_problem_set([dirs[0], bad])
 
 
-def _set_run(dirs, options=None):
+def _set_run(dirs, options=None, learned=False):
     from openmc_amd import lib
     lib.init_problems(dirs, output=False)
     try:
         for k, v in (options or {}).items():
             lib.set_option(k, v)
+        if learned:
+            assert lib.learned_build(256, 100) <= 64
+            lib.set_option("learned_index", 1)
         lib.run()
         h = lib.particle_hashes()
         out = []
@@ -109,6 +112,22 @@ def test_problem_set_matches_problems_alone(small_set, options):
             o = oracle_run(dirs[k])
             assert np.array_equal(got[k]["hashes"], o.hashes(n)), "problem %d differs from the oracle" % k
             assert_tallies_close(got[k]["results"], o.results())
+
+
+@pytest.mark.gpu
+def test_problem_set_learned_index_same_histories(small_set):
+    """The learned index, built over a problem set's shared nuclide table,
+    gives every problem the histories and tallies of binary search, with
+    fewer search probes."""
+    d, dirs, problems, nucs = small_set
+    ref, st_ref = _set_run(dirs, {"trace_hash": 1})
+    got, st = _set_run(dirs, {"trace_hash": 1}, learned=True)
+    for k in range(len(dirs)):
+        assert np.array_equal(got[k]["hashes"], ref[k]["hashes"]), "problem %d histories differ" % k
+        assert_tallies_close(got[k]["results"], ref[k]["results"])
+        assert got[k]["n_real"] == ref[k]["n_real"]
+    assert st["lookups"] == st_ref["lookups"] and st["collisions"] == st_ref["collisions"]
+    assert st["search_probes"] < st_ref["search_probes"]
 
 
 @pytest.mark.gpu
