@@ -411,6 +411,7 @@ struct Part {
 // short queues late in a batch scan a small count matrix.  Workgroup b owns
 // the contiguous chunk [b * chunk, min(n, (b + 1) * chunk)).
 constexpr int SORT_MIN_PER_BLOCK = 2048;
+constexpr int PART_UNROLL = 8; // queue entries a thread loads before using them
 
 __device__ __forceinline__ int sort_workgroups(int n, int max_blocks)
 {
@@ -439,7 +440,21 @@ __global__ void __launch_bounds__(BLOCK) k_part_count(DevParticles S, Part A)
   const int chunk = (n + nw - 1) / nw;
   const int lo = blockIdx.x * chunk;
   const int hi = min(n, lo + chunk);
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+  // PART_UNROLL keys loaded before their LDS atomics, so a thread keeps that
+  // many loads in flight instead of one
+  const int stride = blockDim.x;
+  int i = lo + threadIdx.x;
+  for (; i + (PART_UNROLL - 1) * stride < hi; i += PART_UNROLL * stride) {
+    int k[PART_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u)
+      k[u] = part_key(A, i + u * stride);
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u)
+      if (k[u] >= 0)
+        atomicAdd(&sh_cnt[k[u]], 1);
+  }
+  for (; i < hi; i += stride) {
     const int k = part_key(A, i);
     if (k >= 0)
       atomicAdd(&sh_cnt[k], 1);
@@ -531,7 +546,24 @@ __global__ void __launch_bounds__(BLOCK) k_part_scatter(DevParticles S, Part A)
   const int chunk = (n + nw - 1) / nw;
   const int lo = blockIdx.x * chunk;
   const int hi = min(n, lo + chunk);
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+  const int stride = blockDim.x;
+  int i = lo + threadIdx.x;
+  for (; i + (PART_UNROLL - 1) * stride < hi; i += PART_UNROLL * stride) {
+    int k[PART_UNROLL], v[PART_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) {
+      k[u] = part_key(A, i + u * stride);
+      v[u] = A.val[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) {
+      if (k[u] >= 0) {
+        const int pos = atomicAdd(&sh_off[k[u]], 1);
+        A.out[pos] = v[u];
+      }
+    }
+  }
+  for (; i < hi; i += stride) {
     const int k = part_key(A, i);
     if (k >= 0) {
       const int pos = atomicAdd(&sh_off[k], 1);
