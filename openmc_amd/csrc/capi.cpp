@@ -530,7 +530,9 @@ void alloc_particles(Sim& s, int64_t cap)
   S.lev = dev_alloc<PLevel>(L, static_cast<size_t>(deep * cap));
   // rows padded to a multiple of four values: stores and loads of the row
   // are 16-byte pairs
-  S.max_nn = (std::max(1, s.P.max_mat_nuc) + 3) & ~3;
+  // row stride: a multiple of eight values (64 bytes), so the 64-byte pieces
+  // calculate-xs writes at once are aligned to them
+  S.max_nn = (std::max(1, s.P.max_mat_nuc) + 7) & ~7;
   S.micro_t = dev_alloc<double>(L, static_cast<size_t>(S.max_nn) * cap);
   S.lost = dev_alloc<int>(L, cap);
   S.hash = dev_alloc<uint64_t>(L, cap);
@@ -601,7 +603,7 @@ int64_t bytes_per_particle(const Sim& s)
   // and staging arrays, and the weight-window record and bank when on
   const int64_t ww = s.P.ww.on ? static_cast<int64_t>(sizeof(PWW)) + s.bank_cap * sizeof(PSite) : 0;
   return static_cast<int64_t>(sizeof(PHot)) + deep * static_cast<int64_t>(sizeof(PLevel)) +
-         ((std::max(1, s.P.max_mat_nuc) + 3) & ~3) * 8 + 4 + 8 + 5 * 4 + ww;
+         ((std::max(1, s.P.max_mat_nuc) + 7) & ~7) * 8 + 4 + 8 + 5 * 4 + ww;
 }
 
 void choose_capacity(Sim& s)
