@@ -493,7 +493,7 @@ void update_launch_config(Sim& s)
   s.lc.blocks = s.blocks_opt > 0 ? s.blocks_opt : s.blocks;
   s.lc.xs_lanes = lanes;
   s.lc.sort = s.sort ? 1 : 0;
-  s.lc.sort_blocks = std::min(s.lc.blocks, s.sort_blocks_opt > 0 ? s.sort_blocks_opt : 256);
+  s.lc.sort_blocks = std::min({s.lc.blocks, s.sort_blocks_opt > 0 ? s.sort_blocks_opt : 256, SORT_MAX_BLOCKS});
   // auto: 2 nuclides a lane when calculate-xs writes no rows (1% faster than
   // 4 on the lattice problem, fused or not), else 4
   s.lc.xs_group = s.xs_group_opt > 0 ? s.xs_group_opt : 2;

@@ -15,7 +15,8 @@
 namespace omc {
 
 constexpr int MAX_LEVELS = 4;
-constexpr int SORT_MAX_BINS = 16384; // (material + 1, energy bucket) keys: a 64 KB LDS histogram
+constexpr int SORT_MAX_BINS = 32768; // (material + 1, energy bucket) keys: a 128 KB LDS histogram
+                                     // (gfx950 has 160 KB of LDS per CU)
 constexpr int SORT_MAX_BLOCKS = 256; // workgroups of the sort kernels
 
 // Constants of the reference (include/openmc/constants.h:49-113)
