@@ -1,7 +1,8 @@
 """GPU parity on the benchmark-size data.
 
-The 300-nuclide x 200k-point material (the lookup-dominated workload) and the
-full 60-nuclide 17x17 lattice with its 100^3 mesh, against the oracle:
+The 300-nuclide x 200k-point material (the lookup-dominated workload), the
+full 60-nuclide 17x17 lattice with its 100^3 mesh and the weight-window slab
+with its 200^3 x 10-group tally, against the oracle:
   * every lookup, plain and learned, index for index and bit for bit, on
     random energies, exact grid points, midpoints and every log-grid bucket
     edge of a sample of nuclides;
@@ -110,3 +111,26 @@ def test_cfg1_full_data_histories():
         assert np.array_equal(g["hashes"], h_o), "%d of %d histories differ (learned %s)" % (
             np.sum(g["hashes"] != h_o), n, learned)
         assert_tallies_close(g["results"], o.results())
+
+
+def test_cfg4_full_structure_histories():
+    """8 source particles of the deep-penetration slab as specified (5 layers
+    of 20 nuclides with 200k-point grids, weight windows on the 100x50x50
+    mesh with splitting and roulette, the 200^3 x 10-group flux tally): about
+    1e5 events a source particle through its split secondaries.  Hashes
+    bit-identical to the oracle's, tallies within 1e-12 per bin and the same
+    event counts, on binary search and on the learned index."""
+    print("case: cfg4 full structure, 8 particles", flush=True)
+    n = 8
+    d, model, nucs = prepare(4, n_particles=n, n_batches=1)
+    o = oracle_run(d, threads=8)
+    h_o = o.hashes(n)
+    r_o = o.results()
+    c = o.counters()
+    for learned in (None, (256, 200)):
+        g = gpu_run(d, options={"trace_hash": 1, "bank_cap": 64}, learned=learned)
+        assert np.array_equal(g["hashes"], h_o), "%d of %d histories differ (learned %s)" % (
+            np.sum(g["hashes"] != h_o), n, learned)
+        assert_tallies_close(g["results"], r_o)
+        assert g["stats"]["collisions"] == c["collisions"] and g["stats"]["segments"] == c["segments"]
+        del g

@@ -115,6 +115,37 @@ def test_problem_set_matches_problems_alone(small_set, options):
 
 
 @pytest.mark.gpu
+def test_problem_set_64_problems_full_structure():
+    """The 64-problem set of the batched workload as it is (lattices 3x3 to
+    17x17, 10-60 of the 60 nuclides with 20k-100k point grids, 192 materials,
+    meshes 10^3 to 64^3), with 30-120 particles a problem instead of 1e4-1e6:
+    the same draws of every other property.  The calculate-xs sort then fills
+    the largest key histogram (192 materials x binades x 2^bits buckets up to
+    32768 keys).  Every problem's histories equal the oracle's and its tally
+    agrees within 1e-12 per bin, on binary search and on the learned index."""
+    from openmc_amd import configs, lib
+    from oracle.problem import write_oracle_problem
+    print("case: problem set of 64, 30-120 particles each", flush=True)
+    d, dirs, problems, nucs = configs.problem_set_dir(n_problems=64, n_batches=1, particles=(30, 120))
+    head, _ = lib.check_problem_set(dirs)
+    assert head["n_materials"] == 192
+    binades = int(np.float64(head["energy_max"]).view(np.uint64) >> np.uint64(52)) - int(
+        np.float64(head["energy_min"]).view(np.uint64) >> np.uint64(52)) + 1
+    want = []
+    for k, (model, used) in enumerate(problems):
+        write_oracle_problem(model, used, dirs[k], data_dir=os.path.join("..", "data"))
+        o = oracle_run(dirs[k], threads=8)
+        want.append((o.hashes(model.settings.particles), o.results()))
+    for learned in (False, True):
+        got, st = _set_run(dirs, {"trace_hash": 1}, learned=learned)
+        bits = int(st["sort_bits"])
+        assert 1 + 192 * (binades << bits) <= 32768 < 1 + 192 * (binades << (bits + 1))
+        for k in range(len(dirs)):
+            assert np.array_equal(got[k]["hashes"], want[k][0]), "problem %d differs (learned %s)" % (k, learned)
+            assert_tallies_close(got[k]["results"], want[k][1])
+
+
+@pytest.mark.gpu
 def test_problem_set_learned_index_same_histories(small_set):
     """The learned index, built over a problem set's shared nuclide table,
     gives every problem the histories and tallies of binary search, with
